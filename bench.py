@@ -1,0 +1,244 @@
+"""Benchmark: SO4 2J=8 energy+forces throughput (atom-environments/s), the
+adjoint path against the plain materialised-dxdr path on the same GPU, FP64.
+
+Workload: diamond-cubic Si 8x8x8 supercell (4096 atoms), N(0, 0.08 A)
+seeded displacements, rc = 4.9 A, SNAP 2J = 8, w_Si = 1.0, rfac0 = 0.99363,
+cosine cutoff; energy from a seeded per-element MLP 55-30-30-1 (tanh, FP64).
+One step = neighbour list + descriptors + network energy + forces for the
+whole structure; inputs (positions) are resident in HBM when timing starts.
+
+Prints one JSON line (see the keys below).  ``--gpus N`` with N > 1 exits:
+this build measures one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom-envs/s SO4 2J=8 energy+forces: adjoint vs plain dxdr path, same GPU, FP64"
+PEAK_FP64 = 78.6e12
+
+
+# ------------------------------------------------------ algorithmic counts
+def so4_counts(twol):
+    """Loop-bound counts of the reference algorithm (SNAP.py:1050-1257)."""
+    zterms = zpairs = 0
+    nb = 0
+    for l1 in range(twol + 1):
+        for l2 in range(l1 + 1):
+            for l in range(l1 - l2, min(twol, l1 + l2) + 1, 2):
+                if l >= l1:
+                    nb += 1
+                mb = 0
+                while 2 * mb <= l:
+                    for ma in range(l + 1):
+                        ma1min = max(0, (2 * ma - l - l2 + l1) // 2)
+                        na = min(l1, (2 * ma - l + l2 + l1) // 2) - ma1min + 1
+                        mb1min = max(0, (2 * mb - l - l2 + l1) // 2)
+                        nbb = min(l1, (2 * mb - l + l2 + l1) // 2) - mb1min + 1
+                        zterms += na * nbb
+                        zpairs += nbb
+                    mb += 1
+    half = [(l // 2 + 1) * (l + 1) for l in range(twol + 1)]
+    return dict(zterms=zterms, zpairs=zpairs, half=half, nhalf=sum(half), nb=nb)
+
+
+def flops(twol, k_mean):
+    """Algorithmic FP64 flops per atom-environment (conventions: complex
+    multiply 6, real x complex 2, complex add 2).
+      Z: 10 per inner term (cg * u1 * u2 accumulated) + 4 per outer term;
+      U per neighbour: 18 per half element (two scaled complex products);
+      dU per neighbour: 3 x 34 per half element; adjoint contraction 16 per
+      half element; plain dB: 3 x 4 per (Z element, role)."""
+    c = so4_counts(twol)
+    z = 10 * c["zterms"] + 4 * c["zpairs"]
+    hu = c["nhalf"] - 1
+    u = 18 * hu
+    du = 102 * hu
+    adj_contract = 16 * c["nhalf"]
+    return dict(z_per_atom=z, u_per_nbr=u, du_per_nbr=du + u, adj_contract_per_nbr=adj_contract,
+                adjoint_per_env=z + k_mean * (u + u + du + adj_contract),
+                counts=c)
+
+
+# ------------------------------------------------------------------ GPU
+def build_case(reps=8, sigma=0.08, seed=0):
+    from pyxtal_ff_amd import workloads
+    return workloads.diamond_si(reps=reps, sigma=sigma, seed=seed)
+
+
+class Timers:
+    """HIP-event timing of each library call (torch's current stream is the
+    stream every pxff call is launched on)."""
+
+    def __init__(self):
+        self.acc = {}
+
+    def wrap(self, mod, names):
+        import torch
+        for n in names:
+            f = getattr(mod, n)
+
+            def g(*a, _f=f, _n=n, **k):
+                s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                s.record()
+                out = _f(*a, **k)
+                e.record()
+                self.acc.setdefault(_n, []).append((s, e))
+                return out
+            setattr(mod, n, g)
+            setattr(self, "_orig_" + n, f)
+
+    def unwrap(self, mod, names):
+        for n in names:
+            setattr(mod, n, getattr(self, "_orig_" + n))
+
+    def summary(self):
+        import torch
+        torch.cuda.synchronize()
+        return {k: float(np.mean([s.elapsed_time(e) for s, e in v])) for k, v in self.acc.items()}
+
+
+def run_gpu(steps, warmup):
+    import torch
+    from pyxtal_ff_amd import adjoint, potential
+    from pyxtal_ff_amd import so4 as dso4
+    from pyxtal_ff_amd.neighbors import DeviceNeighborList
+
+    s = build_case()
+    n = len(s)
+    twol, rc = 8, 4.9
+    cfg = dso4.So4Config(twol, rc, 0.99363, "cosine")
+    dev = torch.device("cuda")
+    pos_dev = torch.as_tensor(s.positions, dtype=torch.float64, device=dev)
+    w = torch.ones(n, dtype=torch.float64, device=dev)
+    model = potential.NNPotential(["Si"], cfg.nb, (30, 30), seed=0).to(dev)
+    species = potential.species_index(s.symbols, ["Si"], dev)
+
+    def nlist():
+        # positions stay resident in HBM; cell and pbc are host metadata
+        return DeviceNeighborList(None, [s.cell], [s.pbc], rc, pos_device=pos_dev)
+
+    def step_adjoint():
+        nl = nlist()
+        st = adjoint.SO4State(nl, cfg, w, w)
+        out = potential.energy_forces_adjoint(nl, st, model, species)
+        return out
+
+    def step_plain():
+        nl = nlist()
+        return potential.energy_forces_plain(nl, cfg, w, w, model, species)
+
+    def timed(fn, k, wu):
+        for _ in range(wu):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k, out
+
+    t_adj, out_a = timed(step_adjoint, steps, warmup)
+    t_pl, out_p = timed(step_plain, max(1, steps), max(1, warmup))
+    ferr = float(torch.linalg.norm(out_a["forces"] - out_p["forces"]) / torch.linalg.norm(out_p["forces"]))
+
+    # per-kernel timing (separate instrumented steps)
+    names = ["ui", "zb", "yi", "deidrj", "dbdr", "rows", "dxdr_contract"]
+    tm = Timers()
+    tm.wrap(dso4, names)
+    for _ in range(3):
+        step_adjoint()
+        step_plain()
+    ktimes = tm.summary()
+    tm.unwrap(dso4, names)
+    nl = nlist()
+    tnl = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nlist()
+        torch.cuda.synchronize()
+        tnl.append(time.perf_counter() - t0)
+    kmean = nl.nimg / n
+    return dict(n=n, t_adj=t_adj, t_plain=t_pl, ferr=ferr, ktimes=ktimes, t_nlist=float(np.median(tnl)),
+                nimg=nl.nimg, nrows=nl.nrows, kmean=kmean)
+
+
+# ------------------------------------------------------------ CPU baseline
+def run_cpu_baseline(reps_sample=2):
+    """The oracle (C restatement of the reference's numba kernels: two passes,
+    U recomputed per pair, materialised dB/dr) on one host core, on the same
+    4096-atom structure, plus the NumPy force contraction."""
+    import oracle
+    oracle.build()
+    s = build_case()
+    ts = []
+    for _ in range(reps_sample):
+        t0 = time.perf_counter()
+        d = oracle.snap_calculate(s.positions, s.cell, s.pbc, s.symbols, {"Si": 1.0}, lmax=4, rcut=4.9,
+                                  derivative=True, stress=False)
+        beta = np.ones_like(d["x"])
+        f = np.zeros((len(s), 3))
+        np.add.at(f, d["seq"][:, 1], -np.einsum("rb,rbk->rk", beta[d["seq"][:, 0]], d["dxdr"]))
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    return dict(value=len(s) / t, unit="atom-env/s", cores=1, kind="port",
+                sample="full 4096-atom diamond Si 2J=8 structure, descriptor+dxdr+force contraction, "
+                       "median of %d runs (%.1f s each), 1 thread" % (reps_sample, t))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only (every box in this build has one GPU); "
+              "multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    r = run_gpu(args.steps, args.warmup)
+    n = r["n"]
+    adj_rate = n / r["t_adj"]
+    plain_rate = n / r["t_plain"]
+    fl = flops(8, r["kmean"])
+    kt = r["ktimes"]
+    # dominant kernel of the adjoint step
+    adj_k = {k: kt[k] for k in ("ui", "zb", "yi", "deidrj") if k in kt}
+    dom = max(adj_k, key=adj_k.get)
+    alg = {"zb": fl["z_per_atom"] * n,
+           "deidrj": (fl["du_per_nbr"] + fl["adj_contract_per_nbr"]) * r["nimg"],
+           "ui": fl["u_per_nbr"] * r["nimg"],
+           "yi": 8 * so4_counts(8)["nhalf"] * n}[dom]
+    achieved = alg / (kt[dom] * 1e-3)
+    cpu = None if args.no_cpu else run_cpu_baseline()
+    line = {
+        "metric": METRIC, "value": adj_rate, "unit": "atom-env/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["t_adj"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "diamond Si 8x8x8 (4096 atoms), sigma 0.08 A, SNAP 2J=8, rc 4.9, E+F",
+                   "natoms": n, "images": r["nimg"], "seq_rows": r["nrows"], "path": "adjoint",
+                   "plain_dxdr_atom_env_per_s": plain_rate, "plain_ms_per_step": r["t_plain"] * 1e3,
+                   "adjoint_over_plain": adj_rate / plain_rate, "force_rel_diff_adjoint_vs_plain": r["ferr"],
+                   "nlist_ms": r["t_nlist"] * 1e3, "kernel_ms": kt},
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

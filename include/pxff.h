@@ -1,0 +1,176 @@
+/*
+ * pxff.h -- C ABI of the MI355X (gfx950) descriptor hot path of PyXtal_FF.
+ *
+ * Every pointer argument named d_* is a caller-owned DEVICE pointer; every
+ * function enqueues work on the given HIP stream (`stream` is a hipStream_t
+ * passed as void*) and returns 0 on success or a negative status, with a
+ * message available from pxff_last_error() (thread-local).  No function
+ * allocates device memory except the per-2J constant tables, which are
+ * built once per (device, 2J) under a mutex and kept until process exit.
+ * Arrays are row-major; floating point is FP64; complex arrays are
+ * interleaved (re, im) doubles.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   pyxtal_ff/descriptors/SNAP.py:240-317   build_neighbor_list (ASE NeighborList + regrouping)
+ *   pyxtal_ff/descriptors/SNAP.py:1289-1762 get_bispectrum_components (numba)
+ *   pyxtal_ff/descriptors/SO4.py:1256-1730  get_bispectrum_components (numba, Z weights)
+ *   pyxtal_ff/descriptors/SO3.py:187-291,504-694 SO3.calculate / compute_cs / compute_dcs
+ *   pyxtal_ff/utilities/__init__.py:226-235 per-atom rdxdr reduction
+ *   pyxtal_ff/models/neuralnetwork.py:484-497,657-677 force/stress contraction over dxdr/rdxdr
+ */
+#ifndef PXFF_H
+#define PXFF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PXFF_OK 0
+#define PXFF_EINVAL -1   /* bad argument */
+#define PXFF_EHIP -2     /* HIP runtime error */
+#define PXFF_ENOTIMPL -3 /* unsupported parameter (e.g. odd 2J) */
+
+/* Copy the calling thread's last error message into buf (NUL-terminated). */
+int pxff_last_error(char *buf, size_t len);
+
+/* Library version string. */
+const char *pxff_version(void);
+
+/* ------------------------------------------------------------------ *
+ * Neighbour list (replaces SNAP.py:240-317 / SO3.py:317-376 and the   *
+ * ASE NeighborList call at SNAP.py:250-252).                          *
+ *                                                                     *
+ * A batch of nstruct structures; atoms of structure s are             *
+ * [atom_start[s], atom_start[s+1]).  Per structure: cell [9] (rows =  *
+ * lattice vectors), pbc [3], and a host-computed bin geometry         *
+ * geom [12] int32 = {nb0, nb1, nb2, S0, S1, S2, bin_offset, 0...}.     *
+ * Images (i, j, T) with |pos_j + T.cell - pos_i| < rc, excluding       *
+ * (i, T=0), are produced in the order (i, j, T0, T1, T2).             *
+ * ------------------------------------------------------------------ */
+
+/* Phase 1: bin atoms and count images per centre.
+ *   d_bin_count [nbins_total] and d_bin_start [nbins_total+1] are int32
+ *   workspace; d_atom_bin [natoms] int32, d_wrap [natoms*3] int32 and
+ *   d_bin_atoms [natoms] int32 are workspace; d_img_count [natoms] int64 out. */
+int pxff_nlist_count(const double *d_pos, int64_t natoms, const int64_t *d_atom_struct,
+                     const int64_t *d_atom_start, const double *d_cells, const double *d_inv_cells,
+                     const int32_t *d_pbc, const int32_t *d_geom, int64_t nbins_total, double rc,
+                     int32_t *d_bin_count, int32_t *d_bin_start, int32_t *d_atom_bin, int32_t *d_wrap,
+                     int32_t *d_bin_atoms, int64_t *d_img_count, void *stream);
+
+/* Phase 2: fill images (sorted per centre by (j, T)) and count seq rows.
+ *   d_img_start [natoms+1] int64 = exclusive scan of d_img_count.
+ *   Outputs: d_img_atom [nimg] int64 (centre), d_img_j [nimg] int64,
+ *   d_img_T [nimg*3] int32, d_img_r [nimg*3] double, d_row_count [natoms]
+ *   int64 (unique j plus the (i,i) row). */
+int pxff_nlist_fill(const double *d_pos, int64_t natoms, const int64_t *d_atom_struct,
+                    const int64_t *d_atom_start, const double *d_cells, const int32_t *d_pbc,
+                    const int32_t *d_geom, double rc, const int32_t *d_bin_start, const int32_t *d_atom_bin,
+                    const int32_t *d_wrap, const int32_t *d_bin_atoms, const int64_t *d_img_start,
+                    int64_t *d_img_atom, int64_t *d_img_j, int32_t *d_img_T, double *d_img_r,
+                    int64_t *d_row_count, void *stream);
+
+/* Phase 3: seq rows.  d_row_start [natoms+1] int64 = exclusive scan of
+ * d_row_count, nrows = d_row_start[natoms].  Outputs: d_seq [nrows*2] int64
+ * (the reference's seq), d_row_img [nrows+1] int64 (images of row r are
+ * [row_img[r], row_img[r+1])), d_img_row [nimg] int64, d_nsite [natoms]
+ * int64 (row of (i,i)), d_row_rev [nrows] int64 (row (j,i) of row (i,j)),
+ * d_img_rev [nimg] int64 (image (j,i,-T) of image (i,j,T)).  A missing
+ * reverse is reported through d_flag[0] != 0 (int32, zeroed by the call). */
+int pxff_nlist_rows(int64_t natoms, int64_t nimg, int64_t nrows, const int64_t *d_img_start,
+                    const int64_t *d_img_atom, const int64_t *d_img_j, const int32_t *d_img_T,
+                    const int64_t *d_row_start, int64_t *d_seq, int64_t *d_row_img, int64_t *d_img_row,
+                    int64_t *d_nsite, int64_t *d_row_rev, int64_t *d_img_rev, int32_t *d_flag, void *stream);
+
+/* Exclusive scan: d_out [n+1], d_out[n] = total. */
+int pxff_scan_i64(const int64_t *d_in, int64_t n, int64_t *d_out, void *stream);
+
+/* ------------------------------------------------------------------ *
+ * SO(4) / SNAP bispectrum (replaces get_bispectrum_components,         *
+ * SNAP.py:1289-1762; SO4.py:1256-1730 is the same with rfac0 = 1 and   *
+ * Z weights).                                                          *
+ * Layout: per atom a "half" U/Y array of nhalf complex (levels l, rows *
+ * mb <= l/2, columns ma), Z [nz] complex in the reference's idxz order, *
+ * B [nb] real in the reference's idxb order.                            *
+ * ------------------------------------------------------------------ */
+typedef struct {
+    int twol;      /* 2J = round(2*lmax); even, 0..10 */
+    double rcut;   /* cutoff radius */
+    double rfac0;  /* psi = rfac0*pi*r/rcut (1.0 for SO4.py) */
+    int cutoff_id; /* 1 cosine, 2 tanh, 3..6 poly1..4, 7 exp (SNAP.py:141-170) */
+    int normalize_u; /* u_norm = 4pi/sqrt(l+1) (SNAP.py:1333-1341) */
+} pxff_so4_params;
+
+/* Table sizes for 2J: nu (full U), nhalf (half U), nz, nb. */
+int pxff_so4_sizes(int twol, int *nu, int *nhalf, int *nz, int *nb);
+
+/* U_tot (half layout, self term and u_norm included) per centre.
+ *   d_img_start [natoms+1], d_img_j [nimg], d_img_r [nimg*3]; d_atom_w
+ *   [natoms] = neighbour weight of each atom (pair weight of image = w[j]);
+ *   d_site_w [natoms] = self weight.  d_work_order [natoms] int64 or NULL:
+ *   centres in the order they are scheduled (neighbour-count sorted). */
+int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_j,
+                const double *d_img_r, const double *d_atom_w, const double *d_site_w,
+                const int64_t *d_work_order, double *d_utot, void *stream);
+
+/* Z (and B) from U_tot.  d_z [natoms*nz*2] out; d_b [natoms*nb] out (or NULL). */
+int pxff_so4_zb(const pxff_so4_params *p, int64_t natoms, const double *d_utot, double *d_z, double *d_b,
+                void *stream);
+
+/* Adjoint, step 1: Y [natoms*nhalf*2] = sum over blocks of the dE/dB-weighted
+ * Z (half weights, factor 2 and u_norm folded in).  d_beta [natoms*nb]. */
+int pxff_so4_yi(const pxff_so4_params *p, int64_t natoms, const double *d_z, const double *d_beta,
+                double *d_y, void *stream);
+
+/* Adjoint, step 2: dE/dr per image [nimg*3] from Y (one dU sweep; dB/dR is
+ * never materialised).  d_img_atom [nimg] = centre of each image. */
+int pxff_so4_deidrj(const pxff_so4_params *p, int64_t nimg, const int64_t *d_img_atom, const int64_t *d_img_j,
+                    const double *d_img_r, const double *d_atom_w, const double *d_y, double *d_dedr,
+                    void *stream);
+
+/* Forces and virial from dE/dr per image (deterministic gather via the
+ * reverse-image index).  d_grad [natoms*3] = dE/dR_i (= -F_i);
+ * d_virial [natoms*9] (or NULL) = per-centre sum_e r_e (x) dE/dr_e. */
+int pxff_so4_gather(int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_rev, const double *d_img_r,
+                    const double *d_dedr, double *d_grad, double *d_virial, void *stream);
+
+/* Forward-mode (tangent) of B along per-image displacements dr_e
+ * [nimg*3]: dB [natoms*nb] = sum_e dB/dr_e . dr_e.  Used by the double
+ * backward of the adjoint (force-matched training).  d_dutot is workspace
+ * [natoms*nhalf*2]. */
+int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_j,
+                     const double *d_img_r, const double *d_atom_w, const double *d_dr, const double *d_z,
+                     double *d_dutot, double *d_db, void *stream);
+
+/* Plain (materialised) path, step 1: dB/dr per image [nimg*nb*3]. */
+int pxff_so4_dbdr(const pxff_so4_params *p, int64_t nimg, const int64_t *d_img_atom, const int64_t *d_img_j,
+                  const double *d_img_r, const double *d_atom_w, const double *d_z, double *d_dbdr, void *stream);
+
+/* Plain path, step 2: reference layout rows.  dxdr [nrows*nb*3] (rows (i,j)
+ * = sum over images, row (i,i) = -sum of the others), and, if d_rdxdr is not
+ * NULL, per-row -bstress/V [nrows*nb*9] with bstress[nsite] += Ri (x) dB,
+ * bstress[row] -= Rj (x) dB (SNAP.py:1458-1468).  d_inv_vol [natoms] = 1/V
+ * of each atom's structure. */
+int pxff_so4_rows(int64_t natoms, int nb, const int64_t *d_row_start, const int64_t *d_row_img,
+                  const int64_t *d_seq, const int64_t *d_nsite, const double *d_pos, const double *d_img_r,
+                  const double *d_dbdr, const double *d_inv_vol, double *d_dxdr, double *d_rdxdr, void *stream);
+
+/* Plain path, force contraction (neuralnetwork.py:484-497, 657-677):
+ * dE/dR_m = sum_{rows (i,m)} sum_b beta[i,b] dxdr[row,b,:], gathered per m
+ * through the reverse-row index.  d_grad [natoms*3]. */
+int pxff_dxdr_contract(int64_t natoms, int nb, const int64_t *d_row_start, const int64_t *d_seq,
+                       const int64_t *d_row_rev, const double *d_beta, const double *d_dxdr, double *d_grad,
+                       void *stream);
+
+/* Per-atom rdxdr reduction (utilities/__init__.py:226-235): out [natoms*nb*6]
+ * Voigt (xx, yy, zz, xy, xz, yz) of the sum of the centre's rows. */
+int pxff_rdxdr_reduce(int64_t natoms, int nb, const int64_t *d_row_start, const double *d_rdxdr, double *d_out,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

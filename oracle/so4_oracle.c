@@ -1,0 +1,658 @@
+/*
+ * CPU oracle for the SO(4)/SNAP bispectrum hot path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This file is a statement-level restatement, in C99 with complex double, of
+ * the reference's numba kernels.  It is used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker;
+ * nothing in the pyxtal_ff_amd package links or calls it.
+ *
+ * Reference functions followed (paths relative to the reference checkout):
+ *   factorial table          pyxtal_ff/descriptors/angular_momentum.py:9-182
+ *   deltacg                  pyxtal_ff/descriptors/angular_momentum.py:184-190
+ *   cutoff functions         pyxtal_ff/descriptors/SNAP.py:336-440
+ *   init_clebsch_gordan      pyxtal_ff/descriptors/SNAP.py:442-510
+ *   compute_sfac/dsfac       pyxtal_ff/descriptors/SNAP.py:512-588
+ *   addself_uarraytot        pyxtal_ff/descriptors/SNAP.py:590-627
+ *   init_rootpqarray         pyxtal_ff/descriptors/SNAP.py:861-868
+ *   compute_uarray_recursive pyxtal_ff/descriptors/SNAP.py:870-956
+ *   compute_duarray_recursive pyxtal_ff/descriptors/SNAP.py:958-1017
+ *   add_uarraytot            pyxtal_ff/descriptors/SNAP.py:1019-1029
+ *   dudr                     pyxtal_ff/descriptors/SNAP.py:1031-1048
+ *   compute_zi               pyxtal_ff/descriptors/SNAP.py:1050-1098
+ *   compute_bi               pyxtal_ff/descriptors/SNAP.py:1101-1141
+ *   compute_dbidrj           pyxtal_ff/descriptors/SNAP.py:1143-1257
+ *   get_bispectrum_components pyxtal_ff/descriptors/SNAP.py:1289-1762
+ *     (SO4.py:1256-1730 is the same code with psi = pi*r/rc, i.e. rfac0 = 1)
+ *
+ * Semantics kept from the reference (they matter for parity):
+ *   - the factorial table jumps from 16! to 170! at index 17, so CG values are
+ *     exact only for 2J <= 10 (the reference's own table);
+ *   - the pair loop is grouped by centre; U is recomputed per pair in the
+ *     derivative pass; dB/dr is materialised per neighbour image;
+ *   - self images (I == J) feed B and bstress but not dblist;
+ *   - the pair weight is taken from image 0 of the pair; images with
+ *     r < 1e-8 (padding) are skipped;
+ *   - bstress[nsite] += Ri (x) dB, bstress[N] -= Rj (x) dB with Rj = Ri + r_ij.
+ * Outputs are the real parts (the reference returns blist.real etc.).
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double complex cplx;
+
+/* angular_momentum.py:9-182 -- note the jump after 16! (index 17 = 170!). */
+static const double FAC_TABLE[] = {
+    1, 1, 2, 6, 24, 120, 720, 5040, 40320, 362880, 3628800, 39916800,
+    479001600, 6227020800.0, 87178291200.0, 1307674368000.0, 20922789888000.0,
+    7.59070505394721e+286, 1.22969421873945e+289, 2.0044015765453e+291,
+    3.28721858553429e+293, 5.42391066613159e+295, 9.00369170577843e+297,
+    1.503616514865e+300};
+#define FAC_TABLE_LEN ((int)(sizeof(FAC_TABLE) / sizeof(FAC_TABLE[0])))
+
+static double factorial(long n) {
+    if (n < 0 || n >= FAC_TABLE_LEN) return NAN;
+    return FAC_TABLE[n];
+}
+
+static long floordiv(long a, long b) {
+    long q = a / b;
+    if ((a % b != 0) && ((a < 0) != (b < 0))) q -= 1;
+    return q;
+}
+
+/* angular_momentum.py:184-190 */
+static double deltacg(long l1, long l2, long l) {
+    double sfaccg = factorial(floordiv(l1 + l2 + l, 2) + 1);
+    return sqrt(factorial(floordiv(l1 + l2 - l, 2)) * factorial(floordiv(l1 - l2 + l, 2)) *
+                factorial(floordiv(-l1 + l2 + l, 2)) / sfaccg);
+}
+
+/* ---------------------------------------------------------------- cutoffs */
+/* SNAP.py:336-440; ids follow the cutoff_function setter (SNAP.py:141-170). */
+static double cut_f(double r, double rc, int id) {
+    double x;
+    if (r > rc) return 0.0; /* compute_sfac, SNAP.py:536-537 */
+    switch (id) {
+    case 1: return 0.5 * (cos(M_PI * r / rc) + 1.0);
+    case 2: { double t = tanh(1.0 - r / rc); return t * t * t; }
+    case 3: x = r / rc; return x * x * (2 * x - 3) + 1;
+    case 4: x = r / rc; return x * x * x * (x * (15 - 6 * x) - 10) + 1;
+    case 5: x = r / rc; return x * x * x * x * (x * (x * (20 * x - 70) + 84) - 35) + 1;
+    case 6: x = r / rc; return x * x * x * x * x * (x * (x * (x * (315 - 70 * x) - 540) + 420) - 126) + 1;
+    case 7: x = r / rc; return exp(1.0 - 1.0 / (1.0 - x * x));
+    default: return NAN;
+    }
+}
+
+static double cut_df(double r, double rc, int id) {
+    double x;
+    if (r > rc) return 0.0; /* compute_dsfac, SNAP.py:567-568 */
+    switch (id) {
+    case 1: return -0.5 * M_PI / rc * sin(M_PI * r / rc);
+    case 2: { double t = tanh(1.0 - r / rc); double t2 = t * t; return -(3.0 / rc) * t2 * (1.0 - t2); }
+    case 3: return (6.0 / (rc * rc)) * r * (r / rc - 1.0);
+    case 4: x = r / rc; return (-30.0 / rc) * (x * x * (x - 1) * (x - 1));
+    case 5: x = r / rc; return (140.0 / rc) * (x * x * x * (x - 1) * (x - 1) * (x - 1));
+    case 6: x = r / rc; { double y = x * (x - 1); return (-630.0 / rc) * y * y * y * y; }
+    case 7: x = r / rc; { double d = 1 + x * x; return 2 * x * exp(1.0 - 1.0 / (1.0 - x * x)) / (d * d); }
+    default: return NAN;
+    }
+}
+
+/* ------------------------------------------------------------- the tables */
+typedef struct {
+    int twol, ldim;
+    int ncg, nu, nz, nb;
+    int *idxcg_block; /* [ldim^3] */
+    double *cglist;   /* [ncg] */
+    int *idxu_block;  /* [ldim] */
+    double *u_norm;   /* [nu] */
+    int *idxb;        /* [nb*3] */
+    int *idxz;        /* [nz*10] */
+    int *idxz_block;  /* [ldim^3] */
+    double *rootpq;   /* [ldim*ldim] */
+} so4_tables;
+
+#define IDX3(t, a, b, c) ((((a) * (t)->ldim) + (b)) * (t)->ldim + (c))
+
+/* SNAP.py:442-510 (Varshalovich 8.2.1(3)). */
+static void init_clebsch_gordan(int twol, double *cglist) {
+    int ldim = twol + 1, cnt = 0;
+    for (int l1 = 0; l1 < ldim; l1++)
+        for (int l2 = 0; l2 <= l1; l2++)
+            for (int l = l1 - l2; l <= (twol < l1 + l2 ? twol : l1 + l2); l += 2)
+                for (int m1 = 0; m1 <= l1; m1++) {
+                    int aa2 = 2 * m1 - l1;
+                    for (int m2 = 0; m2 <= l2; m2++) {
+                        int bb2 = 2 * m2 - l2;
+                        int m2x = aa2 + bb2 + l; /* m = m2x / 2 (always even) */
+                        if (m2x < 0 || m2x > 2 * l) {
+                            cglist[cnt++] = 0.0;
+                            continue;
+                        }
+                        long m = m2x / 2;
+                        double sum = 0.0;
+                        long zlo = floordiv(-(l - l2 + aa2), 2), z2 = floordiv(-(l - l1 - bb2), 2);
+                        if (z2 > zlo) zlo = z2;
+                        if (zlo < 0) zlo = 0;
+                        long zhi = floordiv(l1 + l2 - l, 2), z3 = floordiv(l1 - aa2, 2), z4 = floordiv(l2 + bb2, 2);
+                        if (z3 < zhi) zhi = z3;
+                        if (z4 < zhi) zhi = z4;
+                        for (long z = zlo; z <= zhi; z++) {
+                            double ifac = (z % 2 == 0) ? 1.0 : -1.0;
+                            sum += ifac / (factorial(z) * factorial(floordiv(l1 + l2 - l, 2) - z) *
+                                           factorial(floordiv(l1 - aa2, 2) - z) *
+                                           factorial(floordiv(l2 + bb2, 2) - z) *
+                                           factorial(floordiv(l - l2 + aa2, 2) + z) *
+                                           factorial(floordiv(l - l1 - bb2, 2) + z));
+                        }
+                        long cc2 = 2 * m - l;
+                        double dcg = deltacg(l1, l2, l);
+                        double sfaccg = sqrt(factorial(floordiv(l1 + aa2, 2)) * factorial(floordiv(l1 - aa2, 2)) *
+                                             factorial(floordiv(l2 + bb2, 2)) * factorial(floordiv(l2 - bb2, 2)) *
+                                             factorial(floordiv(l + cc2, 2)) * factorial(floordiv(l - cc2, 2)) *
+                                             (double)(l + 1));
+                        cglist[cnt++] = sum * dcg * sfaccg;
+                    }
+                }
+}
+
+/* Index tables, SNAP.py:1296-1416. */
+static int build_tables(int twol, int norm, so4_tables *t) {
+    memset(t, 0, sizeof(*t));
+    int ldim = twol + 1;
+    t->twol = twol;
+    t->ldim = ldim;
+    t->idxcg_block = calloc((size_t)ldim * ldim * ldim, sizeof(int));
+    t->idxz_block = calloc((size_t)ldim * ldim * ldim, sizeof(int));
+    t->idxu_block = calloc(ldim, sizeof(int));
+    int cnt = 0;
+    for (int l1 = 0; l1 < ldim; l1++)
+        for (int l2 = 0; l2 <= l1; l2++)
+            for (int l = l1 - l2; l <= (twol < l1 + l2 ? twol : l1 + l2); l += 2) {
+                t->idxcg_block[IDX3(t, l1, l2, l)] = cnt;
+                cnt += (l1 + 1) * (l2 + 1);
+            }
+    t->ncg = cnt;
+    t->cglist = calloc(cnt > 0 ? cnt : 1, sizeof(double));
+    init_clebsch_gordan(twol, t->cglist);
+
+    cnt = 0;
+    for (int l = 0; l < ldim; l++) {
+        t->idxu_block[l] = cnt;
+        cnt += (l + 1) * (l + 1);
+    }
+    t->nu = cnt;
+    t->u_norm = malloc(sizeof(double) * cnt);
+    {
+        int c = 0;
+        for (int l = 0; l < ldim; l++)
+            for (int k = 0; k < (l + 1) * (l + 1); k++)
+                t->u_norm[c++] = norm ? 4.0 * M_PI / sqrt((double)(l + 1)) : 1.0;
+    }
+
+    cnt = 0;
+    for (int l1 = 0; l1 < ldim; l1++)
+        for (int l2 = 0; l2 <= l1; l2++)
+            for (int l = l1 - l2; l <= (twol < l1 + l2 ? twol : l1 + l2); l += 2)
+                if (l >= l1) cnt++;
+    t->nb = cnt;
+    t->idxb = malloc(sizeof(int) * 3 * (cnt > 0 ? cnt : 1));
+    cnt = 0;
+    for (int l1 = 0; l1 < ldim; l1++)
+        for (int l2 = 0; l2 <= l1; l2++)
+            for (int l = l1 - l2; l <= (twol < l1 + l2 ? twol : l1 + l2); l += 2)
+                if (l >= l1) {
+                    t->idxb[3 * cnt + 0] = l1;
+                    t->idxb[3 * cnt + 1] = l2;
+                    t->idxb[3 * cnt + 2] = l;
+                    cnt++;
+                }
+
+    cnt = 0;
+    for (int l1 = 0; l1 < ldim; l1++)
+        for (int l2 = 0; l2 <= l1; l2++)
+            for (int l = l1 - l2; l <= (twol < l1 + l2 ? twol : l1 + l2); l += 2)
+                for (int mb = 0; 2 * mb <= l; mb++) cnt += l + 1;
+    t->nz = cnt;
+    t->idxz = malloc(sizeof(int) * 10 * (size_t)cnt);
+    cnt = 0;
+    for (int l1 = 0; l1 < ldim; l1++)
+        for (int l2 = 0; l2 <= l1; l2++)
+            for (int l = l1 - l2; l <= (twol < l1 + l2 ? twol : l1 + l2); l += 2) {
+                t->idxz_block[IDX3(t, l1, l2, l)] = cnt;
+                for (int mb = 0; 2 * mb <= l; mb++)
+                    for (int ma = 0; ma <= l; ma++) {
+                        int *z = t->idxz + 10 * (size_t)cnt;
+                        z[0] = l1;
+                        z[1] = l2;
+                        z[2] = l;
+                        int ma1min = (2 * ma - l - l2 + l1) / 2;
+                        if (ma1min < 0) ma1min = 0;
+                        z[3] = ma1min;
+                        z[4] = (2 * ma - l - (2 * ma1min - l1) + l2) / 2;
+                        int t1 = (2 * ma - l + l2 + l1) / 2;
+                        z[5] = (l1 < t1 ? l1 : t1) - ma1min + 1;
+                        int mb1min = (2 * mb - l - l2 + l1) / 2;
+                        if (mb1min < 0) mb1min = 0;
+                        z[6] = mb1min;
+                        z[7] = (2 * mb - l - (2 * mb1min - l1) + l2) / 2;
+                        int t2 = (2 * mb - l + l2 + l1) / 2;
+                        z[8] = (l1 < t2 ? l1 : t2) - mb1min + 1;
+                        z[9] = t->idxu_block[l] + (l + 1) * mb + ma;
+                        cnt++;
+                    }
+            }
+
+    /* init_rootpqarray, SNAP.py:861-868 */
+    t->rootpq = calloc((size_t)ldim * ldim, sizeof(double));
+    for (int p = 1; p < ldim; p++)
+        for (int q = 1; q < ldim; q++) t->rootpq[p * ldim + q] = sqrt((double)p / (double)q);
+    return 0;
+}
+
+static void free_tables(so4_tables *t) {
+    free(t->idxcg_block);
+    free(t->cglist);
+    free(t->idxu_block);
+    free(t->u_norm);
+    free(t->idxb);
+    free(t->idxz);
+    free(t->idxz_block);
+    free(t->rootpq);
+}
+
+/* ------------------------------------------------------ Wigner-U arrays */
+/* addself_uarraytot, SNAP.py:590-627 */
+static void addself_uarraytot(const so4_tables *t, cplx *ulist, int cutoff_id, double rcut) {
+    double fcut = cut_f(0.0, rcut, cutoff_id);
+    for (int l = 0; l < t->ldim; l++) {
+        int llu = t->idxu_block[l];
+        for (int ma = 0; ma <= l; ma++) {
+            ulist[llu] = fcut;
+            llu += l + 2;
+        }
+    }
+}
+
+/* compute_uarray_recursive, SNAP.py:870-956.  ulist must be zeroed. */
+static void compute_uarray_recursive(const so4_tables *t, double x, double y, double z, double psi,
+                                     double r, cplx *ulist) {
+    const int ldim = t->ldim;
+    double cospsi = cos(psi), sinpsi = sin(psi);
+    cplx a = cospsi - I * (sinpsi / r * z);
+    cplx b = (sinpsi / r) * (y - x * I);
+    cplx ca = conj(a), cb = conj(b);
+    ulist[0] = 1.0;
+    for (int l = 1; l < ldim; l++) {
+        int llu = t->idxu_block[l], llup = t->idxu_block[l - 1];
+        for (int mb = 0; 2 * mb <= l; mb++) {
+            ulist[llu] = 0;
+            for (int ma = 0; ma < l; ma++) {
+                double rootpq = t->rootpq[(l - ma) * ldim + (l - mb)];
+                ulist[llu] += rootpq * ca * ulist[llup];
+                rootpq = t->rootpq[(ma + 1) * ldim + (l - mb)];
+                ulist[llu + 1] += -rootpq * cb * ulist[llup];
+                llu++;
+                llup++;
+            }
+            llu++;
+        }
+        llu = t->idxu_block[l];
+        llup = llu + (l + 1) * (l + 1) - 1;
+        int mbpar = 1;
+        for (int mb = 0; 2 * mb <= l; mb++) {
+            int mapar = mbpar;
+            for (int ma = 0; ma <= l; ma++) {
+                ulist[llup] = (mapar == 1) ? conj(ulist[llu]) : -conj(ulist[llu]);
+                mapar = -mapar;
+                llu++;
+                llup--;
+            }
+            mbpar = -mbpar;
+        }
+    }
+}
+
+/* compute_duarray_recursive, SNAP.py:958-1017.  dulist is [nu][3]. */
+static void compute_duarray_recursive(const so4_tables *t, double x, double y, double z, double psi,
+                                      double r, const cplx *ulist, cplx *dulist) {
+    const int ldim = t->ldim;
+    double cospsi = cos(psi), sinpsi = sin(psi);
+    double gr[3] = {x / r, y / r, z / r};
+    cplx a = cospsi - I * (sinpsi / r * z);
+    cplx b = (sinpsi / r) * (y - x * I);
+    cplx da[3], db[3];
+    for (int k = 0; k < 3; k++) {
+        da[k] = -sinpsi * psi / r * gr[k] - I * z * cospsi / r / r * psi * gr[k] + I * z * sinpsi / r / r * gr[k];
+        db[k] = cospsi / r / r * psi * (y - x * I) * gr[k] - sinpsi / r / r * (y - x * I) * gr[k];
+    }
+    da[2] += -I * sinpsi / r;
+    db[1] += sinpsi / r;
+    db[0] += -I * sinpsi / r;
+    cplx ca = conj(a), cb = conj(b);
+    for (int l = 1; l < ldim; l++) {
+        int llu = t->idxu_block[l], llup = t->idxu_block[l - 1];
+        for (int mb = 0; 2 * mb <= l; mb++) {
+            for (int k = 0; k < 3; k++) dulist[3 * llu + k] = 0;
+            for (int ma = 0; ma < l; ma++) {
+                double rootpq = t->rootpq[(l - ma) * ldim + (l - mb)];
+                for (int k = 0; k < 3; k++)
+                    dulist[3 * llu + k] += rootpq * (conj(da[k]) * ulist[llup] + ca * dulist[3 * llup + k]);
+                rootpq = t->rootpq[(ma + 1) * ldim + (l - mb)];
+                for (int k = 0; k < 3; k++)
+                    dulist[3 * (llu + 1) + k] = -rootpq * (conj(db[k]) * ulist[llup] + cb * dulist[3 * llup + k]);
+                llu++;
+                llup++;
+            }
+            llu++;
+        }
+        llu = t->idxu_block[l];
+        llup = llu + (l + 1) * (l + 1) - 1;
+        int mbpar = 1;
+        for (int mb = 0; 2 * mb <= l; mb++) {
+            int mapar = mbpar;
+            for (int ma = 0; ma <= l; ma++) {
+                for (int k = 0; k < 3; k++)
+                    dulist[3 * llup + k] = (mapar == 1) ? conj(dulist[3 * llu + k]) : -conj(dulist[3 * llu + k]);
+                mapar = -mapar;
+                llu++;
+                llup--;
+            }
+            mbpar = -mbpar;
+        }
+    }
+}
+
+/* dudr, SNAP.py:1031-1048 */
+static void dudr(const so4_tables *t, double x, double y, double z, double r, double rcut, const cplx *ulist,
+                 cplx *dulist, int cutoff_id) {
+    double sfac = cut_f(r, rcut, cutoff_id);
+    double dsfac = cut_df(r, rcut, cutoff_id);
+    double gr[3] = {x / r, y / r, z / r};
+    for (int u = 0; u < t->nu; u++)
+        for (int k = 0; k < 3; k++) {
+            dulist[3 * u + k] *= sfac;
+            dulist[3 * u + k] += ulist[u] * (dsfac * gr[k]);
+        }
+}
+
+/* compute_zi, SNAP.py:1050-1098 */
+static void compute_zi(const so4_tables *t, const cplx *ulisttot, cplx *zlist) {
+    for (int llz = 0; llz < t->nz; llz++) {
+        const int *zi = t->idxz + 10 * (size_t)llz;
+        int l1 = zi[0], l2 = zi[1], l = zi[2];
+        int ma1min = zi[3], ma2max = zi[4], na = zi[5];
+        int mb1min = zi[6], mb2max = zi[7], nb = zi[8];
+        const double *cgblock = t->cglist + t->idxcg_block[IDX3(t, l1, l2, l)];
+        zlist[llz] = 0;
+        int llu1 = t->idxu_block[l1] + (l1 + 1) * mb1min;
+        int llu2 = t->idxu_block[l2] + (l2 + 1) * mb2max;
+        int icgb = mb1min * (l2 + 1) + mb2max;
+        for (int ib = 0; ib < nb; ib++) {
+            cplx suma1 = 0;
+            const cplx *u1 = ulisttot + llu1, *u2 = ulisttot + llu2;
+            int ma1 = ma1min, ma2 = ma2max;
+            int icga = ma1min * (l2 + 1) + ma2max;
+            for (int ia = 0; ia < na; ia++) {
+                suma1 += cgblock[icga] * u1[ma1] * u2[ma2];
+                ma1++;
+                ma2--;
+                icga += l2;
+            }
+            zlist[llz] += cgblock[icgb] * suma1;
+            llu1 += l1 + 1;
+            llu2 -= l2 + 1;
+            icgb += l2;
+        }
+    }
+}
+
+/* compute_bi, SNAP.py:1101-1141 */
+static void compute_bi(const so4_tables *t, const cplx *ulisttot, const cplx *zlist, cplx *blist) {
+    for (int llb = 0; llb < t->nb; llb++) {
+        int l1 = t->idxb[3 * llb], l2 = t->idxb[3 * llb + 1], l = t->idxb[3 * llb + 2];
+        int llz = t->idxz_block[IDX3(t, l1, l2, l)];
+        int llu = t->idxu_block[l];
+        cplx sumzu = 0;
+        int mb = 0;
+        for (; 2 * mb < l; mb++)
+            for (int ma = 0; ma <= l; ma++) {
+                sumzu += conj(ulisttot[llu]) * zlist[llz];
+                llz++;
+                llu++;
+            }
+        if (l % 2 == 0) {
+            mb = l / 2;
+            for (int ma = 0; ma < mb; ma++) {
+                sumzu += conj(ulisttot[llu]) * zlist[llz];
+                llz++;
+                llu++;
+            }
+            sumzu += 0.5 * (conj(ulisttot[llu]) * zlist[llz]);
+        }
+        blist[llb] = 2.0 * sumzu;
+    }
+}
+
+/* one half-block sum of compute_dbidrj: 2 * sum conj(dU^lev) Z (middle x0.5) */
+static void dbidrj_half(int lev, int llz, int llu, const cplx *dulist, const cplx *zlist, cplx out[3]) {
+    cplx s[3] = {0, 0, 0};
+    int mb = 0;
+    for (; 2 * mb < lev; mb++)
+        for (int ma = 0; ma <= lev; ma++) {
+            for (int k = 0; k < 3; k++) s[k] += conj(dulist[3 * llu + k]) * zlist[llz];
+            llu++;
+            llz++;
+        }
+    if (lev % 2 == 0) {
+        mb = lev / 2;
+        for (int ma = 0; ma < mb; ma++) {
+            for (int k = 0; k < 3; k++) s[k] += conj(dulist[3 * llu + k]) * zlist[llz];
+            llu++;
+            llz++;
+        }
+        for (int k = 0; k < 3; k++) s[k] += conj(dulist[3 * llu + k]) * zlist[llz] * 0.5;
+    }
+    for (int k = 0; k < 3; k++) out[k] = s[k];
+}
+
+/* compute_dbidrj, SNAP.py:1143-1257.  dblist is [nb][3] (+=). */
+static void compute_dbidrj(const so4_tables *t, const cplx *dulist, const cplx *zlist, cplx *dblist) {
+    cplx s[3];
+    for (int llb = 0; llb < t->nb; llb++) {
+        int l1 = t->idxb[3 * llb], l2 = t->idxb[3 * llb + 1], l = t->idxb[3 * llb + 2];
+        dbidrj_half(l, t->idxz_block[IDX3(t, l1, l2, l)], t->idxu_block[l], dulist, zlist, s);
+        for (int k = 0; k < 3; k++) dblist[3 * llb + k] += 2.0 * s[k];
+        double l1fac = (l + 1) / (l1 + 1.0);
+        dbidrj_half(l1, t->idxz_block[IDX3(t, l, l2, l1)], t->idxu_block[l1], dulist, zlist, s);
+        for (int k = 0; k < 3; k++) dblist[3 * llb + k] += 2.0 * s[k] * l1fac;
+        double l2fac = (l + 1) / (l2 + 1.0);
+        dbidrj_half(l2, t->idxz_block[IDX3(t, l, l1, l2)], t->idxu_block[l2], dulist, zlist, s);
+        for (int k = 0; k < 3; k++) dblist[3 * llb + k] += 2.0 * s[k] * l2fac;
+    }
+}
+
+/* ------------------------------------------------------------ public API */
+
+int so4_oracle_sizes(int twol, int *ncg, int *nu, int *nz, int *nb) {
+    so4_tables t;
+    build_tables(twol, 0, &t);
+    *ncg = t.ncg;
+    *nu = t.nu;
+    *nz = t.nz;
+    *nb = t.nb;
+    free_tables(&t);
+    return 0;
+}
+
+/* Export the index tables (tests pin CG against exact Racah values and the
+ * device tables against these). */
+int so4_oracle_tables(int twol, double *cglist, int *idxz, int *idxb, int *idxu_block) {
+    so4_tables t;
+    build_tables(twol, 0, &t);
+    if (cglist) memcpy(cglist, t.cglist, sizeof(double) * t.ncg);
+    if (idxz) memcpy(idxz, t.idxz, sizeof(int) * 10 * (size_t)t.nz);
+    if (idxb) memcpy(idxb, t.idxb, sizeof(int) * 3 * (size_t)t.nb);
+    if (idxu_block) memcpy(idxu_block, t.idxu_block, sizeof(int) * t.ldim);
+    free_tables(&t);
+    return 0;
+}
+
+/* U array of one neighbour (real/imag interleaved), for tests. */
+int so4_oracle_uarray(int twol, double x, double y, double z, double psi, double r, double *u_out,
+                      double *du_out) {
+    so4_tables t;
+    build_tables(twol, 0, &t);
+    cplx *ul = calloc(t.nu, sizeof(cplx));
+    compute_uarray_recursive(&t, x, y, z, psi, r, ul);
+    if (u_out)
+        for (int i = 0; i < t.nu; i++) {
+            u_out[2 * i] = creal(ul[i]);
+            u_out[2 * i + 1] = cimag(ul[i]);
+        }
+    if (du_out) {
+        cplx *du = calloc(3 * (size_t)t.nu, sizeof(cplx));
+        compute_duarray_recursive(&t, x, y, z, psi, r, ul, du);
+        for (int i = 0; i < 3 * t.nu; i++) {
+            du_out[2 * i] = creal(du[i]);
+            du_out[2 * i + 1] = cimag(du[i]);
+        }
+        free(du);
+    }
+    free(ul);
+    free_tables(&t);
+    return 0;
+}
+
+/* get_bispectrum_components, SNAP.py:1289-1762 (all three branches).
+ *   center_atoms [N][3], neighborlist [P][M][3], seq [P][2], neighbor_w [P][M],
+ *   site_w [N]; outputs (real parts): blist [N][nb], dblist [P][nb][3],
+ *   bstress [P][nb][3][3]; zlist_out (optional) [N][nz] complex interleaved.
+ * Returns 0, or -1 for an unknown cutoff id (the reference raises). */
+int so4_oracle_compute(const double *center_atoms, int64_t N, const double *neighborlist, int64_t P,
+                       int64_t M, const int64_t *seq, const double *neighbor_w, const double *site_w,
+                       int twol, double rcut, int norm, int derivative, int stress, int cutoff_id,
+                       double rfac0, double *blist_out, double *dblist_out, double *bstress_out,
+                       double *zlist_out) {
+    if (cutoff_id < 1 || cutoff_id > 7) return -1;
+    if (P <= 0) return 0;
+    so4_tables t;
+    build_tables(twol, norm, &t);
+    const int nu = t.nu, nz = t.nz, nb = t.nb;
+    cplx *ulisttot = calloc(nu, sizeof(cplx));
+    cplx *zlist = calloc(nz, sizeof(cplx));
+    cplx *ulist = calloc(nu, sizeof(cplx));
+    cplx *dulist = calloc(3 * (size_t)nu, sizeof(cplx));
+    cplx *tempdb = calloc(3 * (size_t)nb, sizeof(cplx));
+    cplx *bl = calloc(nb, sizeof(cplx));
+    cplx *dbl = derivative ? calloc((size_t)P * nb * 3, sizeof(cplx)) : NULL;
+    cplx *bst = (derivative && stress) ? calloc((size_t)P * nb * 9, sizeof(cplx)) : NULL;
+    if (blist_out) memset(blist_out, 0, sizeof(double) * N * nb);
+
+    int64_t isite = seq[0], nstart = 0, nsite = 0;
+    for (int64_t n = 0; n <= P; n++) {
+        int64_t i = 0, j = 0;
+        int last = (n == P);
+        if (!last) {
+            i = seq[2 * n];
+            j = seq[2 * n + 1];
+        }
+        if (last || i != isite) {
+            /* finish centre isite */
+            memset(ulist, 0, sizeof(cplx) * nu);
+            addself_uarraytot(&t, ulist, cutoff_id, rcut);
+            for (int u = 0; u < nu; u++) ulist[u] *= site_w[isite];
+            for (int u = 0; u < nu; u++) {
+                ulisttot[u] += ulist[u];
+                ulisttot[u] *= t.u_norm[u];
+            }
+            compute_zi(&t, ulisttot, zlist);
+            compute_bi(&t, ulisttot, zlist, bl);
+            if (blist_out)
+                for (int k = 0; k < nb; k++) blist_out[isite * nb + k] = creal(bl[k]);
+            if (zlist_out)
+                for (int k = 0; k < nz; k++) {
+                    zlist_out[2 * ((size_t)isite * nz + k)] = creal(zlist[k]);
+                    zlist_out[2 * ((size_t)isite * nz + k) + 1] = cimag(zlist[k]);
+                }
+            if (derivative) {
+                for (int64_t NN = nstart; NN < n; NN++) {
+                    int64_t II = seq[2 * NN], JJ = seq[2 * NN + 1];
+                    const double *Ri = center_atoms + 3 * II;
+                    double Weight = neighbor_w[NN * M + 0];
+                    for (int64_t nb_ = 0; nb_ < M; nb_++) {
+                        const double *v = neighborlist + 3 * (NN * M + nb_);
+                        double x = v[0], y = v[1], z = v[2];
+                        double r = sqrt(x * x + y * y + z * z);
+                        if (r < 1e-8) continue;
+                        double psi = rfac0 * M_PI * r / rcut;
+                        memset(ulist, 0, sizeof(cplx) * nu);
+                        memset(dulist, 0, sizeof(cplx) * 3 * nu);
+                        compute_uarray_recursive(&t, x, y, z, psi, r, ulist);
+                        compute_duarray_recursive(&t, x, y, z, psi, r, ulist, dulist);
+                        dudr(&t, x, y, z, r, rcut, ulist, dulist, cutoff_id);
+                        for (int u = 0; u < nu; u++)
+                            for (int k = 0; k < 3; k++) {
+                                dulist[3 * u + k] *= Weight;
+                                dulist[3 * u + k] *= t.u_norm[u];
+                            }
+                        memset(tempdb, 0, sizeof(cplx) * 3 * nb);
+                        compute_dbidrj(&t, dulist, zlist, tempdb);
+                        if (II != JJ)
+                            for (int q = 0; q < 3 * nb; q++) {
+                                dbl[(size_t)NN * nb * 3 + q] += tempdb[q];
+                                dbl[(size_t)nsite * nb * 3 + q] -= tempdb[q];
+                            }
+                        if (stress) {
+                            double Rj[3] = {x + Ri[0], y + Ri[1], z + Ri[2]};
+                            for (int k = 0; k < nb; k++)
+                                for (int a = 0; a < 3; a++)
+                                    for (int c = 0; c < 3; c++) {
+                                        bst[((size_t)nsite * nb + k) * 9 + 3 * a + c] += Ri[a] * tempdb[3 * k + c];
+                                        bst[((size_t)NN * nb + k) * 9 + 3 * a + c] -= Rj[a] * tempdb[3 * k + c];
+                                    }
+                        }
+                    }
+                }
+            }
+            if (last) break;
+            isite = i;
+            nstart = n;
+            memset(ulisttot, 0, sizeof(cplx) * nu);
+            memset(zlist, 0, sizeof(cplx) * nz);
+        }
+        if (i == j) nsite = n;
+        double weight = neighbor_w[n * M + 0];
+        for (int64_t nb_ = 0; nb_ < M; nb_++) {
+            const double *v = neighborlist + 3 * (n * M + nb_);
+            double x = v[0], y = v[1], z = v[2];
+            double r = sqrt(x * x + y * y + z * z);
+            if (r < 1e-8) continue;
+            double psi = rfac0 * M_PI * r / rcut;
+            memset(ulist, 0, sizeof(cplx) * nu);
+            compute_uarray_recursive(&t, x, y, z, psi, r, ulist);
+            double sfac = cut_f(r, rcut, cutoff_id);
+            for (int u = 0; u < nu; u++) ulisttot[u] += sfac * (ulist[u] * weight);
+        }
+    }
+
+    if (dblist_out && dbl)
+        for (size_t q = 0; q < (size_t)P * nb * 3; q++) dblist_out[q] = creal(dbl[q]);
+    if (bstress_out && bst)
+        for (size_t q = 0; q < (size_t)P * nb * 9; q++) bstress_out[q] = creal(bst[q]);
+
+    free(ulisttot);
+    free(zlist);
+    free(ulist);
+    free(dulist);
+    free(tempdb);
+    free(bl);
+    free(dbl);
+    free(bst);
+    free_tables(&t);
+    return 0;
+}

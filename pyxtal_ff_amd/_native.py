@@ -1,0 +1,119 @@
+"""ctypes binding of libpxff.so (the gfx950 C ABI declared in include/pxff.h).
+
+The library is built in-tree by ``pyxtal_ff_amd/csrc/build.sh`` (driven by
+``__graft_entry__.build()``).  There is no CPU fallback: if the library is
+missing, or no GPU is visible, every entry point raises.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libpxff.so")
+_lib = None
+
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+I32 = ctypes.c_int
+F64 = ctypes.c_double
+
+
+class PxffError(RuntimeError):
+    pass
+
+
+class So4Params(ctypes.Structure):
+    _fields_ = [("twol", ctypes.c_int), ("rcut", ctypes.c_double), ("rfac0", ctypes.c_double),
+                ("cutoff_id", ctypes.c_int), ("normalize_u", ctypes.c_int)]
+
+
+class So3Params(ctypes.Structure):
+    _fields_ = [("nmax", ctypes.c_int), ("lmax", ctypes.c_int), ("rcut", ctypes.c_double),
+                ("alpha", ctypes.c_double), ("cutoff_id", ctypes.c_int), ("nquad", ctypes.c_int)]
+
+
+_SIGS = {
+    "pxff_last_error": ([ctypes.c_char_p, ctypes.c_size_t], I32),
+    "pxff_version": ([], ctypes.c_char_p),
+    "pxff_scan_i64": ([P, I64, P, P], I32),
+    "pxff_nlist_count": ([P, I64, P, P, P, P, P, P, I64, F64, P, P, P, P, P, P, P], I32),
+    "pxff_nlist_fill": ([P, I64, P, P, P, P, P, F64, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "pxff_nlist_rows": ([I64, I64, I64, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "pxff_so4_sizes": ([ctypes.c_int, P, P, P, P], I32),
+    "pxff_so4_ui": ([P, I64, P, P, P, P, P, P, P, P], I32),
+    "pxff_so4_zb": ([P, I64, P, P, P, P], I32),
+    "pxff_so4_yi": ([P, I64, P, P, P, P], I32),
+    "pxff_so4_deidrj": ([P, I64, P, P, P, P, P, P, P], I32),
+    "pxff_so4_gather": ([I64, P, P, P, P, P, P, P], I32),
+    "pxff_so4_tangent": ([P, I64, P, P, P, P, P, P, P, P, P], I32),
+    "pxff_so4_dbdr": ([P, I64, P, P, P, P, P, P, P], I32),
+    "pxff_so4_rows": ([I64, ctypes.c_int, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "pxff_dxdr_contract": ([I64, ctypes.c_int, P, P, P, P, P, P, P], I32),
+    "pxff_rdxdr_reduce": ([I64, ctypes.c_int, P, P, P, P], I32),
+}
+
+OPTIONAL = set()
+
+
+def exported_symbols():
+    """Names this binding requires from the library."""
+    return sorted(_SIGS)
+
+
+def load(require_gpu=True):
+    """Load libpxff.so; raise if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PxffError("libpxff.so not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (args, res) in _SIGS.items():
+        if not hasattr(L, name):
+            if name in OPTIONAL:
+                continue
+            raise PxffError("libpxff.so lacks symbol %s" % name)
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def lib():
+    L = load()
+    if not torch.cuda.is_available():
+        raise PxffError("pyxtal_ff_amd needs a visible gfx950 GPU (torch.cuda.is_available() is False)")
+    return L
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    load().pxff_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc, what):
+    if rc != 0:
+        raise PxffError("%s failed (%d): %s" % (what, rc, last_error()))
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise PxffError("expected a device tensor")
+    if not t.is_contiguous():
+        raise PxffError("expected a contiguous tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def call(name, *args):
+    rc = getattr(lib(), name)(*args)
+    check(rc, name)

@@ -1,0 +1,413 @@
+// Neighbour list for the descriptor hot path on gfx950.
+//
+// Replaces the ASE NeighborList call and the host regrouping of the
+// reference (pyxtal_ff/descriptors/SNAP.py:240-317, SO3.py:317-376): every
+// image (i, j, T) with |pos_j + T.cell - pos_i| < rc except (i, T=0), for a
+// batch of structures at once.
+//
+// Algorithm: each structure's atoms are wrapped into the cell along its
+// periodic directions (fractional coordinate f = s - floor(s)) and binned on
+// an nb0 x nb1 x nb2 grid (nb_d = floor(h_d / rc), h_d = plane spacing).  A
+// centre scans the (2S+1)^3 stencil of bins around its own, S_d =
+// ceil(rc nb_d / h_d); a stencil offset that leaves the grid wraps into the
+// next periodic image, so small cells (nb = 1, S = 1 or 2) are handled by the
+// same code -- the stencil then enumerates lattice translations directly.
+// Candidate separations are computed from the raw positions in a canonical
+// orientation with no FMA contraction (see canon_sep), so the image set is
+// bit-identical to the CPU oracle's brute-force enumeration and exactly
+// symmetric (r_ji = -r_ij), which the force gather relies on.
+#include <hip/hip_runtime.h>
+
+#include "pxff_common.h"
+
+namespace pxff {
+
+__device__ inline int floordiv_i(int a, int b) {
+    int q = a / b;
+    if ((a % b != 0) && ((a < 0) != (b < 0))) q -= 1;
+    return q;
+}
+
+// Separation of image (i, j, T) computed as
+//   r = (pos_j + ((T0 c0 + T1 c1) + T2 c2)) - pos_i       if (i,j,T) is "forward"
+//   r = -((pos_i + ((-T0 c0 - T1 c1) - T2 c2)) - pos_j)   otherwise,
+// forward meaning j > i, or j == i and T > 0 lexicographically.  All
+// operations are individually rounded (no contraction).
+__device__ inline void canon_sep(const double *__restrict__ pos, const double *__restrict__ cell, int64_t i, int64_t j,
+                                 int T0, int T1, int T2, double r[3]) {
+    bool fwd;
+    if (j != i)
+        fwd = j > i;
+    else
+        fwd = (T0 > 0) || (T0 == 0 && T1 > 0) || (T0 == 0 && T1 == 0 && T2 > 0);
+    const int s = fwd ? 1 : -1;
+    const int64_t a = fwd ? j : i, b = fwd ? i : j;
+    const double t0 = (double)(s * T0), t1 = (double)(s * T1), t2 = (double)(s * T2);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double t = __dmul_rn(t0, cell[k]);
+        t = __dadd_rn(t, __dmul_rn(t1, cell[3 + k]));
+        t = __dadd_rn(t, __dmul_rn(t2, cell[6 + k]));
+        double p = __dadd_rn(pos[3 * a + k], t);
+        double v = __dsub_rn(p, pos[3 * b + k]);
+        r[k] = fwd ? v : -v;
+    }
+}
+
+__device__ inline double dist2(const double r[3]) {
+    return __dadd_rn(__dadd_rn(__dmul_rn(r[0], r[0]), __dmul_rn(r[1], r[1])), __dmul_rn(r[2], r[2]));
+}
+
+// ---------------------------------------------------------------- binning
+__global__ void nl_bin_kernel(int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
+                              const double *__restrict__ inv_cells, const int32_t *__restrict__ pbc,
+                              const int32_t *__restrict__ geom, int32_t *__restrict__ atom_bin,
+                              int32_t *__restrict__ wrap, int32_t *__restrict__ bin_count) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms) return;
+    const int64_t s = atom_struct[a];
+    const double *inv = inv_cells + 9 * s;
+    const int32_t *gm = geom + 8 * s;
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double f = pos[3 * a] * inv[d] + pos[3 * a + 1] * inv[3 + d] + pos[3 * a + 2] * inv[6 + d];
+        int k = 0, bd = 0;
+        if (pbc[3 * s + d]) {
+            const double fl = floor(f);
+            k = (int)fl;
+            double fr = f - fl;
+            bd = (int)(fr * gm[d]);
+            if (bd >= gm[d]) bd = gm[d] - 1;
+            if (bd < 0) bd = 0;
+        }
+        wrap[3 * a + d] = k;
+        b[d] = bd;
+    }
+    const int gb = gm[6] + (b[0] * gm[1] + b[1]) * gm[2] + b[2];
+    atom_bin[a] = gb;
+    atomicAdd(&bin_count[gb], 1);
+}
+
+__global__ void nl_bin_fill_kernel(int64_t natoms, const int32_t *__restrict__ atom_bin,
+                                   const int32_t *__restrict__ bin_start, int32_t *__restrict__ cursor,
+                                   int32_t *__restrict__ bin_atoms) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms) return;
+    const int gb = atom_bin[a];
+    const int slot = atomicAdd(&cursor[gb], 1);
+    bin_atoms[bin_start[gb] + slot] = (int32_t)a;
+}
+
+// Enumerate the candidate images of centre i; calls f(j, T0, T1, T2, r) for
+// every image inside the cutoff.
+template <typename F>
+__device__ inline void nl_enumerate(int64_t i, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
+                                    const double *__restrict__ cells, const int32_t *__restrict__ pbc,
+                                    const int32_t *__restrict__ geom, double rc2, const int32_t *__restrict__ bin_start,
+                                    const int32_t *__restrict__ atom_bin, const int32_t *__restrict__ wrap,
+                                    const int32_t *__restrict__ bin_atoms, F &&f) {
+    const int64_t s = atom_struct[i];
+    const double *cell = cells + 9 * s;
+    const int32_t *gm = geom + 8 * s;
+    const int nb0 = gm[0], nb1 = gm[1], nb2 = gm[2];
+    const int S0 = gm[3], S1 = gm[4], S2 = gm[5];
+    const int off = gm[6];
+    const int lb = atom_bin[i] - off;
+    const int bi0 = lb / (nb1 * nb2), bi1 = (lb / nb2) % nb1, bi2 = lb % nb2;
+    const int ki0 = wrap[3 * i], ki1 = wrap[3 * i + 1], ki2 = wrap[3 * i + 2];
+    const bool p0 = pbc[3 * s], p1 = pbc[3 * s + 1], p2 = pbc[3 * s + 2];
+    for (int d0 = -S0; d0 <= S0; d0++) {
+        int tb0 = bi0 + d0, t0 = 0;
+        if (p0) {
+            t0 = floordiv_i(tb0, nb0);
+            tb0 -= t0 * nb0;
+        } else if (tb0 < 0 || tb0 >= nb0)
+            continue;
+        for (int d1 = -S1; d1 <= S1; d1++) {
+            int tb1 = bi1 + d1, t1 = 0;
+            if (p1) {
+                t1 = floordiv_i(tb1, nb1);
+                tb1 -= t1 * nb1;
+            } else if (tb1 < 0 || tb1 >= nb1)
+                continue;
+            for (int d2 = -S2; d2 <= S2; d2++) {
+                int tb2 = bi2 + d2, t2 = 0;
+                if (p2) {
+                    t2 = floordiv_i(tb2, nb2);
+                    tb2 -= t2 * nb2;
+                } else if (tb2 < 0 || tb2 >= nb2)
+                    continue;
+                const int gb = off + (tb0 * nb1 + tb1) * nb2 + tb2;
+                for (int q = bin_start[gb]; q < bin_start[gb + 1]; q++) {
+                    const int64_t j = bin_atoms[q];
+                    const int T0 = t0 - wrap[3 * j] + ki0;
+                    const int T1 = t1 - wrap[3 * j + 1] + ki1;
+                    const int T2 = t2 - wrap[3 * j + 2] + ki2;
+                    if (j == i && T0 == 0 && T1 == 0 && T2 == 0) continue;
+                    double r[3];
+                    canon_sep(pos, cell, i, j, T0, T1, T2, r);
+                    if (dist2(r) < rc2) f(j, T0, T1, T2, r);
+                }
+            }
+        }
+    }
+}
+
+__global__ void nl_count_kernel(int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
+                                const double *__restrict__ cells, const int32_t *__restrict__ pbc,
+                                const int32_t *__restrict__ geom, double rc2, const int32_t *__restrict__ bin_start,
+                                const int32_t *__restrict__ atom_bin, const int32_t *__restrict__ wrap,
+                                const int32_t *__restrict__ bin_atoms, int64_t *__restrict__ img_count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natoms) return;
+    int64_t c = 0;
+    nl_enumerate(i, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap, bin_atoms,
+                 [&](int64_t, int, int, int, const double *) { c++; });
+    img_count[i] = c;
+}
+
+__device__ inline bool key_less(int64_t ja, const int32_t *Ta, int64_t jb, const int32_t *Tb) {
+    if (ja != jb) return ja < jb;
+    if (Ta[0] != Tb[0]) return Ta[0] < Tb[0];
+    if (Ta[1] != Tb[1]) return Ta[1] < Tb[1];
+    return Ta[2] < Tb[2];
+}
+
+__global__ void nl_fill_kernel(int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
+                               const double *__restrict__ cells, const int32_t *__restrict__ pbc,
+                               const int32_t *__restrict__ geom, double rc2, const int32_t *__restrict__ bin_start,
+                               const int32_t *__restrict__ atom_bin, const int32_t *__restrict__ wrap,
+                               const int32_t *__restrict__ bin_atoms, const int64_t *__restrict__ img_start,
+                               int64_t *__restrict__ img_atom, int64_t *__restrict__ img_j, int32_t *__restrict__ img_T,
+                               double *__restrict__ img_r, int64_t *__restrict__ row_count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natoms) return;
+    const int64_t s0 = img_start[i];
+    int64_t c = s0;
+    nl_enumerate(i, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap, bin_atoms,
+                 [&](int64_t j, int T0, int T1, int T2, const double *r) {
+                     img_j[c] = j;
+                     img_T[3 * c] = T0;
+                     img_T[3 * c + 1] = T1;
+                     img_T[3 * c + 2] = T2;
+                     img_r[3 * c] = r[0];
+                     img_r[3 * c + 1] = r[1];
+                     img_r[3 * c + 2] = r[2];
+                     c++;
+                 });
+    const int64_t s1 = c;
+    // insertion sort of the centre's images by (j, T0, T1, T2)
+    for (int64_t a = s0 + 1; a < s1; a++) {
+        const int64_t ja = img_j[a];
+        const int32_t Ta[3] = {img_T[3 * a], img_T[3 * a + 1], img_T[3 * a + 2]};
+        const double ra[3] = {img_r[3 * a], img_r[3 * a + 1], img_r[3 * a + 2]};
+        int64_t b = a - 1;
+        while (b >= s0 && key_less(ja, Ta, img_j[b], img_T + 3 * b)) {
+            img_j[b + 1] = img_j[b];
+            for (int k = 0; k < 3; k++) {
+                img_T[3 * (b + 1) + k] = img_T[3 * b + k];
+                img_r[3 * (b + 1) + k] = img_r[3 * b + k];
+            }
+            b--;
+        }
+        img_j[b + 1] = ja;
+        for (int k = 0; k < 3; k++) {
+            img_T[3 * (b + 1) + k] = Ta[k];
+            img_r[3 * (b + 1) + k] = ra[k];
+        }
+    }
+    int64_t rows = 0;
+    bool has_self = false;
+    for (int64_t a = s0; a < s1; a++) {
+        img_atom[a] = i;
+        if (a == s0 || img_j[a] != img_j[a - 1]) rows++;
+        if (img_j[a] == i) has_self = true;
+    }
+    row_count[i] = rows + (has_self ? 0 : 1);
+}
+
+// seq rows: for centre i, ascending unique j with (i, i) inserted
+// (SNAP.py:279-296).
+__global__ void nl_rows_kernel(int64_t natoms, int64_t nimg, const int64_t *__restrict__ img_start,
+                               const int64_t *__restrict__ img_j, const int64_t *__restrict__ row_start,
+                               int64_t *__restrict__ seq, int64_t *__restrict__ row_img, int64_t *__restrict__ img_row,
+                               int64_t *__restrict__ nsite) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natoms) return;
+    int64_t rw = row_start[i];
+    const int64_t s0 = img_start[i], s1 = img_start[i + 1];
+    bool self_done = false;
+    int64_t a = s0;
+    while (a < s1 || !self_done) {
+        int64_t j = (a < s1) ? img_j[a] : INT64_MAX;
+        if (!self_done && i <= j) {
+            // (i, i) row: with the self images if j == i, else empty
+            seq[2 * rw] = i;
+            seq[2 * rw + 1] = i;
+            nsite[i] = rw;
+            row_img[rw] = a;
+            if (j == i) {
+                while (a < s1 && img_j[a] == i) img_row[a++] = rw;
+            }
+            self_done = true;
+            rw++;
+            continue;
+        }
+        seq[2 * rw] = i;
+        seq[2 * rw + 1] = j;
+        row_img[rw] = a;
+        while (a < s1 && img_j[a] == j) img_row[a++] = rw;
+        rw++;
+    }
+    if (i == natoms - 1) row_img[rw] = nimg;
+}
+
+// reverse image (j, i, -T) and reverse row (j, i) by binary search in centre
+// j's sorted images / rows.
+__global__ void nl_rev_kernel(int64_t natoms, int64_t nimg, const int64_t *__restrict__ img_start,
+                              const int64_t *__restrict__ img_atom, const int64_t *__restrict__ img_j,
+                              const int32_t *__restrict__ img_T, int64_t *__restrict__ img_rev,
+                              int32_t *__restrict__ flag) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nimg) return;
+    const int64_t i = img_atom[e], j = img_j[e];
+    const int32_t T[3] = {-img_T[3 * e], -img_T[3 * e + 1], -img_T[3 * e + 2]};
+    int64_t lo = img_start[j], hi = img_start[j + 1];
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (key_less(img_j[mid], img_T + 3 * mid, i, T))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo < img_start[j + 1] && img_j[lo] == i && img_T[3 * lo] == T[0] && img_T[3 * lo + 1] == T[1] &&
+        img_T[3 * lo + 2] == T[2]) {
+        img_rev[e] = lo;
+    } else {
+        img_rev[e] = e;  // cannot happen for a symmetric list; flagged
+        atomicOr(flag, 1);
+    }
+}
+
+__global__ void nl_row_rev_kernel(int64_t nrows, const int64_t *__restrict__ seq, const int64_t *__restrict__ row_start,
+                                  int64_t *__restrict__ row_rev, int32_t *__restrict__ flag) {
+    const int64_t rw = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (rw >= nrows) return;
+    const int64_t i = seq[2 * rw], j = seq[2 * rw + 1];
+    int64_t lo = row_start[j], hi = row_start[j + 1];
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (seq[2 * mid + 1] < i)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo < row_start[j + 1] && seq[2 * lo + 1] == i) {
+        row_rev[rw] = lo;
+    } else {
+        row_rev[rw] = rw;
+        atomicOr(flag, 2);
+    }
+}
+
+// ------------------------------------------------------------ device scan
+// Exclusive scan by one workgroup of 1024 threads (chunked).  out has n+1
+// entries; out[n] = total.
+template <typename T>
+__global__ __launch_bounds__(1024) void scan_kernel(int64_t n, const T *__restrict__ in, T *__restrict__ out) {
+    __shared__ T part[1024];
+    const int tid = threadIdx.x;
+    const int64_t chunk = (n + 1023) / 1024;
+    const int64_t b = tid * chunk, e = (b + chunk < n) ? b + chunk : n;
+    T s = 0;
+    for (int64_t k = b; k < e; k++) s += in[k];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        T v = (tid >= off) ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    T run = (tid > 0) ? part[tid - 1] : 0;
+    for (int64_t k = b; k < e; k++) {
+        T v = in[k];
+        out[k] = run;
+        run += v;
+    }
+    if (tid == 1023) out[n] = part[1023];
+}
+
+}  // namespace pxff
+
+using namespace pxff;
+
+extern "C" int pxff_scan_i64(const int64_t *d_in, int64_t n, int64_t *d_out, void *stream) {
+    PXFF_REQUIRE(n >= 0, "n must be >= 0");
+    scan_kernel<int64_t><<<1, 1024, 0, (hipStream_t)stream>>>(n, d_in, d_out);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_nlist_count(const double *d_pos, int64_t natoms, const int64_t *d_atom_struct,
+                                const int64_t *d_atom_start, const double *d_cells, const double *d_inv_cells,
+                                const int32_t *d_pbc, const int32_t *d_geom, int64_t nbins_total, double rc,
+                                int32_t *d_bin_count, int32_t *d_bin_start, int32_t *d_atom_bin, int32_t *d_wrap,
+                                int32_t *d_bin_atoms, int64_t *d_img_count, void *stream) {
+    (void)d_atom_start;
+    PXFF_REQUIRE(rc > 0, "rc must be > 0");
+    PXFF_REQUIRE(nbins_total > 0, "nbins_total must be > 0");
+    if (natoms <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    PXFF_HIP_CHECK(hipMemsetAsync(d_bin_count, 0, sizeof(int32_t) * nbins_total, s));
+    nl_bin_kernel<<<div_up(natoms, 256), 256, 0, s>>>(natoms, d_pos, d_atom_struct, d_inv_cells, d_pbc, d_geom,
+                                                      d_atom_bin, d_wrap, d_bin_count);
+    PXFF_LAUNCH_CHECK();
+    scan_kernel<int32_t><<<1, 1024, 0, s>>>(nbins_total, d_bin_count, d_bin_start);
+    PXFF_LAUNCH_CHECK();
+    PXFF_HIP_CHECK(hipMemsetAsync(d_bin_count, 0, sizeof(int32_t) * nbins_total, s));
+    nl_bin_fill_kernel<<<div_up(natoms, 256), 256, 0, s>>>(natoms, d_atom_bin, d_bin_start, d_bin_count, d_bin_atoms);
+    PXFF_LAUNCH_CHECK();
+    nl_count_kernel<<<div_up(natoms, 64), 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom, rc * rc,
+                                                      d_bin_start, d_atom_bin, d_wrap, d_bin_atoms, d_img_count);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_nlist_fill(const double *d_pos, int64_t natoms, const int64_t *d_atom_struct,
+                               const int64_t *d_atom_start, const double *d_cells, const int32_t *d_pbc,
+                               const int32_t *d_geom, double rc, const int32_t *d_bin_start, const int32_t *d_atom_bin,
+                               const int32_t *d_wrap, const int32_t *d_bin_atoms, const int64_t *d_img_start,
+                               int64_t *d_img_atom, int64_t *d_img_j, int32_t *d_img_T, double *d_img_r,
+                               int64_t *d_row_count, void *stream) {
+    (void)d_atom_start;
+    if (natoms <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    nl_fill_kernel<<<div_up(natoms, 64), 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom, rc * rc,
+                                                     d_bin_start, d_atom_bin, d_wrap, d_bin_atoms, d_img_start,
+                                                     d_img_atom, d_img_j, d_img_T, d_img_r, d_row_count);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_nlist_rows(int64_t natoms, int64_t nimg, int64_t nrows, const int64_t *d_img_start,
+                               const int64_t *d_img_atom, const int64_t *d_img_j, const int32_t *d_img_T,
+                               const int64_t *d_row_start, int64_t *d_seq, int64_t *d_row_img, int64_t *d_img_row,
+                               int64_t *d_nsite, int64_t *d_row_rev, int64_t *d_img_rev, int32_t *d_flag,
+                               void *stream) {
+    if (natoms <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    PXFF_HIP_CHECK(hipMemsetAsync(d_flag, 0, sizeof(int32_t), s));
+    nl_rows_kernel<<<div_up(natoms, 64), 64, 0, s>>>(natoms, nimg, d_img_start, d_img_j, d_row_start, d_seq, d_row_img,
+                                                     d_img_row, d_nsite);
+    PXFF_LAUNCH_CHECK();
+    if (nimg > 0) {
+        nl_rev_kernel<<<div_up(nimg, 256), 256, 0, s>>>(natoms, nimg, d_img_start, d_img_atom, d_img_j, d_img_T,
+                                                        d_img_rev, d_flag);
+        PXFF_LAUNCH_CHECK();
+    }
+    nl_row_rev_kernel<<<div_up(nrows, 256), 256, 0, s>>>(nrows, d_seq, d_row_start, d_row_rev, d_flag);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}

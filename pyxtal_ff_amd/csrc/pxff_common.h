@@ -1,0 +1,122 @@
+// Shared host/device helpers of the pxff library (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/pxff.h"
+
+namespace pxff {
+
+void set_error(const std::string &msg);
+
+#define PXFF_HIP_CHECK(expr)                                                                  \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) {                                                               \
+            ::pxff::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));            \
+            return PXFF_EHIP;                                                                 \
+        }                                                                                     \
+    } while (0)
+
+#define PXFF_LAUNCH_CHECK()                                                                   \
+    do {                                                                                      \
+        hipError_t _e = hipGetLastError();                                                    \
+        if (_e != hipSuccess) {                                                               \
+            ::pxff::set_error(std::string("kernel launch: ") + hipGetErrorString(_e));       \
+            return PXFF_EHIP;                                                                 \
+        }                                                                                     \
+    } while (0)
+
+#define PXFF_REQUIRE(cond, msg)                                                               \
+    do {                                                                                      \
+        if (!(cond)) {                                                                        \
+            ::pxff::set_error(msg);                                                           \
+            return PXFF_EINVAL;                                                               \
+        }                                                                                     \
+    } while (0)
+
+// ---------------------------------------------------------------- complex
+struct cd {
+    double re, im;
+};
+
+__host__ __device__ inline cd cmake(double r, double i) { return cd{r, i}; }
+__host__ __device__ inline cd cadd(cd a, cd b) { return cd{a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline cd csub(cd a, cd b) { return cd{a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline cd cmul(cd a, cd b) { return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__host__ __device__ inline cd cscale(cd a, double s) { return cd{a.re * s, a.im * s}; }
+__host__ __device__ inline cd cconj(cd a) { return cd{a.re, -a.im}; }
+// Re(conj(a) * b)
+__host__ __device__ inline double cdotre(cd a, cd b) { return a.re * b.re + a.im * b.im; }
+
+// ---------------------------------------------------------------- cutoffs
+// SNAP.py:336-440 (compute_sfac / compute_dsfac, SNAP.py:512-588): 0 for r > rc.
+__device__ inline void cutoff_fdf(int id, double r, double rc, double &f, double &df) {
+    if (r > rc) {
+        f = 0.0;
+        df = 0.0;
+        return;
+    }
+    double x = r / rc;
+    switch (id) {
+    case 1: {
+        double s, c;
+        sincos(M_PI * r / rc, &s, &c);
+        f = 0.5 * (c + 1.0);
+        df = -0.5 * M_PI / rc * s;
+        break;
+    }
+    case 2: {
+        double t = tanh(1.0 - x);
+        double t2 = t * t;
+        f = t2 * t;
+        df = -(3.0 / rc) * t2 * (1.0 - t2);
+        break;
+    }
+    case 3:
+        f = x * x * (2 * x - 3) + 1;
+        df = (6.0 / (rc * rc)) * r * (x - 1.0);
+        break;
+    case 4:
+        f = x * x * x * (x * (15 - 6 * x) - 10) + 1;
+        df = (-30.0 / rc) * (x * x * (x - 1) * (x - 1));
+        break;
+    case 5:
+        f = x * x * x * x * (x * (x * (20 * x - 70) + 84) - 35) + 1;
+        df = (140.0 / rc) * (x * x * x * (x - 1) * (x - 1) * (x - 1));
+        break;
+    case 6: {
+        f = x * x * x * x * x * (x * (x * (x * (315 - 70 * x) - 540) + 420) - 126) + 1;
+        double y = x * (x - 1);
+        df = (-630.0 / rc) * y * y * y * y;
+        break;
+    }
+    case 7: {
+        double e = exp(1.0 - 1.0 / (1.0 - x * x));
+        double d = 1 + x * x;
+        f = e;
+        df = 2 * x * e / (d * d);
+        break;
+    }
+    default:
+        f = 0.0;
+        df = 0.0;
+    }
+}
+
+// Cutoff value at r = 0 (addself_uarraytot, SNAP.py:590-627).
+inline double cutoff_f0_host(int id) {
+    switch (id) {
+    case 1: return 1.0;
+    case 2: { double t = std::tanh(1.0); return t * t * t; }
+    case 3: case 4: case 5: case 6: return 1.0;
+    case 7: return 1.0;
+    default: return 0.0;
+    }
+}
+
+inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+}  // namespace pxff

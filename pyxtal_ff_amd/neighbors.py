@@ -1,0 +1,161 @@
+"""Device neighbour list (the reference's ``build_neighbor_list``).
+
+Replaces ``pyxtal_ff/descriptors/SNAP.py:240-317`` (ASE ``NeighborList`` with
+cutoffs ``[rc/2]*N``, ``self_interaction=False``, ``bothways=True``,
+``skin=0``, then the per-centre regrouping into ``seq`` rows) for a batch of
+structures in three device phases (count, fill, rows).  Everything stays in
+HBM; only the two totals (images, rows) come back to the host to size the
+outputs.
+
+Per image e: centre ``img_atom``, neighbour ``img_j``, translation ``img_T``,
+separation ``img_r`` (= pos_j + T.cell - pos_i), its seq row ``img_row`` and
+the reverse image ``img_rev`` (j, i, -T).  Per row: ``seq`` (i, j),
+``row_img`` (image range) and ``row_rev``.  Per atom: ``img_start``,
+``row_start`` and ``nsite`` (the (i, i) row).
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+
+def _geometry(cell, pbc, rc):
+    """Bin grid of one structure: nb_d = floor(h_d / rc) (>= 1) bins along
+    lattice direction d, stencil half-width S_d = ceil(rc nb_d / h_d), where
+    h_d is the spacing of the lattice planes normal to direction d."""
+    inv = np.linalg.inv(cell)
+    geom = np.zeros(8, dtype=np.int32)
+    for d in range(3):
+        if not pbc[d]:
+            geom[d], geom[3 + d] = 1, 0
+            continue
+        h = 1.0 / np.linalg.norm(inv[:, d])
+        nb = max(1, int(math.floor(h / rc)))
+        nb = min(nb, 1024)
+        S = int(math.ceil(rc * nb / h * (1 + 1e-12) + 1e-12))
+        geom[d], geom[3 + d] = nb, S
+    return inv, geom
+
+
+class DeviceNeighborList:
+    """Neighbour images and seq rows of a batch of structures on the GPU.
+
+    positions: list of [N_s, 3] arrays, cells: [nstruct, 3, 3],
+    pbcs: [nstruct, 3].  With ``pos_device`` (a float64 [N, 3] device tensor
+    of a single structure) positions are taken from HBM and ``positions``
+    must be None."""
+
+    def __init__(self, positions, cells, pbcs, rc, device="cuda", pos_device=None):
+        if pos_device is not None:
+            if positions is not None:
+                raise ValueError("give positions or pos_device, not both")
+            pos_list = [np.zeros((pos_device.shape[0], 0))]
+        else:
+            pos_list = [np.asarray(p, dtype=np.float64).reshape(-1, 3) for p in positions]
+        cells = np.asarray(cells, dtype=np.float64).reshape(-1, 3, 3)
+        pbcs = np.asarray(pbcs, dtype=bool).reshape(-1, 3)
+        nst = len(pos_list)
+        if cells.shape[0] != nst or pbcs.shape[0] != nst:
+            raise ValueError("positions, cells and pbcs must describe the same structures")
+        self.rc = float(rc)
+        counts = np.array([len(p) for p in pos_list], dtype=np.int64)
+        atom_start = np.zeros(nst + 1, dtype=np.int64)
+        atom_start[1:] = np.cumsum(counts)
+        self.natoms = int(atom_start[-1])
+        self.nstruct = nst
+        self.atom_start_host = atom_start
+        invs = np.zeros((nst, 9))
+        geoms = np.zeros((nst, 8), dtype=np.int32)
+        off = 0
+        for s in range(nst):
+            inv, g = _geometry(cells[s], pbcs[s], self.rc)
+            invs[s] = inv.reshape(9)
+            g[6] = off
+            off += int(g[0]) * int(g[1]) * int(g[2])
+            geoms[s] = g
+        nbins = off
+        dev = torch.device(device)
+        f64 = dict(dtype=torch.float64, device=dev)
+        i64 = dict(dtype=torch.int64, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        if pos_device is not None:
+            if pos_device.dtype != torch.float64 or not pos_device.is_cuda:
+                raise ValueError("pos_device must be a float64 device tensor")
+            self.pos = pos_device.contiguous()
+        else:
+            pos_all = np.concatenate(pos_list, axis=0) if nst else np.zeros((0, 3))
+            self.pos = torch.as_tensor(pos_all, **f64).contiguous()
+        self.atom_struct = torch.as_tensor(np.repeat(np.arange(nst), counts), **i64)
+        self.atom_start = torch.as_tensor(atom_start, **i64)
+        self.cells = torch.as_tensor(cells.reshape(nst, 9), **f64).contiguous()
+        self.inv_cells = torch.as_tensor(invs, **f64).contiguous()
+        self.pbc = torch.as_tensor(pbcs.astype(np.int32), **i32).contiguous()
+        self.geom = torch.as_tensor(geoms, **i32).contiguous()
+        vols = np.abs(np.linalg.det(cells)) if nst else np.zeros(0)
+        self.volumes = vols
+        self.inv_vol = torch.as_tensor(np.repeat(1.0 / vols, counts), **f64)
+        self._build(nbins, f64, i64, i32)
+
+    def _build(self, nbins, f64, i64, i32):
+        n = self.natoms
+        s = nat.stream()
+        L = nat.lib()
+        bin_count = torch.zeros(max(nbins, 1), **i32)
+        bin_start = torch.zeros(max(nbins, 1) + 1, **i32)
+        atom_bin = torch.zeros(max(n, 1), **i32)
+        wrap = torch.zeros(max(n, 1) * 3, **i32)
+        bin_atoms = torch.zeros(max(n, 1), **i32)
+        img_count = torch.zeros(max(n, 1), **i64)
+        p = nat.ptr
+        nat.check(L.pxff_nlist_count(p(self.pos), n, p(self.atom_struct), p(self.atom_start), p(self.cells),
+                                     p(self.inv_cells), p(self.pbc), p(self.geom), max(nbins, 1), self.rc,
+                                     p(bin_count), p(bin_start), p(atom_bin), p(wrap), p(bin_atoms), p(img_count), s),
+                  "pxff_nlist_count")
+        img_start = torch.zeros(n + 1, **i64)
+        nat.check(L.pxff_scan_i64(p(img_count), n, p(img_start), s), "pxff_scan_i64")
+        nimg = int(img_start[n].item())
+        self.nimg = nimg
+        self.img_start = img_start
+        m = max(nimg, 1)
+        self.img_atom = torch.zeros(m, **i64)
+        self.img_j = torch.zeros(m, **i64)
+        self.img_T = torch.zeros(m * 3, **i32)
+        self.img_r = torch.zeros(m * 3, **f64)
+        row_count = torch.zeros(max(n, 1), **i64)
+        nat.check(L.pxff_nlist_fill(p(self.pos), n, p(self.atom_struct), p(self.atom_start), p(self.cells),
+                                    p(self.pbc), p(self.geom), self.rc, p(bin_start), p(atom_bin), p(wrap),
+                                    p(bin_atoms), p(img_start), p(self.img_atom), p(self.img_j), p(self.img_T),
+                                    p(self.img_r), p(row_count), s), "pxff_nlist_fill")
+        row_start = torch.zeros(n + 1, **i64)
+        nat.check(L.pxff_scan_i64(p(row_count), n, p(row_start), s), "pxff_scan_i64")
+        nrows = int(row_start[n].item())
+        self.nrows = nrows
+        self.row_start = row_start
+        r = max(nrows, 1)
+        self.seq = torch.zeros(r * 2, **i64)
+        self.row_img = torch.zeros(r + 1, **i64)
+        self.img_row = torch.zeros(m, **i64)
+        self.nsite = torch.zeros(max(n, 1), **i64)
+        self.row_rev = torch.zeros(r, **i64)
+        self.img_rev = torch.zeros(m, **i64)
+        flag = torch.zeros(1, **i32)
+        nat.check(L.pxff_nlist_rows(n, nimg, nrows, p(img_start), p(self.img_atom), p(self.img_j), p(self.img_T),
+                                    p(row_start), p(self.seq), p(self.row_img), p(self.img_row), p(self.nsite),
+                                    p(self.row_rev), p(self.img_rev), p(flag), s), "pxff_nlist_rows")
+        fl = int(flag.item())
+        if fl != 0:
+            raise nat.PxffError("neighbour list is not symmetric (flag %d)" % fl)
+
+    # views in the reference's shapes
+    def seq_array(self):
+        return self.seq[: 2 * self.nrows].view(self.nrows, 2)
+
+    def images(self):
+        """(i, j, T, r) per image, ordered by (i, j, T)."""
+        k = self.nimg
+        return (self.img_atom[:k], self.img_j[:k], self.img_T[: 3 * k].view(k, 3), self.img_r[: 3 * k].view(k, 3))
+
+    def counts(self):
+        return (self.img_start[1:] - self.img_start[:-1])

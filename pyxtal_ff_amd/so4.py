@@ -1,0 +1,162 @@
+"""Device SO(4)/SNAP bispectrum operations on a ``DeviceNeighborList``.
+
+Each function enqueues hand-written gfx950 kernels from libpxff.so on the
+current torch stream; tensors are float64 on the GPU.  The math follows the
+reference's ``get_bispectrum_components`` (``pyxtal_ff/descriptors/SNAP.py:
+1289-1762``; ``SO4.py:1256-1730`` is the same with rfac0 = 1 and Z weights).
+"""
+import ctypes
+from functools import lru_cache
+
+import torch
+
+from . import _native as nat
+
+CUTOFF_IDS = {"cosine": 1, "tanh": 2, "poly1": 3, "poly2": 4, "poly3": 5, "poly4": 6, "exp": 7}
+
+
+@lru_cache(maxsize=None)
+def sizes(twol):
+    """dict(nu, nhalf, nz, nb) for 2J = twol."""
+    v = [ctypes.c_int() for _ in range(4)]
+    nat.check(nat.load().pxff_so4_sizes(int(twol), *[ctypes.byref(x) for x in v]), "pxff_so4_sizes")
+    return dict(nu=v[0].value, nhalf=v[1].value, nz=v[2].value, nb=v[3].value)
+
+
+class So4Config:
+    def __init__(self, twol, rcut, rfac0=1.0, cutoff="cosine", normalize_u=False):
+        if isinstance(cutoff, str):
+            cutoff = CUTOFF_IDS[cutoff]
+        self.twol = int(twol)
+        self.rcut = float(rcut)
+        self.rfac0 = float(rfac0)
+        self.cutoff_id = int(cutoff)
+        self.normalize_u = bool(normalize_u)
+        self._p = nat.So4Params(self.twol, self.rcut, self.rfac0, self.cutoff_id, int(self.normalize_u))
+        s = sizes(self.twol)
+        self.nu, self.nhalf, self.nz, self.nb = s["nu"], s["nhalf"], s["nz"], s["nb"]
+
+    @property
+    def params(self):
+        return ctypes.byref(self._p)
+
+
+def _f64(n, dev):
+    return torch.empty(max(int(n), 1), dtype=torch.float64, device=dev)
+
+
+def ui(nl, cfg, atom_w, site_w, work_order=None, img_r=None):
+    """U_tot half rows [N, nhalf] complex (as [N, nhalf, 2] float64)."""
+    p = nat.ptr
+    out = _f64(nl.natoms * cfg.nhalf * 2, nl.pos.device)
+    r = nl.img_r if img_r is None else img_r.contiguous()
+    nat.check(nat.lib().pxff_so4_ui(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r), p(atom_w),
+                                    p(site_w), p(work_order), p(out), nat.stream()), "pxff_so4_ui")
+    return out
+
+
+def zb(nl, cfg, utot, want_b=True):
+    """Z [N, nz, 2] and B [N, nb]."""
+    p = nat.ptr
+    dev = nl.pos.device
+    z = _f64(nl.natoms * cfg.nz * 2, dev)
+    b = _f64(nl.natoms * cfg.nb, dev) if want_b else None
+    nat.check(nat.lib().pxff_so4_zb(cfg.params, nl.natoms, p(utot), p(z), p(b), nat.stream()), "pxff_so4_zb")
+    return z, (b[: nl.natoms * cfg.nb].view(nl.natoms, cfg.nb) if want_b else None)
+
+
+def forward(nl, cfg, atom_w, site_w, work_order=None, img_r=None):
+    """(B [N, nb], Z, U_tot)."""
+    u = ui(nl, cfg, atom_w, site_w, work_order, img_r)
+    z, b = zb(nl, cfg, u)
+    return b, z, u
+
+
+def yi(nl, cfg, z, beta):
+    p = nat.ptr
+    y = _f64(nl.natoms * cfg.nhalf * 2, nl.pos.device)
+    beta = beta.contiguous()
+    nat.check(nat.lib().pxff_so4_yi(cfg.params, nl.natoms, p(z), p(beta), p(y), nat.stream()), "pxff_so4_yi")
+    return y
+
+
+def deidrj(nl, cfg, atom_w, y, img_r=None):
+    """dE/dr per image [nimg, 3]."""
+    p = nat.ptr
+    out = _f64(nl.nimg * 3, nl.pos.device)
+    r = nl.img_r if img_r is None else img_r.contiguous()
+    nat.check(nat.lib().pxff_so4_deidrj(cfg.params, nl.nimg, p(nl.img_atom), p(nl.img_j), p(r), p(atom_w), p(y),
+                                        p(out), nat.stream()), "pxff_so4_deidrj")
+    return out[: nl.nimg * 3].view(nl.nimg, 3)
+
+
+def gather(nl, dedr, virial=False, img_r=None):
+    """dE/dR per atom [N, 3] and (optionally) per-centre virial [N, 3, 3]."""
+    p = nat.ptr
+    dev = nl.pos.device
+    grad = _f64(nl.natoms * 3, dev)
+    vir = _f64(nl.natoms * 9, dev) if virial else None
+    r = nl.img_r if img_r is None else img_r.contiguous()
+    d = dedr.contiguous()
+    nat.check(nat.lib().pxff_so4_gather(nl.natoms, p(nl.img_start), p(nl.img_rev), p(r), p(d), p(grad), p(vir),
+                                        nat.stream()), "pxff_so4_gather")
+    g = grad[: nl.natoms * 3].view(nl.natoms, 3)
+    return (g, vir[: nl.natoms * 9].view(nl.natoms, 3, 3)) if virial else g
+
+
+def tangent(nl, cfg, atom_w, z, dr, img_r=None):
+    """dB [N, nb] = sum_e dB/dr_e . dr_e (forward-mode derivative of B)."""
+    p = nat.ptr
+    dev = nl.pos.device
+    du = _f64(nl.natoms * cfg.nhalf * 2, dev)
+    db = _f64(nl.natoms * cfg.nb, dev)
+    r = nl.img_r if img_r is None else img_r.contiguous()
+    d = dr.contiguous()
+    nat.check(nat.lib().pxff_so4_tangent(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r), p(atom_w),
+                                         p(d), p(z), p(du), p(db), nat.stream()), "pxff_so4_tangent")
+    return db[: nl.natoms * cfg.nb].view(nl.natoms, cfg.nb)
+
+
+def dbdr(nl, cfg, atom_w, z, img_r=None):
+    """Materialised dB/dr per image [nimg, nb, 3]."""
+    p = nat.ptr
+    out = _f64(nl.nimg * cfg.nb * 3, nl.pos.device)
+    r = nl.img_r if img_r is None else img_r.contiguous()
+    nat.check(nat.lib().pxff_so4_dbdr(cfg.params, nl.nimg, p(nl.img_atom), p(nl.img_j), p(r), p(atom_w), p(z),
+                                      p(out), nat.stream()), "pxff_so4_dbdr")
+    return out
+
+
+def rows(nl, nb, dbdr_img, stress=True):
+    """Reference-layout dxdr [nrows, nb, 3] and per-row rdxdr [nrows, nb, 3, 3]
+    (= -bstress / V, SNAP.py:219-238)."""
+    p = nat.ptr
+    dev = nl.pos.device
+    dx = _f64(nl.nrows * nb * 3, dev)
+    rd = _f64(nl.nrows * nb * 9, dev) if stress else None
+    nat.check(nat.lib().pxff_so4_rows(nl.natoms, nb, p(nl.row_start), p(nl.row_img), p(nl.seq), p(nl.nsite),
+                                      p(nl.pos), p(nl.img_r), p(dbdr_img), p(nl.inv_vol), p(dx), p(rd),
+                                      nat.stream()), "pxff_so4_rows")
+    dx = dx[: nl.nrows * nb * 3].view(nl.nrows, nb, 3)
+    if stress:
+        rd = rd[: nl.nrows * nb * 9].view(nl.nrows, nb, 3, 3)
+    return dx, rd
+
+
+def dxdr_contract(nl, nb, beta, dxdr):
+    """dE/dR [N, 3] = sum over rows (i, m) of beta_i . dxdr[(i, m)]."""
+    p = nat.ptr
+    grad = _f64(nl.natoms * 3, nl.pos.device)
+    beta = beta.contiguous()
+    nat.check(nat.lib().pxff_dxdr_contract(nl.natoms, nb, p(nl.row_start), p(nl.seq), p(nl.row_rev), p(beta),
+                                           p(dxdr), p(grad), nat.stream()), "pxff_dxdr_contract")
+    return grad[: nl.natoms * 3].view(nl.natoms, 3)
+
+
+def rdxdr_reduce(nl, nb, rdxdr):
+    """Per-atom Voigt rdxdr [N, nb, 6] (utilities/__init__.py:226-235)."""
+    p = nat.ptr
+    out = _f64(nl.natoms * nb * 6, nl.pos.device)
+    nat.check(nat.lib().pxff_rdxdr_reduce(nl.natoms, nb, p(nl.row_start), p(rdxdr), p(out), nat.stream()),
+              "pxff_rdxdr_reduce")
+    return out[: nl.natoms * nb * 6].view(nl.natoms, nb, 6)
