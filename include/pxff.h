@@ -73,17 +73,24 @@ int pxff_nlist_fill(const double *d_pos, int64_t natoms, const int64_t *d_atom_s
                     int64_t *d_img_atom, int64_t *d_img_j, int32_t *d_img_T, double *d_img_r,
                     int64_t *d_row_count, void *stream);
 
-/* Phase 3: seq rows.  d_row_start [natoms+1] int64 = exclusive scan of
- * d_row_count, nrows = d_row_start[natoms].  Outputs: d_seq [nrows*2] int64
- * (the reference's seq), d_row_img [nrows+1] int64 (images of row r are
- * [row_img[r], row_img[r+1])), d_img_row [nimg] int64, d_nsite [natoms]
- * int64 (row of (i,i)), d_row_rev [nrows] int64 (row (j,i) of row (i,j)),
- * d_img_rev [nimg] int64 (image (j,i,-T) of image (i,j,T)).  A missing
- * reverse is reported through d_flag[0] != 0 (int32, zeroed by the call). */
+/* Phase 3: reverse images.  d_img_rev [nimg] int64 = index of image
+ * (j, i, -T) for image (i, j, T) -- the list is exactly symmetric, which
+ * the deterministic force gather relies on.  A missing reverse (cannot
+ * happen for this construction) sets d_flag[0] (int32, zeroed by the call). */
+int pxff_nlist_rev(int64_t natoms, int64_t nimg, const int64_t *d_img_start, const int64_t *d_img_atom,
+                   const int64_t *d_img_j, const int32_t *d_img_T, int64_t *d_img_rev, int32_t *d_flag,
+                   void *stream);
+
+/* Phase 4 (only for the reference's row layout): seq rows.  d_row_start
+ * [natoms+1] int64 = exclusive scan of d_row_count, nrows =
+ * d_row_start[natoms].  Outputs: d_seq [nrows*2] int64 (the reference's
+ * seq), d_row_img [nrows+1] int64 (images of row r are [row_img[r],
+ * row_img[r+1])), d_img_row [nimg] int64, d_nsite [natoms] int64 (row of
+ * (i,i)), d_row_rev [nrows] int64 (row (j,i) of row (i,j)); a missing
+ * reverse row sets d_flag[0]. */
 int pxff_nlist_rows(int64_t natoms, int64_t nimg, int64_t nrows, const int64_t *d_img_start,
-                    const int64_t *d_img_atom, const int64_t *d_img_j, const int32_t *d_img_T,
-                    const int64_t *d_row_start, int64_t *d_seq, int64_t *d_row_img, int64_t *d_img_row,
-                    int64_t *d_nsite, int64_t *d_row_rev, int64_t *d_img_rev, int32_t *d_flag, void *stream);
+                    const int64_t *d_img_j, const int64_t *d_row_start, int64_t *d_seq, int64_t *d_row_img,
+                    int64_t *d_img_row, int64_t *d_nsite, int64_t *d_row_rev, int32_t *d_flag, void *stream);
 
 /* Exclusive scan: d_out [n+1], d_out[n] = total. */
 int pxff_scan_i64(const int64_t *d_in, int64_t n, int64_t *d_out, void *stream);
@@ -169,6 +176,53 @@ int pxff_dxdr_contract(int64_t natoms, int nb, const int64_t *d_row_start, const
  * Voigt (xx, yy, zz, xy, xz, yz) of the sum of the centre's rows. */
 int pxff_rdxdr_reduce(int64_t natoms, int nb, const int64_t *d_row_start, const double *d_rdxdr, double *d_out,
                       void *stream);
+
+/* ------------------------------------------------------------------ *
+ * SO(3) power spectrum (replaces SO3.calculate / compute_cs /          *
+ * compute_dcs, SO3.py:187-291, 504-694).                               *
+ * Host-precomputed: d_G [nmax*nquad] = g_n(r_q) (pi/NQ)(rc/2) r_q^2     *
+ * exp(-alpha r_q^2) sqrt(1 - x_q^2) (SO3.py:466-502, 589-617), d_rq     *
+ * [nquad] = rc (x_q + 1)/2, d_lnorm [lmax+1] = sqrt(2 sqrt2 pi /        *
+ * sqrt(2l+1)) (SO3.py:209-210).  d_atom_z [natoms] int64 = atomic        *
+ * numbers (neighbour weight Z_j, sign-flipped with weight_on when        *
+ * Z_j != Z_i, SO3.py:349-354).                                          *
+ * ctot layout per atom: [nmax][lmax+1][2 lmax+1] complex (m + lmax).    *
+ * x layout: [ntri(n>=n') * (lmax+1) + l] (SO3.py:259).                  *
+ * ------------------------------------------------------------------ */
+typedef struct {
+    int nmax;      /* 1..11 */
+    int lmax;      /* 0..32 (device path: nmax*(lmax+1)*(2lmax+1) <= 4096) */
+    double rcut;
+    double alpha;
+    int cutoff_id; /* 1 cosine, 2 tanh, 3..6 poly1..4, 7 exp, 8 unity */
+    int nquad;     /* 10*(nmax+lmax+1) (SO3.py:495-502) */
+    int weight_on;
+} pxff_so3_params;
+
+/* Per-centre expansion ctot [natoms*csz*2] (sum over the centre's images);
+ * with d_dr [nimg*3] non-NULL, its tangent sum_e dc_e/dr_e . dr_e. */
+int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_j,
+                  const double *d_img_r, const int64_t *d_atom_z, const double *d_G, const double *d_rq,
+                  const double *d_lnorm, const double *d_dr, double *d_out, void *stream);
+
+/* Power spectrum x [natoms*ncoef]; with d_dctot non-NULL the tangent of x. */
+int pxff_so3_power(const pxff_so3_params *p, int64_t natoms, const double *d_ctot, const double *d_dctot,
+                   double *d_x, void *stream);
+
+/* Adjoint: A [natoms*csz*2] = S(dE/dx) . ctot. */
+int pxff_so3_adjoint_a(const pxff_so3_params *p, int64_t natoms, const double *d_ctot, const double *d_beta,
+                       double *d_a, void *stream);
+
+/* Adjoint: dE/dr per image [nimg*3] (expansion recomputed per image). */
+int pxff_so3_deidrj(const pxff_so3_params *p, int64_t nimg, const int64_t *d_img_atom, const int64_t *d_img_j,
+                    const double *d_img_r, const int64_t *d_atom_z, const double *d_G, const double *d_rq,
+                    const double *d_lnorm, const double *d_a, double *d_dedr, void *stream);
+
+/* Plain path: dP/dr per image [nimg*ncoef*3] (then pxff_so4_rows builds the
+ * reference's dxdr / rdxdr rows, whose row rules SO3.py:243-265 shares). */
+int pxff_so3_dpdr(const pxff_so3_params *p, int64_t nimg, const int64_t *d_img_atom, const int64_t *d_img_j,
+                  const double *d_img_r, const int64_t *d_atom_z, const double *d_G, const double *d_rq,
+                  const double *d_lnorm, const double *d_ctot, double *d_dpdr, void *stream);
 
 #ifdef __cplusplus
 }

@@ -30,7 +30,8 @@ class So4Params(ctypes.Structure):
 
 class So3Params(ctypes.Structure):
     _fields_ = [("nmax", ctypes.c_int), ("lmax", ctypes.c_int), ("rcut", ctypes.c_double),
-                ("alpha", ctypes.c_double), ("cutoff_id", ctypes.c_int), ("nquad", ctypes.c_int)]
+                ("alpha", ctypes.c_double), ("cutoff_id", ctypes.c_int), ("nquad", ctypes.c_int),
+                ("weight_on", ctypes.c_int)]
 
 
 _SIGS = {
@@ -39,7 +40,8 @@ _SIGS = {
     "pxff_scan_i64": ([P, I64, P, P], I32),
     "pxff_nlist_count": ([P, I64, P, P, P, P, P, P, I64, F64, P, P, P, P, P, P, P], I32),
     "pxff_nlist_fill": ([P, I64, P, P, P, P, P, F64, P, P, P, P, P, P, P, P, P, P, P], I32),
-    "pxff_nlist_rows": ([I64, I64, I64, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "pxff_nlist_rev": ([I64, I64, P, P, P, P, P, P, P], I32),
+    "pxff_nlist_rows": ([I64, I64, I64, P, P, P, P, P, P, P, P, P, P], I32),
     "pxff_so4_sizes": ([ctypes.c_int, P, P, P, P], I32),
     "pxff_so4_ui": ([P, I64, P, P, P, P, P, P, P, P], I32),
     "pxff_so4_zb": ([P, I64, P, P, P, P], I32),
@@ -51,6 +53,11 @@ _SIGS = {
     "pxff_so4_rows": ([I64, ctypes.c_int, P, P, P, P, P, P, P, P, P, P, P], I32),
     "pxff_dxdr_contract": ([I64, ctypes.c_int, P, P, P, P, P, P, P], I32),
     "pxff_rdxdr_reduce": ([I64, ctypes.c_int, P, P, P, P], I32),
+    "pxff_so3_ctot": ([P, I64, P, P, P, P, P, P, P, P, P, P], I32),
+    "pxff_so3_power": ([P, I64, P, P, P, P], I32),
+    "pxff_so3_adjoint_a": ([P, I64, P, P, P, P], I32),
+    "pxff_so3_deidrj": ([P, I64, P, P, P, P, P, P, P, P, P, P], I32),
+    "pxff_so3_dpdr": ([P, I64, P, P, P, P, P, P, P, P, P, P], I32),
 }
 
 OPTIONAL = set()

@@ -78,18 +78,40 @@ def bispectrum(r, state):
     return _BFromR.apply(r, state)
 
 
+class _Separations(torch.autograd.Function):
+    """r_e = pos[j] - pos[i] + (lattice shift of e); backward
+    dE/dR_i = sum_{e of i} (dE/dr_rev(e) - dE/dr_e) by the deterministic
+    gather kernel (pxff_so4_gather) instead of scatter-adds."""
+
+    @staticmethod
+    def forward(ctx, pos, nl):
+        ctx.nl = nl
+        k = nl.nimg
+        if pos.data_ptr() == nl.pos.data_ptr():
+            return nl.img_r[: 3 * k].view(k, 3).clone()
+        i = nl.img_atom[:k]
+        j = nl.img_j[:k]
+        return pos.index_select(0, j) - pos.index_select(0, i) + _shift(nl)
+
+    @staticmethod
+    def backward(ctx, g):
+        return dev_so4.gather(ctx.nl, g.contiguous()), None
+
+
+def _shift(nl):
+    if getattr(nl, "_shift", None) is None:
+        k = nl.nimg
+        base = nl.img_r[: 3 * k].view(k, 3)
+        nl._shift = base - (nl.pos.index_select(0, nl.img_j[:k]) - nl.pos.index_select(0, nl.img_atom[:k]))
+    return nl._shift
+
+
 def separations(nl, pos, strain=None):
     """r_e = pos[j] - pos[i] + T.cell (+ r_e . strain) as a differentiable
     function of positions (and of a homogeneous strain, whose gradient is
-    the virial sum_e r_e (x) dE/dr_e)."""
-    k = nl.nimg
-    i = nl.img_atom[:k]
-    j = nl.img_j[:k]
-    base = nl.img_r[: 3 * k].view(k, 3)
-    if not hasattr(nl, "_shift"):
-        # lattice part of each separation, fixed for the image set
-        nl._shift = base - (nl.pos.index_select(0, j) - nl.pos.index_select(0, i))
-    r = pos.index_select(0, j) - pos.index_select(0, i) + nl._shift
+    the virial sum_e r_e (x) dE/dr_e).  With pos equal to the positions the
+    list was built from, r_e are the list's (bit-exact) separations."""
+    r = _Separations.apply(pos, nl)
     if strain is not None:
         r = r + r @ strain
     return r

@@ -154,19 +154,6 @@ __device__ inline void nl_enumerate(int64_t i, const double *__restrict__ pos, c
     }
 }
 
-__global__ void nl_count_kernel(int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
-                                const double *__restrict__ cells, const int32_t *__restrict__ pbc,
-                                const int32_t *__restrict__ geom, double rc2, const int32_t *__restrict__ bin_start,
-                                const int32_t *__restrict__ atom_bin, const int32_t *__restrict__ wrap,
-                                const int32_t *__restrict__ bin_atoms, int64_t *__restrict__ img_count) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natoms) return;
-    int64_t c = 0;
-    nl_enumerate(i, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap, bin_atoms,
-                 [&](int64_t, int, int, int, const double *) { c++; });
-    img_count[i] = c;
-}
-
 __device__ inline bool key_less(int64_t ja, const int32_t *Ta, int64_t jb, const int32_t *Tb) {
     if (ja != jb) return ja < jb;
     if (Ta[0] != Tb[0]) return Ta[0] < Tb[0];
@@ -180,10 +167,11 @@ __global__ void nl_fill_kernel(int64_t natoms, const double *__restrict__ pos, c
                                const int32_t *__restrict__ atom_bin, const int32_t *__restrict__ wrap,
                                const int32_t *__restrict__ bin_atoms, const int64_t *__restrict__ img_start,
                                int64_t *__restrict__ img_atom, int64_t *__restrict__ img_j, int32_t *__restrict__ img_T,
-                               double *__restrict__ img_r, int64_t *__restrict__ row_count) {
+                               double *__restrict__ img_r, int64_t *__restrict__ row_count, int64_t min_count) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natoms) return;
     const int64_t s0 = img_start[i];
+    if (img_start[i + 1] - s0 < min_count) return;
     int64_t c = s0;
     nl_enumerate(i, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap, bin_atoms,
                  [&](int64_t j, int T0, int T1, int T2, const double *r) {
@@ -311,6 +299,188 @@ __global__ void nl_row_rev_kernel(int64_t nrows, const int64_t *__restrict__ seq
     }
 }
 
+// ------------------------------------------------ wave-per-centre build
+// One workgroup = one wavefront = one centre.  The stencil bins are taken 64
+// at a time: each lane owns one stencil bin, an exclusive wave prefix sum of
+// the bin sizes flattens the candidates (bin, slot), and every lane tests
+// one candidate per step (the owner of candidate c is the last bin whose
+// prefix is <= c).  Hits are compacted with a ballot into an LDS list; the
+// fill pass ranks the list by (j, T) (rank = number of smaller keys) and
+// writes it in order.  Centres with more than NL_KMAX images are left to
+// the thread-per-centre kernel above (nl_fill_kernel with a count floor).
+constexpr int NL_KMAX = 512;
+
+struct NlHit {
+    int64_t j;
+    int32_t T[4];
+    double r[3];
+};
+
+template <bool FILL>
+__device__ inline int nl_wave_enumerate(int64_t i, int lane, const double *__restrict__ pos,
+                                        const int64_t *__restrict__ atom_struct, const double *__restrict__ cells,
+                                        const int32_t *__restrict__ pbc, const int32_t *__restrict__ geom, double rc2,
+                                        const int32_t *__restrict__ bin_start, const int32_t *__restrict__ atom_bin,
+                                        const int32_t *__restrict__ wrap, const int32_t *__restrict__ bin_atoms,
+                                        int *s_pref, int *s_info, NlHit *hits) {
+    const int64_t s = atom_struct[i];
+    const double *cell = cells + 9 * s;
+    const int32_t *gm = geom + 8 * s;
+    const int nb0 = gm[0], nb1 = gm[1], nb2 = gm[2];
+    const int S0 = gm[3], S1 = gm[4], S2 = gm[5];
+    const int off = gm[6];
+    const int lb = atom_bin[i] - off;
+    const int bi0 = lb / (nb1 * nb2), bi1 = (lb / nb2) % nb1, bi2 = lb % nb2;
+    const int ki0 = wrap[3 * i], ki1 = wrap[3 * i + 1], ki2 = wrap[3 * i + 2];
+    const bool p0 = pbc[3 * s], p1 = pbc[3 * s + 1], p2 = pbc[3 * s + 2];
+    const int w0 = 2 * S0 + 1, w1 = 2 * S1 + 1, w2 = 2 * S2 + 1;
+    const int nsten = w0 * w1 * w2;
+    int count = 0;
+    for (int sb = 0; sb < nsten; sb += 64) {
+        const int st = sb + lane;
+        int gb = -1, t0 = 0, t1 = 0, t2 = 0, sz = 0;
+        if (st < nsten) {
+            const int d0 = st / (w1 * w2) - S0, d1 = (st / w2) % w1 - S1, d2 = st % w2 - S2;
+            int tb0 = bi0 + d0, tb1 = bi1 + d1, tb2 = bi2 + d2;
+            bool ok = true;
+            if (p0) { t0 = floordiv_i(tb0, nb0); tb0 -= t0 * nb0; } else { ok = ok && tb0 >= 0 && tb0 < nb0; }
+            if (p1) { t1 = floordiv_i(tb1, nb1); tb1 -= t1 * nb1; } else { ok = ok && tb1 >= 0 && tb1 < nb1; }
+            if (p2) { t2 = floordiv_i(tb2, nb2); tb2 -= t2 * nb2; } else { ok = ok && tb2 >= 0 && tb2 < nb2; }
+            if (ok) {
+                gb = off + (tb0 * nb1 + tb1) * nb2 + tb2;
+                sz = bin_start[gb + 1] - bin_start[gb];
+            }
+        }
+        int incl = sz;
+        for (int o = 1; o < 64; o <<= 1) {
+            int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        const int total = __shfl(incl, 63);
+        __syncthreads();  // previous chunk's readers are done with s_pref / s_info
+        s_pref[lane] = incl - sz;
+        s_info[4 * lane] = gb;
+        s_info[4 * lane + 1] = t0;
+        s_info[4 * lane + 2] = t1;
+        s_info[4 * lane + 3] = t2;
+        __syncthreads();
+        for (int cb = 0; cb < total; cb += 64) {
+            const int c = cb + lane;
+            bool hit = false;
+            int64_t j = 0;
+            int T0 = 0, T1 = 0, T2 = 0;
+            double r[3] = {0, 0, 0};
+            if (c < total) {
+                int lo = 0, hi = 63;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pref[mid] <= c)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                const int g = s_info[4 * lo];
+                j = bin_atoms[bin_start[g] + (c - s_pref[lo])];
+                T0 = s_info[4 * lo + 1] - wrap[3 * j] + ki0;
+                T1 = s_info[4 * lo + 2] - wrap[3 * j + 1] + ki1;
+                T2 = s_info[4 * lo + 3] - wrap[3 * j + 2] + ki2;
+                if (!(j == i && T0 == 0 && T1 == 0 && T2 == 0)) {
+                    canon_sep(pos, cell, i, j, T0, T1, T2, r);
+                    hit = dist2(r) < rc2;
+                }
+            }
+            const uint64_t mask = __ballot(hit);
+            if (FILL && hit) {
+                const int rank = count + (int)__popcll(mask & ((1ull << lane) - 1ull));
+                if (rank < NL_KMAX) {
+                    NlHit &h = hits[rank];
+                    h.j = j;
+                    h.T[0] = T0;
+                    h.T[1] = T1;
+                    h.T[2] = T2;
+                    h.r[0] = r[0];
+                    h.r[1] = r[1];
+                    h.r[2] = r[2];
+                }
+            }
+            count += (int)__popcll(mask);
+        }
+    }
+    return count;
+}
+
+__global__ __launch_bounds__(64) void nl_count_wave_kernel(
+    int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
+    const double *__restrict__ cells, const int32_t *__restrict__ pbc, const int32_t *__restrict__ geom, double rc2,
+    const int32_t *__restrict__ bin_start, const int32_t *__restrict__ atom_bin, const int32_t *__restrict__ wrap,
+    const int32_t *__restrict__ bin_atoms, int64_t *__restrict__ img_count) {
+    __shared__ int s_pref[64];
+    __shared__ int s_info[256];
+    const int lane = threadIdx.x;
+    const int64_t i = blockIdx.x;
+    if (i >= natoms) return;  // uniform for the whole workgroup
+    const int c = nl_wave_enumerate<false>(i, lane, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap,
+                                           bin_atoms, s_pref, s_info, nullptr);
+    if (lane == 0) img_count[i] = c;
+}
+
+__device__ inline bool hit_less(const NlHit &a, const NlHit &b) {
+    if (a.j != b.j) return a.j < b.j;
+    if (a.T[0] != b.T[0]) return a.T[0] < b.T[0];
+    if (a.T[1] != b.T[1]) return a.T[1] < b.T[1];
+    return a.T[2] < b.T[2];
+}
+
+__global__ __launch_bounds__(64) void nl_fill_wave_kernel(
+    int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
+    const double *__restrict__ cells, const int32_t *__restrict__ pbc, const int32_t *__restrict__ geom, double rc2,
+    const int32_t *__restrict__ bin_start, const int32_t *__restrict__ atom_bin, const int32_t *__restrict__ wrap,
+    const int32_t *__restrict__ bin_atoms, const int64_t *__restrict__ img_start, int64_t *__restrict__ img_atom,
+    int64_t *__restrict__ img_j, int32_t *__restrict__ img_T, double *__restrict__ img_r,
+    int64_t *__restrict__ row_count) {
+    __shared__ int s_pref[64];
+    __shared__ int s_info[256];
+    __shared__ NlHit hits[NL_KMAX];
+    const int lane = threadIdx.x;
+    const int64_t i = blockIdx.x;
+    if (i >= natoms) return;
+    const int64_t s0 = img_start[i];
+    const int k = (int)(img_start[i + 1] - s0);
+    if (k > NL_KMAX) return;  // handled by nl_fill_kernel (overflow pass)
+    const int c = nl_wave_enumerate<true>(i, lane, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap,
+                                          bin_atoms, s_pref, s_info, hits);
+    (void)c;
+    __syncthreads();
+    int firsts = 0;
+    bool self = false;
+    for (int e = lane; e < k; e += 64) {
+        const NlHit h = hits[e];
+        int rank = 0;
+        bool first = true;
+        for (int q = 0; q < k; q++) {
+            const NlHit &o = hits[q];
+            if (hit_less(o, h)) {
+                rank++;
+                if (o.j == h.j) first = false;
+            }
+        }
+        const int64_t d = s0 + rank;
+        img_atom[d] = i;
+        img_j[d] = h.j;
+        img_T[3 * d] = h.T[0];
+        img_T[3 * d + 1] = h.T[1];
+        img_T[3 * d + 2] = h.T[2];
+        img_r[3 * d] = h.r[0];
+        img_r[3 * d + 1] = h.r[1];
+        img_r[3 * d + 2] = h.r[2];
+        if (first) firsts++;
+        if (h.j == i) self = true;
+    }
+    for (int o = 32; o > 0; o >>= 1) firsts += __shfl_xor(firsts, o);
+    const bool any_self = __ballot(self) != 0;
+    if (lane == 0) row_count[i] = firsts + (any_self ? 0 : 1);
+}
+
 // ------------------------------------------------------------ device scan
 // Exclusive scan by one workgroup of 1024 threads (chunked).  out has n+1
 // entries; out[n] = total.
@@ -369,8 +539,8 @@ extern "C" int pxff_nlist_count(const double *d_pos, int64_t natoms, const int64
     PXFF_HIP_CHECK(hipMemsetAsync(d_bin_count, 0, sizeof(int32_t) * nbins_total, s));
     nl_bin_fill_kernel<<<div_up(natoms, 256), 256, 0, s>>>(natoms, d_atom_bin, d_bin_start, d_bin_count, d_bin_atoms);
     PXFF_LAUNCH_CHECK();
-    nl_count_kernel<<<div_up(natoms, 64), 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom, rc * rc,
-                                                      d_bin_start, d_atom_bin, d_wrap, d_bin_atoms, d_img_count);
+    nl_count_wave_kernel<<<(unsigned)natoms, 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom, rc * rc,
+                                                           d_bin_start, d_atom_bin, d_wrap, d_bin_atoms, d_img_count);
     PXFF_LAUNCH_CHECK();
     return 0;
 }
@@ -384,17 +554,34 @@ extern "C" int pxff_nlist_fill(const double *d_pos, int64_t natoms, const int64_
     (void)d_atom_start;
     if (natoms <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
+    nl_fill_wave_kernel<<<(unsigned)natoms, 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom, rc * rc,
+                                                          d_bin_start, d_atom_bin, d_wrap, d_bin_atoms, d_img_start,
+                                                          d_img_atom, d_img_j, d_img_T, d_img_r, d_row_count);
+    PXFF_LAUNCH_CHECK();
+    // centres with more images than the LDS list holds
     nl_fill_kernel<<<div_up(natoms, 64), 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom, rc * rc,
                                                      d_bin_start, d_atom_bin, d_wrap, d_bin_atoms, d_img_start,
-                                                     d_img_atom, d_img_j, d_img_T, d_img_r, d_row_count);
+                                                     d_img_atom, d_img_j, d_img_T, d_img_r, d_row_count,
+                                                     (int64_t)NL_KMAX + 1);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_nlist_rev(int64_t natoms, int64_t nimg, const int64_t *d_img_start, const int64_t *d_img_atom,
+                              const int64_t *d_img_j, const int32_t *d_img_T, int64_t *d_img_rev, int32_t *d_flag,
+                              void *stream) {
+    if (natoms <= 0 || nimg <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    PXFF_HIP_CHECK(hipMemsetAsync(d_flag, 0, sizeof(int32_t), s));
+    nl_rev_kernel<<<div_up(nimg, 256), 256, 0, s>>>(natoms, nimg, d_img_start, d_img_atom, d_img_j, d_img_T, d_img_rev,
+                                                    d_flag);
     PXFF_LAUNCH_CHECK();
     return 0;
 }
 
 extern "C" int pxff_nlist_rows(int64_t natoms, int64_t nimg, int64_t nrows, const int64_t *d_img_start,
-                               const int64_t *d_img_atom, const int64_t *d_img_j, const int32_t *d_img_T,
-                               const int64_t *d_row_start, int64_t *d_seq, int64_t *d_row_img, int64_t *d_img_row,
-                               int64_t *d_nsite, int64_t *d_row_rev, int64_t *d_img_rev, int32_t *d_flag,
+                               const int64_t *d_img_j, const int64_t *d_row_start, int64_t *d_seq, int64_t *d_row_img,
+                               int64_t *d_img_row, int64_t *d_nsite, int64_t *d_row_rev, int32_t *d_flag,
                                void *stream) {
     if (natoms <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
@@ -402,11 +589,6 @@ extern "C" int pxff_nlist_rows(int64_t natoms, int64_t nimg, int64_t nrows, cons
     nl_rows_kernel<<<div_up(natoms, 64), 64, 0, s>>>(natoms, nimg, d_img_start, d_img_j, d_row_start, d_seq, d_row_img,
                                                      d_img_row, d_nsite);
     PXFF_LAUNCH_CHECK();
-    if (nimg > 0) {
-        nl_rev_kernel<<<div_up(nimg, 256), 256, 0, s>>>(natoms, nimg, d_img_start, d_img_atom, d_img_j, d_img_T,
-                                                        d_img_rev, d_flag);
-        PXFF_LAUNCH_CHECK();
-    }
     nl_row_rev_kernel<<<div_up(nrows, 256), 256, 0, s>>>(nrows, d_seq, d_row_start, d_row_rev, d_flag);
     PXFF_LAUNCH_CHECK();
     return 0;

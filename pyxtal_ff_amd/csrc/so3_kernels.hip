@@ -1,0 +1,735 @@
+// SO(3) power spectrum kernels for gfx950 (FP64, wave64).
+//
+// Reference: pyxtal_ff/descriptors/SO3.py:187-291 (calculate) and
+// SO3.py:466-694 (W, g, Gauss-Chebyshev quadrature, compute_cs/compute_dcs).
+// Per neighbour image e with separation r (|r| = R) and weight w_e (Z_j,
+// sign-flipped when weight_on and Z_j != Z_i), for l <= lmax, n < nmax:
+//   I_nl(R)  = sum_q G[n][q] i_l(2 alpha R r_q)
+//   dI_nl(R) = sum_q G[n][q] 2 alpha r_q i_l'(2 alpha R r_q)
+//   c_nlm    = norm_l w_e 4 pi exp(-alpha R^2) fc(R) I_nl(R) Y_lm(rhat)
+// with G[n][q] = g_n(r_q) (pi/NQ)(rc/2) r_q^2 exp(-alpha r_q^2) sqrt(1-x_q^2)
+// precomputed on the host, Y_lm with the Condon-Shortley phase (scipy's
+// sph_harm, not conjugated) and grad Y_lm from the covariant components of
+// SO3.py:649-674.
+//
+// Work mapping: a group of L = lmax+1 lanes serves one image, lane l owns
+// angular order l: it evaluates i_l and i_l' at the NQ quadrature nodes
+// (positive power series below a threshold, upward recurrence above it,
+// both stable there), contracts them with G into NMAX registers, and reads
+// the group's Y table from LDS.  Per-centre sums (ctot, its tangent) use
+// per-group LDS partials reduced in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "pxff_common.h"
+
+namespace pxff {
+
+constexpr int SO3_LMAXC = 32;  // SO3.py:95-101 allows lmax <= 32
+
+// i_l(x) and i_{l-1}(x) (i_1 when l == 0: needed for i_0' = i_1).
+__device__ inline void sph_bessel_i_pair(int l, double x, double &il, double &ilm1) {
+    const double thr = 16.0 + 0.5 * l * (l + 1);
+    if (x < thr) {
+        // i_n(x) = x^n / (2n+1)!! sum_k (x^2/2)^k / (k! (2n+3)(2n+5)...(2n+2k+1))
+        const int n2 = (l == 0) ? 1 : l - 1;
+        double v[2];
+        const int ns[2] = {l, n2};
+        const double h = 0.5 * x * x;
+        for (int s = 0; s < 2; s++) {
+            const int n = ns[s];
+            double pre = 1.0;
+            for (int k = 1; k <= n; k++) pre *= x / (2 * k + 1);
+            double t = 1.0, sum = 1.0;
+            for (int k = 1; k < 400; k++) {
+                t *= h / (k * (2.0 * n + 2 * k + 1));
+                sum += t;
+                if (t < 1e-17 * sum) break;
+            }
+            v[s] = pre * sum;
+        }
+        il = v[0];
+        ilm1 = v[1];
+        return;
+    }
+    const double e = exp(x), ei = 1.0 / e;
+    const double sh = 0.5 * (e - ei), ch = 0.5 * (e + ei);
+    double i0 = sh / x;
+    double i1 = (x * ch - sh) / (x * x);
+    if (l == 0) {
+        il = i0;
+        ilm1 = i1;
+        return;
+    }
+    double a = i0, b = i1;
+    for (int k = 1; k < l; k++) {
+        double c = a - (2 * k + 1) / x * b;
+        a = b;
+        b = c;
+    }
+    il = b;
+    ilm1 = a;
+}
+
+__device__ inline void so3_cutoff(int id, double r, double rc, double &f, double &df) {
+    if (id == 8) {  // 'unity': the reference returns ones for the value and the derivative (SO3.py:459-464)
+        f = 1.0;
+        df = 1.0;
+        return;
+    }
+    double x = r / rc;
+    switch (id) {
+    case 1: {
+        double s, c;
+        sincos(M_PI * r / rc, &s, &c);
+        f = 0.5 * (c + 1.0);
+        df = -0.5 * M_PI / rc * s;
+        return;
+    }
+    case 2: {
+        double t = tanh(1.0 - x), t2 = t * t;
+        f = t2 * t;
+        df = -(3.0 / rc) * t2 * (1.0 - t2);
+        return;
+    }
+    case 3:
+        f = x * x * (2 * x - 3) + 1;
+        df = (6.0 / (rc * rc)) * r * (x - 1.0);
+        return;
+    case 4:
+        f = x * x * x * (x * (15 - 6 * x) - 10) + 1;
+        df = (-30.0 / rc) * (x * x * (x - 1) * (x - 1));
+        return;
+    case 5:
+        f = x * x * x * x * (x * (x * (20 * x - 70) + 84) - 35) + 1;
+        df = (140.0 / rc) * (x * x * x * (x - 1) * (x - 1) * (x - 1));
+        return;
+    case 6: {
+        f = x * x * x * x * x * (x * (x * (x * (315 - 70 * x) - 540) + 420) - 126) + 1;
+        double y = x * (x - 1);
+        df = (-630.0 / rc) * y * y * y * y;
+        return;
+    }
+    case 7: {
+        double e = exp(1.0 - 1.0 / (1.0 - x * x)), d = 1 + x * x;
+        f = e;
+        df = 2 * x * e / (d * d);
+        return;
+    }
+    default:
+        f = 0.0;
+        df = 0.0;
+    }
+}
+
+// Y_l^m for l <= L (one value of l per call), m = -l..l, written to
+// Y[l*(2*LP+1) + LP + m] (LP = row stride half-width).  Normalised associated
+// Legendre recurrence with the Condon-Shortley phase.
+__device__ inline void ylm_row(int l, double t, double s, cd eiphi, cd *Y, int LP) {
+    const double inv4pi = 0.25 / M_PI;
+    cd em = cmake(1.0, 0.0);  // e^{i m phi}
+    double pmm = sqrt(inv4pi);  // Pbar_m^m, m = 0
+    for (int m = 0; m <= l; m++) {
+        if (m > 0) {
+            pmm = -sqrt((2.0 * m + 1.0) / (2.0 * m)) * s * pmm;
+            em = cmul(em, eiphi);
+        }
+        double p;
+        if (l == m) {
+            p = pmm;
+        } else {
+            double pm2 = pmm;
+            double pm1 = sqrt(2.0 * m + 3.0) * t * pmm;
+            for (int k = m + 2; k <= l; k++) {
+                double a = sqrt((4.0 * k * k - 1.0) / ((double)k * k - (double)m * m));
+                double b = sqrt(((k - 1.0) * (k - 1.0) - (double)m * m) / (4.0 * (k - 1.0) * (k - 1.0) - 1.0));
+                double pk = a * (t * pm1 - b * pm2);
+                pm2 = pm1;
+                pm1 = pk;
+            }
+            p = pm1;
+        }
+        cd y = cscale(em, p);
+        Y[l * (2 * LP + 1) + LP + m] = y;
+        if (m > 0) {
+            // Y_l^{-m} = (-1)^m conj(Y_l^m)
+            double sg = (m & 1) ? -1.0 : 1.0;
+            Y[l * (2 * LP + 1) + LP - m] = cmake(sg * y.re, -sg * y.im);
+        }
+    }
+}
+
+struct So3Args {
+    int nmax, lmax, nq, cutoff_id, weight_on;
+    double rc, alpha;
+    const double *G;      // [nmax][nq]
+    const double *rq;     // [nq]
+    const double *lnorm;  // [lmax+1]
+};
+
+// Radial part of lane (image, l): I[n], dI[n] for n < NMAX.
+template <int NMAX>
+__device__ inline void radial(const So3Args &a, int l, double R, const double *Gs, const double *rqs, double *I,
+                              double *dI, bool want_d) {
+#pragma unroll
+    for (int n = 0; n < NMAX; n++) {
+        I[n] = 0.0;
+        dI[n] = 0.0;
+    }
+    const double ta = 2.0 * a.alpha;
+    for (int q = 0; q < a.nq; q++) {
+        const double rq = rqs[q];
+        const double x = ta * R * rq;
+        double il, ilm1;
+        sph_bessel_i_pair(l, x, il, ilm1);
+#pragma unroll
+        for (int n = 0; n < NMAX; n++) I[n] += Gs[n * a.nq + q] * il;
+        if (want_d) {
+            // scipy spherical_in(derivative=True): i_0' = i_1; i_l' = i_{l-1} - (l+1)/x i_l
+            const double dil = (l == 0) ? ilm1 : ilm1 - (l + 1) / x * il;
+            const double f = ta * rq * dil;
+#pragma unroll
+            for (int n = 0; n < NMAX; n++) dI[n] += Gs[n * a.nq + q] * f;
+        }
+    }
+}
+
+// Covariant gradient of Y_l^m (SO3.py:649-674), from the group's Y table
+// (rows 0..lmax+1, stride 2*LP+1).
+__device__ inline void grad_ylm(int l, int m, double R, const cd *Y, int LP, cd g[3]) {
+    if (l == 0) {
+        g[0] = g[1] = g[2] = cmake(0, 0);
+        return;
+    }
+    auto y = [&](int ll, int mm) -> cd { return Y[ll * (2 * LP + 1) + LP + mm]; };
+    const double dl = l, dm = m;
+    cd x0 = cscale(y(l + 1, m), -sqrt(((dl + 1) * (dl + 1) - dm * dm) / (2 * dl + 1) / (2 * dl + 3)) * dl / R);
+    if (abs(m) <= l - 1)
+        x0 = cadd(x0, cscale(y(l - 1, m), sqrt((dl * dl - dm * dm) / (2 * dl - 1) / (2 * dl + 1)) * (dl + 1) / R));
+    cd xp = cscale(y(l + 1, m + 1), -sqrt((dl + dm + 1) * (dl + dm + 2) / 2 / (2 * dl + 1) / (2 * dl + 3)) * dl / R);
+    if (abs(m + 1) <= l - 1)
+        xp = csub(xp, cscale(y(l - 1, m + 1), sqrt((dl - dm - 1) * (dl - dm) / 2 / (2 * dl - 1) / (2 * dl + 1)) * (dl + 1) / R));
+    cd xm = cscale(y(l + 1, m - 1), -sqrt((dl - dm + 1) * (dl - dm + 2) / 2 / (2 * dl + 1) / (2 * dl + 3)) * dl / R);
+    if (abs(m - 1) <= l - 1)
+        xm = csub(xm, cscale(y(l - 1, m - 1), sqrt((dl + dm - 1) * (dl + dm) / 2 / (2 * dl - 1) / (2 * dl + 1)) * (dl + 1) / R));
+    const double r2 = 1.0 / sqrt(2.0);
+    g[0] = cscale(csub(xm, xp), r2);
+    cd s = cadd(xm, xp);
+    g[1] = cmake(-s.im * r2, s.re * r2);  // i/sqrt2 * (xm + xp)
+    g[2] = x0;
+}
+
+enum { SO3_CTOT = 0, SO3_TANGENT = 1, SO3_ADJOINT = 2, SO3_PLAIN = 3 };
+
+// One image's per-lane context.
+struct So3Img {
+    bool valid;
+    int64_t e, i;
+    double x, y, z, R, w, fc, dfc, ex, rh[3];
+};
+
+__device__ inline void load_image(const So3Args &a, bool valid, int64_t e, int64_t centre, const int64_t *img_j,
+                                  const double *img_r, const int64_t *atom_z, So3Img &m) {
+    m.valid = valid;
+    m.e = e;
+    m.i = centre;
+    m.x = 0;
+    m.y = 0;
+    m.z = 1;
+    m.R = 1;
+    m.w = 0;
+    if (m.valid) {
+        m.x = img_r[3 * e];
+        m.y = img_r[3 * e + 1];
+        m.z = img_r[3 * e + 2];
+        m.R = sqrt(m.x * m.x + m.y * m.y + m.z * m.z);
+        const int64_t zj = atom_z[img_j[e]], zi = atom_z[m.i];
+        m.w = (a.weight_on && zj != zi) ? -(double)zj : (double)zj;
+    }
+    so3_cutoff(a.cutoff_id, m.R, a.rc, m.fc, m.dfc);
+    m.ex = 4.0 * M_PI * exp(-a.alpha * m.R * m.R);
+    m.rh[0] = m.x / m.R;
+    m.rh[1] = m.y / m.R;
+    m.rh[2] = m.z / m.R;
+}
+
+// Fill the Y table of one image: rows l' = 0..lmax+1 spread over the L lanes
+// of the group (lane l also takes row l + L when it exists).
+__device__ inline void fill_ylm(const So3Img &m, int l, int L, int lmax, cd *Y, int LP) {
+    const double t = m.z / m.R;
+    const double rho = sqrt(m.x * m.x + m.y * m.y);
+    const double s = rho / m.R;
+    const double ph = atan2(m.y, m.x);
+    double sp, cp;
+    sincos(ph, &sp, &cp);
+    const cd eiphi = cmake(cp, sp);
+    for (int ll = l; ll <= lmax + 1; ll += L) ylm_row(ll, t, s, eiphi, Y, LP);
+}
+
+// Per-image factors of one lane (image, l): with
+//   P_m  = w norm_l ex fc Y_lm
+//   U_km = w norm_l (r_k ex (dfc - 2 alpha R fc) Y_lm + ex fc dY_lm/dr_k)
+//   V_km = w norm_l ex fc r_k Y_lm
+// one has c_nlm = I_n P_m and dc_nlm/dr_k = I_n U_km + dI_n V_km
+// (SO3.py:676-694: gradient of exp, of the radial integral, of Y_lm, of fc).
+__device__ inline void image_factors(const So3Args &a, const So3Img &m, int l, int mm, const cd *Y, int LP, cd &P,
+                                     cd U[3], cd V[3]) {
+    const double wn = m.w * a.lnorm[l];
+    const cd y = Y[l * (2 * LP + 1) + LP + mm];
+    cd gy[3];
+    grad_ylm(l, mm, m.R, Y, LP, gy);
+    P = cscale(y, wn * m.ex * m.fc);
+    const double cu = wn * m.ex * (m.dfc - 2.0 * a.alpha * m.R * m.fc);
+    const double cv = wn * m.ex * m.fc;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        U[k] = cadd(cscale(y, cu * m.rh[k]), cscale(gy[k], cv));
+        V[k] = cscale(y, cv * m.rh[k]);
+    }
+}
+
+// ------------------------------------------------------------ accumulate
+// ctot[i][n][l][m] = sum_e c_e  (MODE 0)  or its tangent
+// sum_e sum_k dr_e,k dc_e/dr_k (MODE 1).  One workgroup (64 threads) per
+// centre, persistent.  Round: NG images; lanes (image, l) compute the radial
+// sums and the per-(l, m) factors into LDS; then every thread adds the
+// contributions of the round's images to the ctot entries it owns (LDS
+// accumulator, fixed order -> deterministic).
+template <int NMAX, int MODE>
+__global__ __launch_bounds__(64) void so3_accum_kernel(So3Args a, int64_t natoms, const int64_t *__restrict__ img_start,
+                                                        const int64_t *__restrict__ img_j,
+                                                        const double *__restrict__ img_r,
+                                                        const int64_t *__restrict__ atom_z,
+                                                        const double *__restrict__ dr, double *__restrict__ out) {
+    extern __shared__ double smem[];
+    const int L = a.lmax + 1;
+    const int LP = a.lmax + 1;
+    const int NG = 64 / L;
+    const int M2 = 2 * a.lmax + 1;
+    const int ysz = (a.lmax + 2) * (2 * LP + 1);
+    const int csz = NMAX * L * M2;
+    double *Gs = smem;                           // [NMAX*nq]
+    double *rqs = Gs + NMAX * a.nq;              // [nq]
+    double *lnorm = rqs + a.nq;                  // [L]
+    double *Ir = lnorm + L;                      // [NG][NMAX][L]
+    double *dIr = Ir + NG * NMAX * L;            // [NG][NMAX][L]
+    cd *Yt = reinterpret_cast<cd *>(dIr + NG * NMAX * L + ((NMAX * a.nq + a.nq + L) & 1));  // [NG][ysz]
+    cd *X = Yt + NG * ysz;                       // [NG][L][M2]
+    cd *Z2 = X + NG * L * M2;                    // [NG][L][M2]
+    cd *acc = Z2 + NG * L * M2;                  // [csz]
+    const int lane = threadIdx.x;
+    const int g = lane / L, l = lane % L;
+    for (int k = lane; k < NMAX * a.nq; k += 64) Gs[k] = a.G[k];
+    for (int k = lane; k < a.nq; k += 64) rqs[k] = a.rq[k];
+    for (int k = lane; k < L; k += 64) lnorm[k] = a.lnorm[k];
+    So3Args al = a;
+    al.lnorm = lnorm;
+    for (int64_t c = blockIdx.x; c < natoms; c += gridDim.x) {
+        __syncthreads();
+        for (int k = lane; k < csz; k += 64) acc[k] = cmake(0.0, 0.0);
+        const int64_t s0 = img_start[c], s1 = img_start[c + 1];
+        for (int64_t base = s0; base < s1; base += NG) {
+            const int64_t e = base + g;
+            const bool valid = (g < NG) && (e < s1);
+            So3Img m;
+            load_image(al, valid, e, c, img_j, img_r, atom_z, m);
+            double I[NMAX], dI[NMAX];
+            if (valid) {
+                radial<NMAX>(al, l, m.R, Gs, rqs, I, dI, MODE == SO3_TANGENT);
+                fill_ylm(m, l, L, a.lmax, Yt + g * ysz, LP);
+#pragma unroll
+                for (int n = 0; n < NMAX; n++) {
+                    Ir[(g * NMAX + n) * L + l] = I[n];
+                    dIr[(g * NMAX + n) * L + l] = dI[n];
+                }
+            }
+            __syncthreads();
+            if (valid) {
+                double d[3] = {0, 0, 0};
+                if (MODE == SO3_TANGENT) {
+                    d[0] = dr[3 * e];
+                    d[1] = dr[3 * e + 1];
+                    d[2] = dr[3 * e + 2];
+                }
+                const cd *Y = Yt + g * ysz;
+                for (int mm = -l; mm <= l; mm++) {
+                    cd P, U[3], V[3];
+                    image_factors(al, m, l, mm, Y, LP, P, U, V);
+                    const int o = (g * L + l) * M2 + a.lmax + mm;
+                    if (MODE == SO3_CTOT) {
+                        X[o] = P;
+                        Z2[o] = cmake(0, 0);
+                    } else {
+                        X[o] = cadd(cadd(cscale(U[0], d[0]), cscale(U[1], d[1])), cscale(U[2], d[2]));
+                        Z2[o] = cadd(cadd(cscale(V[0], d[0]), cscale(V[1], d[1])), cscale(V[2], d[2]));
+                    }
+                }
+            }
+            __syncthreads();
+            const int nvalid = (int)((s1 - base) < NG ? (s1 - base) : NG);
+            for (int k = lane; k < csz; k += 64) {
+                const int n = k / (L * M2), rem = k - n * L * M2, ll = rem / M2, mi = rem - ll * M2;
+                if (mi - a.lmax < -ll || mi - a.lmax > ll) continue;
+                cd s = acc[k];
+                for (int gg = 0; gg < nvalid; gg++) {
+                    const int o = (gg * L + ll) * M2 + mi;
+                    const double iv = Ir[(gg * NMAX + n) * L + ll];
+                    s = cadd(s, cscale(X[o], iv));
+                    if (MODE == SO3_TANGENT) s = cadd(s, cscale(Z2[o], dIr[(gg * NMAX + n) * L + ll]));
+                }
+                acc[k] = s;
+            }
+        }
+        __syncthreads();
+        cd *oc = reinterpret_cast<cd *>(out) + (size_t)c * csz;
+        for (int k = lane; k < csz; k += 64) oc[k] = acc[k];
+    }
+}
+
+// ------------------------------------------------ per-image (flat) kernels
+// MODE SO3_ADJOINT: dE/dr_e,k = Re sum_{n,l,m} dc_e,nlm,k conj(A_i,nlm)
+// MODE SO3_PLAIN:   dP_e[tril(a,b)][l][k] = Q_k[a][b] + Q_k[b][a],
+//                   Q_k[n][n'] = Re sum_m dc_e,nlm,k conj(ctot_i,n'lm)
+// (SO3.py:243-247).  One wavefront per NG images; lanes (image, l).
+template <int NMAX, int MODE>
+__global__ __launch_bounds__(64) void so3_image_kernel(So3Args a, int64_t nimg, const int64_t *__restrict__ img_atom,
+                                                        const int64_t *__restrict__ img_j,
+                                                        const double *__restrict__ img_r,
+                                                        const int64_t *__restrict__ atom_z,
+                                                        const double *__restrict__ cin, double *__restrict__ out) {
+    extern __shared__ double smem[];
+    const int L = a.lmax + 1;
+    const int LP = a.lmax + 1;
+    const int NG = 64 / L;
+    const int M2 = 2 * a.lmax + 1;
+    const int ysz = (a.lmax + 2) * (2 * LP + 1);
+    const int csz = NMAX * L * M2;
+    double *Gs = smem;
+    double *rqs = Gs + NMAX * a.nq;
+    double *lnorm = rqs + a.nq;
+    double *red = lnorm + L;  // [64][3]
+    cd *Yt = reinterpret_cast<cd *>(red + 64 * 3 + ((NMAX * a.nq + a.nq + L) & 1));
+    const int lane = threadIdx.x;
+    const int g = lane / L, l = lane % L;
+    for (int k = lane; k < NMAX * a.nq; k += 64) Gs[k] = a.G[k];
+    for (int k = lane; k < a.nq; k += 64) rqs[k] = a.rq[k];
+    for (int k = lane; k < L; k += 64) lnorm[k] = a.lnorm[k];
+    So3Args al = a;
+    al.lnorm = lnorm;
+    __syncthreads();
+    const int64_t e = (int64_t)blockIdx.x * NG + g;
+    const bool valid = (g < NG) && (e < nimg);
+    So3Img m;
+    load_image(al, valid, e, valid ? img_atom[e] : 0, img_j, img_r, atom_z, m);
+    double I[NMAX], dI[NMAX];
+    if (valid) {
+        radial<NMAX>(al, l, m.R, Gs, rqs, I, dI, true);
+        fill_ylm(m, l, L, a.lmax, Yt + g * ysz, LP);
+    }
+    __syncthreads();
+    double r3[3] = {0, 0, 0};
+    if (valid) {
+        const cd *Y = Yt + g * ysz;
+        const cd *ci = reinterpret_cast<const cd *>(cin) + (size_t)m.i * csz;
+        // alpha_k[n'] = Re sum_m U_km conj(C_n'lm), beta_k[n'] = Re sum_m V_km conj(C_n'lm)
+        double al_[3][NMAX], be_[3][NMAX];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int n = 0; n < NMAX; n++) {
+                al_[k][n] = 0.0;
+                be_[k][n] = 0.0;
+            }
+        for (int mm = -l; mm <= l; mm++) {
+            cd P, U[3], V[3];
+            image_factors(al, m, l, mm, Y, LP, P, U, V);
+#pragma unroll
+            for (int n = 0; n < NMAX; n++) {
+                const cd cv = ci[(n * L + l) * M2 + a.lmax + mm];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    al_[k][n] += cdotre(cv, U[k]);
+                    be_[k][n] += cdotre(cv, V[k]);
+                }
+            }
+        }
+        if (MODE == SO3_ADJOINT) {
+            // cin = A: dE/dr_k = sum_n I_n alpha_k[n] + dI_n beta_k[n]
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double s = 0.0;
+#pragma unroll
+                for (int n = 0; n < NMAX; n++) s += I[n] * al_[k][n] + dI[n] * be_[k][n];
+                r3[k] = s;
+            }
+        } else {
+            // cin = ctot: write dP of this l for every tril pair
+            const int ntri = NMAX * (NMAX + 1) / 2;
+            double *o = out + (size_t)e * ntri * L * 3;
+            int t = 0;
+#pragma unroll
+            for (int aa = 0; aa < NMAX; aa++)
+#pragma unroll
+                for (int bb = 0; bb <= aa; bb++, t++)
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        o[(t * L + l) * 3 + k] = I[aa] * al_[k][bb] + dI[aa] * be_[k][bb] + I[bb] * al_[k][aa] +
+                                                 dI[bb] * be_[k][aa];
+        }
+    }
+    if (MODE == SO3_ADJOINT) {
+        red[3 * lane] = r3[0];
+        red[3 * lane + 1] = r3[1];
+        red[3 * lane + 2] = r3[2];
+        __syncthreads();
+        if (valid && l == 0) {
+            for (int k = 0; k < 3; k++) {
+                double s = 0.0;
+                for (int q = 0; q < L; q++) s += red[3 * (lane + q) + k];
+                out[3 * e + k] = s;
+            }
+        }
+    }
+}
+
+// P[i][tril(a,b)*L + l] = Re sum_m C[a][l][m] conj(C[b][l][m])   (SO3.py:228,259)
+// with mode tangent: Re sum_m dC[a] conj(C[b]) + C[a] conj(dC[b]).
+__global__ void so3_power_kernel(int64_t natoms, int nmax, int lmax, const double *__restrict__ ct,
+                                 const double *__restrict__ dct, double *__restrict__ x) {
+    const int L = lmax + 1, M2 = 2 * lmax + 1;
+    const int ntri = nmax * (nmax + 1) / 2;
+    const int nc = ntri * L;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= natoms * nc) return;
+    const int64_t i = t / nc;
+    const int q = (int)(t - i * nc);
+    const int tri = q / L, l = q - tri * L;
+    int aa = 0;
+    while ((aa + 1) * (aa + 2) / 2 <= tri) aa++;
+    const int bb = tri - aa * (aa + 1) / 2;
+    const int csz = nmax * L * M2;
+    const cd *c = reinterpret_cast<const cd *>(ct) + (size_t)i * csz;
+    const cd *ca = c + (aa * L + l) * M2 + lmax, *cb = c + (bb * L + l) * M2 + lmax;
+    double s = 0.0;
+    if (!dct) {
+        for (int mm = -l; mm <= l; mm++) s += cdotre(cb[mm], ca[mm]);
+    } else {
+        const cd *d = reinterpret_cast<const cd *>(dct) + (size_t)i * csz;
+        const cd *da = d + (aa * L + l) * M2 + lmax, *db = d + (bb * L + l) * M2 + lmax;
+        for (int mm = -l; mm <= l; mm++) s += cdotre(cb[mm], da[mm]) + cdotre(db[mm], ca[mm]);
+    }
+    x[t] = s;
+}
+
+// A[i][n][l][m] = sum_n' S[n][n'][l] C[i][n'][l][m], S from dE/dP with the
+// diagonal doubled (P is quadratic in C).
+__global__ void so3_adj_a_kernel(int64_t natoms, int nmax, int lmax, const double *__restrict__ ct,
+                                 const double *__restrict__ beta, double *__restrict__ A) {
+    const int L = lmax + 1, M2 = 2 * lmax + 1;
+    const int ntri = nmax * (nmax + 1) / 2;
+    const int csz = nmax * L * M2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= natoms * csz) return;
+    const int64_t i = t / csz;
+    const int k = (int)(t - i * csz);
+    const int n = k / (L * M2), rem = k - n * L * M2, l = rem / M2, mi = rem - l * M2;
+    const cd *c = reinterpret_cast<const cd *>(ct) + (size_t)i * csz;
+    const double *b = beta + (size_t)i * ntri * L;
+    cd s = cmake(0, 0);
+    if (mi - lmax >= -l && mi - lmax <= l) {
+        for (int np = 0; np < nmax; np++) {
+            const int hi = n > np ? n : np, lo = n > np ? np : n;
+            double f = b[(hi * (hi + 1) / 2 + lo) * L + l];
+            if (np == n) f *= 2.0;
+            s = cadd(s, cscale(c[(np * L + l) * M2 + mi], f));
+        }
+    }
+    reinterpret_cast<cd *>(A)[t] = s;
+}
+
+}  // namespace pxff
+
+// =================================================================== C ABI
+using namespace pxff;
+
+namespace {
+int so3_check(const pxff_so3_params *p) {
+    PXFF_REQUIRE(p != nullptr, "null params");
+    if (p->nmax < 1 || p->nmax > 11 || p->lmax < 0 || p->lmax > SO3_LMAXC) {
+        set_error("SO3: nmax must be in 1..11 and lmax in 0..32");
+        return PXFF_EINVAL;
+    }
+    PXFF_REQUIRE(p->rcut > 0 && p->alpha > 0 && p->nquad > 0, "SO3: rcut, alpha, nquad must be > 0");
+    PXFF_REQUIRE(p->cutoff_id >= 1 && p->cutoff_id <= 8, "SO3: cutoff id must be 1..8");
+    const int L = p->lmax + 1;
+    if (p->nmax * L * (2 * p->lmax + 1) > 4096) {
+        set_error("SO3: nmax*(lmax+1)*(2lmax+1) > 4096 is not supported on the device path");
+        return PXFF_ENOTIMPL;
+    }
+    return 0;
+}
+
+So3Args make_args(const pxff_so3_params *p, const double *G, const double *rq, const double *lnorm) {
+    So3Args a;
+    a.nmax = p->nmax;
+    a.lmax = p->lmax;
+    a.nq = p->nquad;
+    a.cutoff_id = p->cutoff_id;
+    a.weight_on = p->weight_on;
+    a.rc = p->rcut;
+    a.alpha = p->alpha;
+    a.G = G;
+    a.rq = rq;
+    a.lnorm = lnorm;
+    return a;
+}
+
+size_t accum_lds(const So3Args &a, int nmax) {
+    const int L = a.lmax + 1, NG = 64 / L, M2 = 2 * a.lmax + 1, LP = L;
+    const int ysz = (a.lmax + 2) * (2 * LP + 1);
+    size_t d = (size_t)nmax * a.nq + a.nq + L + 2 * (size_t)NG * nmax * L + 1;
+    d += 2 * ((size_t)NG * ysz + 2 * (size_t)NG * L * M2 + (size_t)nmax * L * M2);
+    return d * sizeof(double);
+}
+
+size_t image_lds(const So3Args &a, int nmax) {
+    const int L = a.lmax + 1, NG = 64 / L, LP = L;
+    const int ysz = (a.lmax + 2) * (2 * LP + 1);
+    size_t d = (size_t)nmax * a.nq + a.nq + L + 64 * 3 + 1 + 2 * (size_t)NG * ysz;
+    return d * sizeof(double);
+}
+
+int g_cus = 0;
+int cus() {
+    if (!g_cus) {
+        int dev = 0;
+        hipDeviceProp_t pr;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&pr, dev) != hipSuccess) return 256;
+        g_cus = pr.multiProcessorCount;
+    }
+    return g_cus;
+}
+}  // namespace
+
+#define SO3_DISPATCH_NMAX(nm, ...)                          \
+    switch (nm) {                                           \
+    case 1: { constexpr int NM = 1; __VA_ARGS__; } break;   \
+    case 2: { constexpr int NM = 2; __VA_ARGS__; } break;   \
+    case 3: { constexpr int NM = 3; __VA_ARGS__; } break;   \
+    case 4: { constexpr int NM = 4; __VA_ARGS__; } break;   \
+    case 5: { constexpr int NM = 5; __VA_ARGS__; } break;   \
+    case 6: { constexpr int NM = 6; __VA_ARGS__; } break;   \
+    case 7: { constexpr int NM = 7; __VA_ARGS__; } break;   \
+    case 8: { constexpr int NM = 8; __VA_ARGS__; } break;   \
+    case 9: { constexpr int NM = 9; __VA_ARGS__; } break;   \
+    case 10: { constexpr int NM = 10; __VA_ARGS__; } break; \
+    case 11: { constexpr int NM = 11; __VA_ARGS__; } break; \
+    default: set_error("SO3: nmax out of range"); return PXFF_EINVAL; \
+    }
+
+template <typename K>
+static int launch_set_lds(K kern, size_t lds) {
+    if (lds > 65536)
+        PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    return 0;
+}
+
+extern "C" int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start,
+                             const int64_t *d_img_j, const double *d_img_r, const int64_t *d_atom_z,
+                             const double *d_G, const double *d_rq, const double *d_lnorm, const double *d_dr,
+                             double *d_out, void *stream) {
+    int rc = so3_check(p);
+    if (rc) return rc;
+    if (natoms <= 0) return 0;
+    So3Args a = make_args(p, d_G, d_rq, d_lnorm);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t lds = accum_lds(a, p->nmax);
+    PXFF_REQUIRE(lds <= 160 * 1024, "SO3: LDS need exceeds 160 KB");
+    const int grid = (int)std::min<int64_t>(natoms, (int64_t)cus() * 8);
+    if (d_dr) {
+        SO3_DISPATCH_NMAX(p->nmax, {
+            auto k = so3_accum_kernel<NM, SO3_TANGENT>;
+            if ((rc = launch_set_lds(k, lds))) return rc;
+            k<<<grid, 64, lds, s>>>(a, natoms, d_img_start, d_img_j, d_img_r, d_atom_z, d_dr, d_out);
+        });
+    } else {
+        SO3_DISPATCH_NMAX(p->nmax, {
+            auto k = so3_accum_kernel<NM, SO3_CTOT>;
+            if ((rc = launch_set_lds(k, lds))) return rc;
+            k<<<grid, 64, lds, s>>>(a, natoms, d_img_start, d_img_j, d_img_r, d_atom_z, nullptr, d_out);
+        });
+    }
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_so3_power(const pxff_so3_params *p, int64_t natoms, const double *d_ctot, const double *d_dctot,
+                              double *d_x, void *stream) {
+    int rc = so3_check(p);
+    if (rc) return rc;
+    if (natoms <= 0) return 0;
+    const int64_t nc = (int64_t)p->nmax * (p->nmax + 1) / 2 * (p->lmax + 1);
+    so3_power_kernel<<<div_up(natoms * nc, 256), 256, 0, (hipStream_t)stream>>>(natoms, p->nmax, p->lmax, d_ctot,
+                                                                                d_dctot, d_x);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_so3_adjoint_a(const pxff_so3_params *p, int64_t natoms, const double *d_ctot,
+                                  const double *d_beta, double *d_a, void *stream) {
+    int rc = so3_check(p);
+    if (rc) return rc;
+    if (natoms <= 0) return 0;
+    const int64_t csz = (int64_t)p->nmax * (p->lmax + 1) * (2 * p->lmax + 1);
+    so3_adj_a_kernel<<<div_up(natoms * csz, 256), 256, 0, (hipStream_t)stream>>>(natoms, p->nmax, p->lmax, d_ctot,
+                                                                                 d_beta, d_a);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+static int so3_image_launch(const pxff_so3_params *p, int mode, int64_t nimg, const int64_t *d_img_atom,
+                            const int64_t *d_img_j, const double *d_img_r, const int64_t *d_atom_z, const double *d_G,
+                            const double *d_rq, const double *d_lnorm, const double *d_cin, double *d_out,
+                            void *stream) {
+    int rc = so3_check(p);
+    if (rc) return rc;
+    if (nimg <= 0) return 0;
+    So3Args a = make_args(p, d_G, d_rq, d_lnorm);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t lds = image_lds(a, p->nmax);
+    PXFF_REQUIRE(lds <= 160 * 1024, "SO3: LDS need exceeds 160 KB");
+    const int NG = 64 / (p->lmax + 1);
+    const int grid = div_up(nimg, NG);
+    if (mode == SO3_ADJOINT) {
+        SO3_DISPATCH_NMAX(p->nmax, {
+            auto k = so3_image_kernel<NM, SO3_ADJOINT>;
+            if ((rc = launch_set_lds(k, lds))) return rc;
+            k<<<grid, 64, lds, s>>>(a, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_cin, d_out);
+        });
+    } else {
+        SO3_DISPATCH_NMAX(p->nmax, {
+            auto k = so3_image_kernel<NM, SO3_PLAIN>;
+            if ((rc = launch_set_lds(k, lds))) return rc;
+            k<<<grid, 64, lds, s>>>(a, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_cin, d_out);
+        });
+    }
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_so3_deidrj(const pxff_so3_params *p, int64_t nimg, const int64_t *d_img_atom,
+                               const int64_t *d_img_j, const double *d_img_r, const int64_t *d_atom_z,
+                               const double *d_G, const double *d_rq, const double *d_lnorm, const double *d_a,
+                               double *d_dedr, void *stream) {
+    return so3_image_launch(p, SO3_ADJOINT, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_G, d_rq, d_lnorm, d_a,
+                            d_dedr, stream);
+}
+
+extern "C" int pxff_so3_dpdr(const pxff_so3_params *p, int64_t nimg, const int64_t *d_img_atom,
+                             const int64_t *d_img_j, const double *d_img_r, const int64_t *d_atom_z,
+                             const double *d_G, const double *d_rq, const double *d_lnorm, const double *d_ctot,
+                             double *d_dpdr, void *stream) {
+    return so3_image_launch(p, SO3_PLAIN, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_G, d_rq, d_lnorm, d_ctot,
+                            d_dpdr, stream);
+}

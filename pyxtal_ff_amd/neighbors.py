@@ -102,51 +102,109 @@ class DeviceNeighborList:
         n = self.natoms
         s = nat.stream()
         L = nat.lib()
-        bin_count = torch.zeros(max(nbins, 1), **i32)
-        bin_start = torch.zeros(max(nbins, 1) + 1, **i32)
-        atom_bin = torch.zeros(max(n, 1), **i32)
-        wrap = torch.zeros(max(n, 1) * 3, **i32)
-        bin_atoms = torch.zeros(max(n, 1), **i32)
-        img_count = torch.zeros(max(n, 1), **i64)
+        e32 = dict(dtype=torch.int32, device=self.pos.device)
+        bin_count = torch.empty(max(nbins, 1), **e32)
+        bin_start = torch.empty(max(nbins, 1) + 1, **e32)
+        atom_bin = torch.empty(max(n, 1), **e32)
+        wrap = torch.empty(max(n, 1) * 3, **e32)
+        bin_atoms = torch.empty(max(n, 1), **e32)
+        img_count = torch.empty(max(n, 1), **i64)
         p = nat.ptr
         nat.check(L.pxff_nlist_count(p(self.pos), n, p(self.atom_struct), p(self.atom_start), p(self.cells),
                                      p(self.inv_cells), p(self.pbc), p(self.geom), max(nbins, 1), self.rc,
                                      p(bin_count), p(bin_start), p(atom_bin), p(wrap), p(bin_atoms), p(img_count), s),
                   "pxff_nlist_count")
-        img_start = torch.zeros(n + 1, **i64)
+        img_start = torch.empty(n + 1, **i64)
         nat.check(L.pxff_scan_i64(p(img_count), n, p(img_start), s), "pxff_scan_i64")
-        nimg = int(img_start[n].item())
+        nimg = int(img_start[n].item())  # the one host synchronisation: sizes the image arrays
         self.nimg = nimg
         self.img_start = img_start
         m = max(nimg, 1)
-        self.img_atom = torch.zeros(m, **i64)
-        self.img_j = torch.zeros(m, **i64)
-        self.img_T = torch.zeros(m * 3, **i32)
-        self.img_r = torch.zeros(m * 3, **f64)
-        row_count = torch.zeros(max(n, 1), **i64)
+        self.img_atom = torch.empty(m, **i64)
+        self.img_j = torch.empty(m, **i64)
+        self.img_T = torch.empty(m * 3, **i32)
+        self.img_r = torch.empty(m * 3, **f64)
+        self._row_count = torch.empty(max(n, 1), **i64)
         nat.check(L.pxff_nlist_fill(p(self.pos), n, p(self.atom_struct), p(self.atom_start), p(self.cells),
                                     p(self.pbc), p(self.geom), self.rc, p(bin_start), p(atom_bin), p(wrap),
                                     p(bin_atoms), p(img_start), p(self.img_atom), p(self.img_j), p(self.img_T),
-                                    p(self.img_r), p(row_count), s), "pxff_nlist_fill")
-        row_start = torch.zeros(n + 1, **i64)
-        nat.check(L.pxff_scan_i64(p(row_count), n, p(row_start), s), "pxff_scan_i64")
-        nrows = int(row_start[n].item())
-        self.nrows = nrows
-        self.row_start = row_start
-        r = max(nrows, 1)
-        self.seq = torch.zeros(r * 2, **i64)
-        self.row_img = torch.zeros(r + 1, **i64)
-        self.img_row = torch.zeros(m, **i64)
-        self.nsite = torch.zeros(max(n, 1), **i64)
-        self.row_rev = torch.zeros(r, **i64)
-        self.img_rev = torch.zeros(m, **i64)
-        flag = torch.zeros(1, **i32)
-        nat.check(L.pxff_nlist_rows(n, nimg, nrows, p(img_start), p(self.img_atom), p(self.img_j), p(self.img_T),
-                                    p(row_start), p(self.seq), p(self.row_img), p(self.img_row), p(self.nsite),
-                                    p(self.row_rev), p(self.img_rev), p(flag), s), "pxff_nlist_rows")
-        fl = int(flag.item())
+                                    p(self.img_r), p(self._row_count), s), "pxff_nlist_fill")
+        self.img_rev = torch.empty(m, **i64)
+        self._flag = torch.zeros(2, **i32)
+        nat.check(L.pxff_nlist_rev(n, nimg, p(img_start), p(self.img_atom), p(self.img_j), p(self.img_T),
+                                   p(self.img_rev), p(self._flag), s), "pxff_nlist_rev")
+        self._rows_built = False
+
+    def verify(self):
+        """Raise if the reverse-image / reverse-row search failed (reads a
+        device flag, i.e. synchronises)."""
+        fl = int(self._flag.max().item())
         if fl != 0:
             raise nat.PxffError("neighbour list is not symmetric (flag %d)" % fl)
+
+    def _build_rows(self):
+        if self._rows_built:
+            return
+        n = self.natoms
+        dev = self.pos.device
+        i64 = dict(dtype=torch.int64, device=dev)
+        p = nat.ptr
+        s = nat.stream()
+        L = nat.lib()
+        row_start = torch.empty(n + 1, **i64)
+        nat.check(L.pxff_scan_i64(p(self._row_count), n, p(row_start), s), "pxff_scan_i64")
+        nrows = int(row_start[n].item())
+        r = max(nrows, 1)
+        m = max(self.nimg, 1)
+        self._nrows = nrows
+        self._row_start = row_start
+        self._seq = torch.empty(r * 2, **i64)
+        self._row_img = torch.empty(r + 1, **i64)
+        self._img_row = torch.empty(m, **i64)
+        self._nsite = torch.empty(max(n, 1), **i64)
+        self._row_rev = torch.empty(r, **i64)
+        flag = self._flag[1:2]
+        nat.check(L.pxff_nlist_rows(n, self.nimg, nrows, p(self.img_start), p(self.img_j), p(row_start),
+                                    p(self._seq), p(self._row_img), p(self._img_row), p(self._nsite),
+                                    p(self._row_rev), p(flag), s), "pxff_nlist_rows")
+        self._rows_built = True
+        self.verify()
+
+    # seq rows are built on first use (the adjoint path never needs them)
+    @property
+    def nrows(self):
+        self._build_rows()
+        return self._nrows
+
+    @property
+    def row_start(self):
+        self._build_rows()
+        return self._row_start
+
+    @property
+    def seq(self):
+        self._build_rows()
+        return self._seq
+
+    @property
+    def row_img(self):
+        self._build_rows()
+        return self._row_img
+
+    @property
+    def img_row(self):
+        self._build_rows()
+        return self._img_row
+
+    @property
+    def nsite(self):
+        self._build_rows()
+        return self._nsite
+
+    @property
+    def row_rev(self):
+        self._build_rows()
+        return self._row_rev
 
     # views in the reference's shapes
     def seq_array(self):

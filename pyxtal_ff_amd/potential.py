@@ -84,7 +84,7 @@ def species_index(symbols, elements, device):
 def energy_forces_adjoint(nl, state, model, species, stress=False):
     """E (per structure sum), F [N,3], optional virial-based stress via the
     adjoint path."""
-    pos = nl.pos.detach().clone().requires_grad_(True)
+    pos = nl.pos.detach().requires_grad_(True)
     strain = torch.zeros((3, 3), dtype=torch.float64, device=pos.device, requires_grad=stress)
     r = adjoint.separations(nl, pos, strain if stress else None)
     b = adjoint.bispectrum(r, state)
