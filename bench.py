@@ -109,7 +109,7 @@ def run_gpu(steps, warmup):
     import torch
     from pyxtal_ff_amd import adjoint, potential
     from pyxtal_ff_amd import so4 as dso4
-    from pyxtal_ff_amd.neighbors import DeviceNeighborList
+    from pyxtal_ff_amd.neighbors import NeighborListBuilder
 
     s = build_case()
     n = len(s)
@@ -121,9 +121,12 @@ def run_gpu(steps, warmup):
     model = potential.NNPotential(["Si"], cfg.nb, (30, 30), seed=0).to(dev)
     species = potential.species_index(s.symbols, ["Si"], dev)
 
+    # structure metadata (cell, pbc, bin grid) is set up once, as in an MD
+    # run; every step rebuilds the neighbour list from the HBM positions
+    builder = NeighborListBuilder([n], [s.cell], [s.pbc], rc)
+
     def nlist():
-        # positions stay resident in HBM; cell and pbc are host metadata
-        return DeviceNeighborList(None, [s.cell], [s.pbc], rc, pos_device=pos_dev)
+        return builder.build(pos_dev)
 
     def step_adjoint():
         nl = nlist()

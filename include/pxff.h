@@ -114,7 +114,10 @@ typedef struct {
 /* Table sizes for 2J: nu (full U), nhalf (half U), nz, nb. */
 int pxff_so4_sizes(int twol, int *nu, int *nhalf, int *nz, int *nb);
 
-/* U_tot (half layout, self term and u_norm included) per centre.
+/* U_tot (half rows, self term and u_norm included) per centre, stored
+ * tile-major for the Z kernel: element h of atom i at
+ * [(i/64)*nhalf + h]*64 + i%64 (complex); d_utot must hold ceil(natoms/64)*64
+ * atoms and be zeroed by the caller (padding lanes are read).
  *   d_img_start [natoms+1], d_img_j [nimg], d_img_r [nimg*3]; d_atom_w
  *   [natoms] = neighbour weight of each atom (pair weight of image = w[j]);
  *   d_site_w [natoms] = self weight.  d_work_order [natoms] int64 or NULL:
@@ -123,7 +126,8 @@ int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_s
                 const double *d_img_r, const double *d_atom_w, const double *d_site_w,
                 const int64_t *d_work_order, double *d_utot, void *stream);
 
-/* Z (and B) from U_tot.  d_z [natoms*nz*2] out; d_b [natoms*nb] out (or NULL). */
+/* Z (and B) from the tile-major U_tot.  d_z [natoms*nz*2] out (atom-major);
+ * d_b [natoms*nb] out (or NULL). */
 int pxff_so4_zb(const pxff_so4_params *p, int64_t natoms, const double *d_utot, double *d_z, double *d_b,
                 void *stream);
 

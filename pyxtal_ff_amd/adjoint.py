@@ -95,7 +95,23 @@ class _Separations(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g):
-        return dev_so4.gather(ctx.nl, g.contiguous()), None
+        return _Gather.apply(g, ctx.nl), None
+
+
+class _Gather(torch.autograd.Function):
+    """dE/dR from dE/dr (pxff_so4_gather); its own backward is the transpose
+    dr_e = G[j] - G[i], so forces stay differentiable (force training)."""
+
+    @staticmethod
+    def forward(ctx, dedr, nl):
+        ctx.nl = nl
+        return dev_so4.gather(nl, dedr.contiguous())
+
+    @staticmethod
+    def backward(ctx, G):
+        nl = ctx.nl
+        k = nl.nimg
+        return G.index_select(0, nl.img_j[:k]) - G.index_select(0, nl.img_atom[:k]), None
 
 
 def _shift(nl):

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "pxff_common.h"
@@ -42,6 +43,8 @@ struct So4Dev {
     int *hoff, *h_level, *h_diag;
     double *h_weight;
     int *idxu_block;
+    int nzrows, nzchunks;
+    int *zrows, *zchunk;
 };
 
 namespace {
@@ -96,6 +99,10 @@ int get_so4_tables(int twol, int norm, const So4Dev **out) {
     if ((rc = upload(t.h_diag, &d.h_diag))) return rc;
     if ((rc = upload(t.h_weight, &d.h_weight))) return rc;
     if ((rc = upload(t.idxu_block, &d.idxu_block))) return rc;
+    d.nzrows = t.nzrows;
+    d.nzchunks = 16;
+    if ((rc = upload(t.zrows, &d.zrows))) return rc;
+    if ((rc = upload(chunk_zrows(t, d.nzchunks), &d.zchunk))) return rc;
     g_tabs[key] = d;
     *out = &g_tabs[key];
     return 0;
@@ -253,7 +260,7 @@ __device__ inline void sweep_rows(int mb, const double *__restrict__ rpq, const 
 // image per round; per-group partial sums live in LDS and are reduced in a
 // fixed order at the end (deterministic).  With TANGENT the kernel instead
 // accumulates dU_tot . dr_e (forward-mode derivative of U_tot).
-template <int TWOJ, bool TANGENT>
+template <int TWOJ, bool TANGENT, bool TILED>
 __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
                                                         const int64_t *__restrict__ img_j,
                                                         const double *__restrict__ img_r,
@@ -350,7 +357,10 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
             const int hm = loc / (l + 1), hma = loc % (l + 1);
             if (!TANGENT && hm == hma) acc.re += fcut0 * site_w[i];  // addself_uarraytot
             acc = cscale(acc, lnorm[l]);                               // ulisttot *= u_norm
-            reinterpret_cast<cd *>(out)[(size_t)i * NH + h] = acc;
+            if (TILED)  // [atom / 64][h][atom % 64], the layout of the Z kernel's loads
+                reinterpret_cast<cd *>(out)[((size_t)(i >> 6) * NH + h) * 64 + (i & 63)] = acc;
+            else
+                reinterpret_cast<cd *>(out)[(size_t)i * NH + h] = acc;
         }
     }
 }
@@ -445,6 +455,130 @@ __global__ __launch_bounds__(256) void so4_zb_kernel(int64_t natoms, const doubl
             }
         }
     }
+}
+
+// ------------------------------------------------- Z, atom per lane
+// compute_zi (SNAP.py:1050-1098) with one atom per lane: a wavefront owns a
+// tile of 64 atoms and a chunk of Z rows (block (l1, l2, l), row mb), so the
+// loop structure and the Clebsch-Gordan coefficients are the same for all
+// lanes (scalar loads, uniform branches).  For a row, every (mb1, mb2) pair
+// of the ib loop brings U^{l1} row mb1 and U^{l2} row mb2 into registers
+// (coalesced loads from the tile-major U_tot, rows beyond l/2 by inversion
+// symmetry), and all (ma1, ma2) products are accumulated by ma1 + ma2 =
+// ma + (l1 + l2 - l)/2 into registers with static indices:
+//   Z[ma][mb] = sum_{mb1+mb2=mb+s} C[mb1][mb2] sum_{ma1+ma2=ma+s} C[ma1][ma2] U1[mb1][ma1] U2[mb2][ma2].
+// Workgroup ids are mapped so that the chunks of one tile run on the same XCD
+// (workgroups are dispatched to the 8 XCDs round-robin), sharing its L2.
+// Compile-time loop: f(std::integral_constant<int, 0>) ... f(<N-1>).
+template <typename F, int... I>
+__device__ inline void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ inline void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+template <int TWOJ>
+__device__ inline void load_urow(const cd *__restrict__ ut, int lv, int mb, int hoffl, int lane, cd *u) {
+    const bool flip = 2 * mb > lv;
+    const int mrow = flip ? lv - mb : mb;
+    const int base = hoffl + mrow * (lv + 1);
+#pragma unroll
+    for (int ma = 0; ma <= TWOJ; ma++) {
+        if (ma <= lv) {
+            const int col = flip ? lv - ma : ma;
+            cd v = ut[(base + col) * 64 + lane];
+            if (flip) {
+                const double sg = ((ma + mb) & 1) ? -1.0 : 1.0;
+                v = cmake(sg * v.re, -sg * v.im);
+            }
+            u[ma] = v;
+        } else {
+            u[ma] = cmake(0.0, 0.0);
+        }
+    }
+}
+
+template <int TWOJ>
+__global__ __launch_bounds__(64) void so4_z_tile_kernel(int64_t natoms, int ntiles, int nch, const int *__restrict__ zchunk,
+                                                         const int *__restrict__ zrows, const double *__restrict__ cg,
+                                                         const double *__restrict__ utile, double *__restrict__ zout,
+                                                         int nz) {
+    constexpr int NH = nhalf_of(TWOJ);
+    constexpr int NT = 2 * TWOJ + 1;
+    const int b = blockIdx.x;
+    const int xcd = b & 7, k = b >> 3;
+    const int tile = xcd + 8 * (k / nch), ch = k % nch;
+    if (tile >= ntiles) return;
+    const int lane = threadIdx.x;
+    const int64_t atom = (int64_t)tile * 64 + lane;
+    const bool valid = atom < natoms;
+    const cd *ut = reinterpret_cast<const cd *>(utile) + (size_t)tile * NH * 64;
+    cd *zo = reinterpret_cast<cd *>(zout) + (size_t)atom * nz;
+    for (int r = zchunk[ch]; r < zchunk[ch + 1]; r++) {
+        const int *row = zrows + 12 * r;
+        const int zrow = row[0], l1 = row[1], l2 = row[2], l = row[3], sh = row[4], mb = row[5];
+        const int mb1lo = row[6], nbb = row[7], cgblk = row[8], h1 = row[9], h2 = row[10];
+        const int l2p = l2 + 1;
+        cd acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[t] = cmake(0.0, 0.0);
+        for (int ib = 0; ib < nbb; ib++) {
+            const int mb1 = mb1lo + ib, mb2 = mb + sh - mb1;
+            cd u1[TWOJ + 1], u2[TWOJ + 1];
+            load_urow<TWOJ>(ut, l1, mb1, h1, lane, u1);
+            load_urow<TWOJ>(ut, l2, mb2, h2, lane, u2);
+            const double cgb = cg[cgblk + mb1 * l2p + mb2];
+            static_for<TWOJ + 1>([&](auto A) {
+                constexpr int a = decltype(A)::value;
+                if (a <= l1) {
+                    static_for<TWOJ + 1>([&](auto C) {
+                        constexpr int c = decltype(C)::value;
+                        const int t = a + c;
+                        if (c <= l2 && t >= sh && t <= sh + l) {
+                            const double w = cgb * cg[cgblk + a * l2p + c];
+                            const cd pr = cmul(u1[a], u2[c]);
+                            acc[a + c].re += w * pr.re;
+                            acc[a + c].im += w * pr.im;
+                        }
+                    });
+                }
+            });
+        }
+        if (valid) {
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                if (t >= sh && t <= sh + l) zo[zrow + t - sh] = acc[t];
+        }
+    }
+}
+
+// B (compute_bi, SNAP.py:1101-1141) from the tile-major U_tot and Z: one
+// thread per (atom, b).
+template <int TWOJ>
+__global__ void so4_b_kernel(int64_t natoms, const double *__restrict__ utile, const double *__restrict__ z, int nz,
+                             int nb, const int *__restrict__ b_zoff, const int *__restrict__ b_level,
+                             double *__restrict__ bout) {
+    constexpr int NH = nhalf_of(TWOJ);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= natoms * nb) return;
+    const int64_t i = t / nb;
+    const int b = (int)(t - i * nb);
+    const cd *ut = reinterpret_cast<const cd *>(utile) + (size_t)(i >> 6) * NH * 64 + (i & 63);
+    const cd *zb = reinterpret_cast<const cd *>(z) + (size_t)i * nz + b_zoff[b];
+    const int l = b_level[b];
+    const int h0 = hoff_of(l);
+    double s = 0.0;
+    int llz = 0;
+    for (int mb = 0; 2 * mb < l; mb++)
+        for (int ma = 0; ma <= l; ma++, llz++) s += cdotre(ut[(h0 + llz) * 64], zb[llz]);
+    if (l % 2 == 0) {
+        const int mb = l / 2;
+        for (int ma = 0; ma < mb; ma++, llz++) s += cdotre(ut[(h0 + llz) * 64], zb[llz]);
+        s += 0.5 * cdotre(ut[(h0 + llz) * 64], zb[llz]);
+    }
+    bout[t] = 2.0 * s;
 }
 
 // ---------------------------------------------------------------------- Y
@@ -892,7 +1026,7 @@ extern "C" int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64
     hipStream_t s = (hipStream_t)stream;
     const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * 16);
     const double fcut0 = cutoff_f0_host(p->cutoff_id);
-    PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, false><<<grid, 64, 0, s>>>(
+    PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, false, true><<<grid, 64, 0, s>>>(
                                     natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_site_w, nullptr, d_work_order,
                                     T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, fcut0, d_utot)));
     PXFF_LAUNCH_CHECK();
@@ -907,15 +1041,16 @@ extern "C" int pxff_so4_zb(const pxff_so4_params *p, int64_t natoms, const doubl
     const So4Dev *T = nullptr;
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    const size_t lds = sizeof(double) * (((T->ncg + 1) & ~1) + 2 * (size_t)T->nu);
-    const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * 4);
-    PXFF_DISPATCH_TWOJ(p->twol, {
-        auto kern = so4_zb_kernel<TJ>;
-        if (lds > 65536) PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        kern<<<grid, 256, lds, s>>>(natoms, d_utot, T->cg, T->ncg, T->zsched, T->nz, T->b_zoff, T->b_level, T->nb,
-                                    d_z, d_b);
-    });
+    const int ntiles = (int)((natoms + 63) / 64);
+    const int grid = 8 * ((ntiles + 7) / 8) * T->nzchunks;
+    PXFF_DISPATCH_TWOJ(p->twol, (so4_z_tile_kernel<TJ><<<grid, 64, 0, s>>>(natoms, ntiles, T->nzchunks, T->zchunk,
+                                                                          T->zrows, T->cg, d_utot, d_z, T->nz)));
     PXFF_LAUNCH_CHECK();
+    if (d_b) {
+        PXFF_DISPATCH_TWOJ(p->twol, (so4_b_kernel<TJ><<<div_up(natoms * T->nb, 256), 256, 0, s>>>(
+                                        natoms, d_utot, d_z, T->nz, T->nb, T->b_zoff, T->b_level, d_b)));
+        PXFF_LAUNCH_CHECK();
+    }
     return 0;
 }
 
@@ -974,7 +1109,7 @@ extern "C" int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const 
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
     const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * 16);
-    PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true><<<grid, 64, 0, s>>>(
+    PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, false><<<grid, 64, 0, s>>>(
                                     natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr, d_dr, nullptr,
                                     T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, 0.0, d_dutot)));
     PXFF_LAUNCH_CHECK();

@@ -236,11 +236,45 @@ bool build_so4_tables(int twol, int normalize_u, So4HostTables &t) {
     }
     for (int l = 0; l < ldim; l++) t.blk_level_start[l + 1] += t.blk_level_start[l];
 
+    // Z rows for the atom-per-lane kernel, in idxz order
+    for (int l1 = 0; l1 < ldim; l1++)
+        for (int l2 = 0; l2 <= l1; l2++)
+            for (int l = l1 - l2; l <= std::min(twol, l1 + l2); l += 2) {
+                const int s2 = (l1 + l2 - l) / 2;
+                for (int mb = 0; 2 * mb <= l; mb++) {
+                    const int lo = std::max(0, mb + s2 - l2), hi = std::min(l1, mb + s2);
+                    int pairs = 0;
+                    for (int a = 0; a <= l1; a++)
+                        for (int b = 0; b <= l2; b++)
+                            if (a + b >= s2 && a + b <= s2 + l) pairs++;
+                    const int row[12] = {t.idxz_block[I3(l1, l2, l)] + mb * (l + 1), l1, l2, l, s2, mb, lo, hi - lo + 1,
+                                         t.idxcg_block[I3(l1, l2, l)], t.hoff[l1], t.hoff[l2],
+                                         (hi - lo + 1) * (pairs + l1 + l2 + 2) + (l + 1)};
+                    t.zrows.insert(t.zrows.end(), row, row + 12);
+                    t.nzrows++;
+                }
+            }
+
     // init_rootpqarray, SNAP.py:861-868
     t.rootpq.assign(ldim * ldim, 0.0);
     for (int p = 1; p < ldim; p++)
         for (int q = 1; q < ldim; q++) t.rootpq[p * ldim + q] = std::sqrt((double)p / (double)q);
     return true;
+}
+
+std::vector<int> chunk_zrows(const So4HostTables &t, int nchunks) {
+    long total = 0;
+    for (int r = 0; r < t.nzrows; r++) total += t.zrows[12 * r + 11];
+    std::vector<int> start(1, 0);
+    long acc = 0;
+    for (int r = 0; r < t.nzrows; r++) {
+        acc += t.zrows[12 * r + 11];
+        if ((int)start.size() < nchunks && acc * nchunks >= total * (long)start.size() && r + 1 < t.nzrows)
+            start.push_back(r + 1);
+    }
+    while ((int)start.size() < nchunks) start.push_back(t.nzrows);
+    start.push_back(t.nzrows);
+    return start;
 }
 
 }  // namespace pxff

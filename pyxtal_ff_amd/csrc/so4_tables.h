@@ -37,7 +37,14 @@ struct So4HostTables {
     std::vector<int> h_level;            // [nhalf] level of each half element
     std::vector<double> h_weight;        // [nhalf] half weight (1, 0.5 or 0)
     std::vector<int> h_diag;             // [nhalf] 1 if mb == ma
+    // Z rows (block, mb) for the atom-per-lane kernel: per row
+    // {zrow, l1, l2, l, s, mb, mb1lo, nbb, cgblock, hoff[l1], hoff[l2], cost}
+    std::vector<int> zrows;              // [nzrows*12]
+    int nzrows = 0;
 };
+
+// Split the Z rows into nchunks contiguous chunks of about equal cost.
+std::vector<int> chunk_zrows(const So4HostTables &t, int nchunks);
 
 // Build the tables.  Returns false for unsupported 2J.
 bool build_so4_tables(int twol, int normalize_u, So4HostTables &t);

@@ -39,28 +39,20 @@ def _geometry(cell, pbc, rc):
     return inv, geom
 
 
-class DeviceNeighborList:
-    """Neighbour images and seq rows of a batch of structures on the GPU.
+class NeighborListBuilder:
+    """Structure metadata (cells, bin grids, atom ranges) and workspace for
+    repeated neighbour-list builds of the same set of structures -- e.g. the
+    steps of an MD run or a training epoch.  ``build(pos)`` redoes the whole
+    list from the given positions; only the metadata is reused."""
 
-    positions: list of [N_s, 3] arrays, cells: [nstruct, 3, 3],
-    pbcs: [nstruct, 3].  With ``pos_device`` (a float64 [N, 3] device tensor
-    of a single structure) positions are taken from HBM and ``positions``
-    must be None."""
-
-    def __init__(self, positions, cells, pbcs, rc, device="cuda", pos_device=None):
-        if pos_device is not None:
-            if positions is not None:
-                raise ValueError("give positions or pos_device, not both")
-            pos_list = [np.zeros((pos_device.shape[0], 0))]
-        else:
-            pos_list = [np.asarray(p, dtype=np.float64).reshape(-1, 3) for p in positions]
+    def __init__(self, natoms_per_struct, cells, pbcs, rc, device="cuda"):
+        counts = np.asarray(natoms_per_struct, dtype=np.int64).reshape(-1)
         cells = np.asarray(cells, dtype=np.float64).reshape(-1, 3, 3)
         pbcs = np.asarray(pbcs, dtype=bool).reshape(-1, 3)
-        nst = len(pos_list)
+        nst = len(counts)
         if cells.shape[0] != nst or pbcs.shape[0] != nst:
             raise ValueError("positions, cells and pbcs must describe the same structures")
         self.rc = float(rc)
-        counts = np.array([len(p) for p in pos_list], dtype=np.int64)
         atom_start = np.zeros(nst + 1, dtype=np.int64)
         atom_start[1:] = np.cumsum(counts)
         self.natoms = int(atom_start[-1])
@@ -75,40 +67,74 @@ class DeviceNeighborList:
             g[6] = off
             off += int(g[0]) * int(g[1]) * int(g[2])
             geoms[s] = g
-        nbins = off
+        self.nbins = off
         dev = torch.device(device)
+        self.device = dev
         f64 = dict(dtype=torch.float64, device=dev)
         i64 = dict(dtype=torch.int64, device=dev)
         i32 = dict(dtype=torch.int32, device=dev)
-        if pos_device is not None:
-            if pos_device.dtype != torch.float64 or not pos_device.is_cuda:
-                raise ValueError("pos_device must be a float64 device tensor")
-            self.pos = pos_device.contiguous()
-        else:
-            pos_all = np.concatenate(pos_list, axis=0) if nst else np.zeros((0, 3))
-            self.pos = torch.as_tensor(pos_all, **f64).contiguous()
         self.atom_struct = torch.as_tensor(np.repeat(np.arange(nst), counts), **i64)
         self.atom_start = torch.as_tensor(atom_start, **i64)
         self.cells = torch.as_tensor(cells.reshape(nst, 9), **f64).contiguous()
         self.inv_cells = torch.as_tensor(invs, **f64).contiguous()
         self.pbc = torch.as_tensor(pbcs.astype(np.int32), **i32).contiguous()
         self.geom = torch.as_tensor(geoms, **i32).contiguous()
-        vols = np.abs(np.linalg.det(cells)) if nst else np.zeros(0)
-        self.volumes = vols
-        self.inv_vol = torch.as_tensor(np.repeat(1.0 / vols, counts), **f64)
-        self._build(nbins, f64, i64, i32)
+        self.volumes = np.abs(np.linalg.det(cells)) if nst else np.zeros(0)
+        self.inv_vol = torch.as_tensor(np.repeat(1.0 / self.volumes, counts), **f64)
+        n = self.natoms
+        nb = max(self.nbins, 1)
+        self._ws = dict(bin_count=torch.empty(nb, **i32), bin_start=torch.empty(nb + 1, **i32),
+                        atom_bin=torch.empty(max(n, 1), **i32), wrap=torch.empty(max(n, 1) * 3, **i32),
+                        bin_atoms=torch.empty(max(n, 1), **i32), img_count=torch.empty(max(n, 1), **i64))
+
+    def build(self, pos):
+        """A DeviceNeighborList of float64 device positions pos [natoms, 3]."""
+        return DeviceNeighborList(None, None, None, self.rc, pos_device=pos, builder=self)
+
+
+class DeviceNeighborList:
+    """Neighbour images and seq rows of a batch of structures on the GPU.
+
+    positions: list of [N_s, 3] arrays, cells: [nstruct, 3, 3],
+    pbcs: [nstruct, 3].  With ``pos_device`` (a float64 [N, 3] device tensor)
+    positions are taken from HBM; ``builder`` supplies cached metadata."""
+
+    def __init__(self, positions, cells, pbcs, rc, device="cuda", pos_device=None, builder=None):
+        if builder is None:
+            if pos_device is not None:
+                if positions is not None:
+                    raise ValueError("give positions or pos_device, not both")
+                counts = [pos_device.shape[0]]
+            else:
+                positions = [np.asarray(p, dtype=np.float64).reshape(-1, 3) for p in positions]
+                counts = [len(p) for p in positions]
+            builder = NeighborListBuilder(counts, cells, pbcs, rc, device)
+        b = builder
+        self.builder = b
+        self.rc = b.rc
+        self.natoms, self.nstruct = b.natoms, b.nstruct
+        self.atom_start_host, self.atom_struct, self.atom_start = b.atom_start_host, b.atom_struct, b.atom_start
+        self.cells, self.inv_cells, self.pbc, self.geom = b.cells, b.inv_cells, b.pbc, b.geom
+        self.volumes, self.inv_vol = b.volumes, b.inv_vol
+        f64 = dict(dtype=torch.float64, device=b.device)
+        i64 = dict(dtype=torch.int64, device=b.device)
+        i32 = dict(dtype=torch.int32, device=b.device)
+        if pos_device is not None:
+            if pos_device.dtype != torch.float64 or not pos_device.is_cuda or pos_device.shape != (b.natoms, 3):
+                raise ValueError("pos_device must be a float64 [natoms, 3] device tensor")
+            self.pos = pos_device.contiguous()
+        else:
+            pos_all = np.concatenate(positions, axis=0) if b.nstruct else np.zeros((0, 3))
+            self.pos = torch.as_tensor(pos_all, **f64).contiguous()
+        self._build(b.nbins, f64, i64, i32)
 
     def _build(self, nbins, f64, i64, i32):
         n = self.natoms
         s = nat.stream()
         L = nat.lib()
-        e32 = dict(dtype=torch.int32, device=self.pos.device)
-        bin_count = torch.empty(max(nbins, 1), **e32)
-        bin_start = torch.empty(max(nbins, 1) + 1, **e32)
-        atom_bin = torch.empty(max(n, 1), **e32)
-        wrap = torch.empty(max(n, 1) * 3, **e32)
-        bin_atoms = torch.empty(max(n, 1), **e32)
-        img_count = torch.empty(max(n, 1), **i64)
+        ws = self.builder._ws
+        bin_count, bin_start, atom_bin = ws["bin_count"], ws["bin_start"], ws["atom_bin"]
+        wrap, bin_atoms, img_count = ws["wrap"], ws["bin_atoms"], ws["img_count"]
         p = nat.ptr
         nat.check(L.pxff_nlist_count(p(self.pos), n, p(self.atom_struct), p(self.atom_start), p(self.cells),
                                      p(self.inv_cells), p(self.pbc), p(self.geom), max(nbins, 1), self.rc,

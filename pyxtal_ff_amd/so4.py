@@ -46,9 +46,11 @@ def _f64(n, dev):
 
 
 def ui(nl, cfg, atom_w, site_w, work_order=None, img_r=None):
-    """U_tot half rows [N, nhalf] complex (as [N, nhalf, 2] float64)."""
+    """U_tot half rows, tile-major: [ceil(N/64), nhalf, 64] complex (float64
+    pairs), padding lanes zero."""
     p = nat.ptr
-    out = _f64(nl.natoms * cfg.nhalf * 2, nl.pos.device)
+    ntiles = (nl.natoms + 63) // 64
+    out = torch.zeros(max(ntiles * 64 * cfg.nhalf * 2, 1), dtype=torch.float64, device=nl.pos.device)
     r = nl.img_r if img_r is None else img_r.contiguous()
     nat.check(nat.lib().pxff_so4_ui(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r), p(atom_w),
                                     p(site_w), p(work_order), p(out), nat.stream()), "pxff_so4_ui")
