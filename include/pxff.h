@@ -33,6 +33,17 @@ extern "C" {
 #define PXFF_EHIP -2     /* HIP runtime error */
 #define PXFF_ENOTIMPL -3 /* unsupported parameter (e.g. odd 2J) */
 
+/* Launch-parameter knobs (process-wide), set by the autotuner:
+ *   PXFF_TUNE_Z_CHUNKS              Z rows split into this many wavefront chunks per 64-atom tile (1..256)
+ *   PXFF_TUNE_ACCUM_WAVES_PER_CU    persistent grid of the per-centre kernels = CUs x this (1..256)
+ *   PXFF_TUNE_IMAGE_WAVES_PER_BLOCK wavefronts per workgroup of the per-image adjoint kernel (1, 2, 4) */
+#define PXFF_TUNE_Z_CHUNKS 0
+#define PXFF_TUNE_ACCUM_WAVES_PER_CU 1
+#define PXFF_TUNE_IMAGE_WAVES_PER_BLOCK 2
+#define PXFF_TUNE_COUNT 3
+int pxff_tune_set(int knob, int value);
+int pxff_tune_get(int knob); /* -1 for an unknown knob */
+
 /* Copy the calling thread's last error message into buf (NUL-terminated). */
 int pxff_last_error(char *buf, size_t len);
 
@@ -130,6 +141,12 @@ int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_s
  * d_b [natoms*nb] out (or NULL). */
 int pxff_so4_zb(const pxff_so4_params *p, int64_t natoms, const double *d_utot, double *d_z, double *d_b,
                 void *stream);
+
+/* B alone (compute_bi, SNAP.py:1101-1141) from the tile-major U_tot and Z:
+ * d_b [natoms*nb] out.  pxff_so4_zb with d_b != NULL is pxff_so4_zb with
+ * d_b == NULL followed by this call. */
+int pxff_so4_b(const pxff_so4_params *p, int64_t natoms, const double *d_utot, const double *d_z, double *d_b,
+               void *stream);
 
 /* Adjoint, step 1: Y [natoms*nhalf*2] = sum over blocks of the dE/dB-weighted
  * Z (half weights, factor 2 and u_norm folded in).  d_beta [natoms*nb]. */

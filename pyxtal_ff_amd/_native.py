@@ -38,6 +38,8 @@ _SIGS = {
     "pxff_last_error": ([ctypes.c_char_p, ctypes.c_size_t], I32),
     "pxff_version": ([], ctypes.c_char_p),
     "pxff_scan_i64": ([P, I64, P, P], I32),
+    "pxff_tune_set": ([ctypes.c_int, ctypes.c_int], I32),
+    "pxff_tune_get": ([ctypes.c_int], I32),
     "pxff_nlist_count": ([P, I64, P, P, P, P, P, P, I64, F64, P, P, P, P, P, P, P], I32),
     "pxff_nlist_fill": ([P, I64, P, P, P, P, P, F64, P, P, P, P, P, P, P, P, P, P, P], I32),
     "pxff_nlist_rev": ([I64, I64, P, P, P, P, P, P, P], I32),
@@ -45,6 +47,7 @@ _SIGS = {
     "pxff_so4_sizes": ([ctypes.c_int, P, P, P, P], I32),
     "pxff_so4_ui": ([P, I64, P, P, P, P, P, P, P, P], I32),
     "pxff_so4_zb": ([P, I64, P, P, P, P], I32),
+    "pxff_so4_b": ([P, I64, P, P, P, P], I32),
     "pxff_so4_yi": ([P, I64, P, P, P, P], I32),
     "pxff_so4_deidrj": ([P, I64, P, P, P, P, P, P, P], I32),
     "pxff_so4_gather": ([I64, P, P, P, P, P, P, P], I32),

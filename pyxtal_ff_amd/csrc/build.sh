@@ -1,5 +1,6 @@
 #!/bin/bash
 # Build libpxff.so for gfx950 (hipcc).  Usage: build.sh [outdir]
+# Translation units compile in parallel; the link waits for all of them.
 set -euo pipefail
 HERE="$(cd "$(dirname "$0")" && pwd)"
 OUT="${1:-$HERE/../_lib}"
@@ -8,16 +9,19 @@ HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
 # -packed-fp32-ops: hipcc must not emit packed FP32 instructions on this pool
 FLAGS=(-O3 -std=c++17 -fPIC --offload-arch=gfx950 -Xclang -target-feature -Xclang -packed-fp32-ops -Wall -Wno-unused-result)
 OBJS=()
+PIDS=()
 for src in pxff_api.cpp so4_tables.cpp nlist.hip so4_kernels.hip so3_kernels.hip; do
   [ -f "$HERE/$src" ] || continue
   obj="$HERE/_build/${src%.*}.o"
   if [[ "$src" == *.hip ]]; then
-    "$HIPCC" "${FLAGS[@]}" -c "$HERE/$src" -o "$obj"
+    "$HIPCC" "${FLAGS[@]}" -c "$HERE/$src" -o "$obj" &
   else
-    "$HIPCC" "${FLAGS[@]}" -x hip -c "$HERE/$src" -o "$obj"
+    "$HIPCC" "${FLAGS[@]}" -x hip -c "$HERE/$src" -o "$obj" &
   fi
+  PIDS+=($!)
   OBJS+=("$obj")
 done
+for p in "${PIDS[@]}"; do wait "$p"; done
 "$HIPCC" -shared --offload-arch=gfx950 -o "$OUT/libpxff.so.tmp" "${OBJS[@]}"
 mv "$OUT/libpxff.so.tmp" "$OUT/libpxff.so"
 echo "built $OUT/libpxff.so"

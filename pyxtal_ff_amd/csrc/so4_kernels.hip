@@ -23,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -105,6 +106,39 @@ int get_so4_tables(int twol, int norm, const So4Dev **out) {
     if ((rc = upload(chunk_zrows(t, d.nzchunks), &d.zchunk))) return rc;
     g_tabs[key] = d;
     *out = &g_tabs[key];
+    return 0;
+}
+
+// ------------------------------------------------------------ tuning knobs
+// Process-wide launch parameters set by the autotuner (pxff_tune_set).
+static int g_tune[PXFF_TUNE_COUNT] = {16, 16, 4};
+static const int g_tune_lo[PXFF_TUNE_COUNT] = {1, 1, 1};
+static const int g_tune_hi[PXFF_TUNE_COUNT] = {256, 256, 4};
+
+// Z-row chunk tables per (device, 2J, normalisation, nchunks)
+static std::mutex g_chunk_mutex;
+static std::map<std::tuple<int, int, int>, int *> g_chunks;
+static int get_zchunks(int twol, int norm, int nch, const int **out) {
+    int dev = 0;
+    PXFF_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_chunk_mutex);
+    auto key = std::make_tuple(dev, twol * 2 + (norm ? 1 : 0), nch);
+    auto it = g_chunks.find(key);
+    if (it != g_chunks.end()) {
+        *out = it->second;
+        return 0;
+    }
+    So4HostTables t;
+    if (!build_so4_tables(twol, norm, t)) {
+        set_error("unsupported 2J");
+        return PXFF_ENOTIMPL;
+    }
+    std::vector<int> c = chunk_zrows(t, nch);
+    int *d = nullptr;
+    PXFF_HIP_CHECK(hipMalloc((void **)&d, c.size() * sizeof(int)));
+    PXFF_HIP_CHECK(hipMemcpy(d, c.data(), c.size() * sizeof(int), hipMemcpyHostToDevice));
+    g_chunks[key] = d;
+    *out = d;
     return 0;
 }
 
@@ -500,57 +534,105 @@ __device__ inline void load_urow(const cd *__restrict__ ut, int lv, int mb, int 
     }
 }
 
+// One Z row (block (L1, L2, l), row mb) for the lane's atom, with the (l1,
+// l2) of the block known at compile time: the loads of U rows are exact,
+// and the pairs (a, c) of one t = a + c are branch-free (only the t range
+// [sh, sh+l], which depends on l, is tested -- one uniform branch per t).
+template <int TWOJ, int L1, int L2>
+__device__ __forceinline__ void zrow(const cd *__restrict__ ut, int lane, const double *__restrict__ cg, int cgblk, int l,
+                                  int sh, int mb, int mb1lo, int nbb, int h1, int h2, cd *__restrict__ zo, bool valid,
+                                  int zrow0) {
+    constexpr int NT = L1 + L2 + 1;
+    constexpr int L2P = L2 + 1;
+    cd acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = cmake(0.0, 0.0);
+    const double *cgr = cg + cgblk;
+    for (int ib = 0; ib < nbb; ib++) {
+        const int mb1 = mb1lo + ib, mb2 = mb + sh - mb1;
+        cd u1[L1 + 1], u2[L2 + 1];
+        load_urow<L1>(ut, L1, mb1, h1, lane, u1);
+        load_urow<L2>(ut, L2, mb2, h2, lane, u2);
+        const double cgb = cgr[mb1 * L2P + mb2];
+        // keep the CG reads of the pair loop inside the row loop: hoisted,
+        // (L1+1)(L2+1) doubles would not fit in registers beside acc and U
+        asm volatile("" ::: "memory");
+        static_for<NT>([&](auto T) {
+            constexpr int t = decltype(T)::value;
+            if (t >= sh && t <= sh + l) {
+                static_for<L1 + 1>([&](auto A) {
+                    constexpr int a = decltype(A)::value;
+                    constexpr int c = t - a;
+                    if constexpr (c >= 0 && c <= L2) {
+                        const double w = cgb * cgr[a * L2P + c];
+                        const cd pr = cmul(u1[a], u2[c]);
+                        acc[t].re += w * pr.re;
+                        acc[t].im += w * pr.im;
+                    }
+                });
+            }
+        });
+    }
+    if (valid) {
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+            if (t >= sh && t <= sh + l) zo[zrow0 + t - sh] = acc[t];
+    }
+}
+
+// Uniform dispatch to zrow<TWOJ, l1, l2>.
+template <int TWOJ, int L1, int L2>
+__device__ inline void zrow_dispatch(int l1, int l2, const cd *ut, int lane, const double *cg, int cgblk, int l, int sh,
+                                     int mb, int mb1lo, int nbb, int h1, int h2, cd *zo, bool valid, int zrow0) {
+    if constexpr (L1 <= TWOJ) {
+        if (l1 == L1 && l2 == L2) {
+            zrow<TWOJ, L1, L2>(ut, lane, cg, cgblk, l, sh, mb, mb1lo, nbb, h1, h2, zo, valid, zrow0);
+            return;
+        }
+        if constexpr (L2 < L1)
+            zrow_dispatch<TWOJ, L1, L2 + 1>(l1, l2, ut, lane, cg, cgblk, l, sh, mb, mb1lo, nbb, h1, h2, zo, valid, zrow0);
+        else
+            zrow_dispatch<TWOJ, L1 + 1, 0>(l1, l2, ut, lane, cg, cgblk, l, sh, mb, mb1lo, nbb, h1, h2, zo, valid, zrow0);
+    }
+}
+
 template <int TWOJ>
-__global__ __launch_bounds__(64) void so4_z_tile_kernel(int64_t natoms, int ntiles, int nch, const int *__restrict__ zchunk,
-                                                         const int *__restrict__ zrows, const double *__restrict__ cg,
-                                                         const double *__restrict__ utile, double *__restrict__ zout,
-                                                         int nz) {
+__global__ __launch_bounds__(256) void so4_z_tile_kernel(int64_t natoms, int ntiles, int nch, const int *__restrict__ zchunk,
+                                                          const int *__restrict__ zrows, const double *__restrict__ cg_g,
+                                                          int ncg, const double *__restrict__ utile,
+                                                          double *__restrict__ zout, int nz) {
     constexpr int NH = nhalf_of(TWOJ);
-    constexpr int NT = 2 * TWOJ + 1;
+    extern __shared__ double cg[];  // the whole CG table; uniform-address reads broadcast
+    for (int k = threadIdx.x; k < ncg; k += blockDim.x) cg[k] = cg_g[k];
+    __syncthreads();
+    // workgroup = 4 wavefronts on 4 consecutive chunks of one tile
     const int b = blockIdx.x;
+    const int nq = (nch + 3) / 4;
     const int xcd = b & 7, k = b >> 3;
-    const int tile = xcd + 8 * (k / nch), ch = k % nch;
-    if (tile >= ntiles) return;
-    const int lane = threadIdx.x;
+    const int tile = xcd + 8 * (k / nq);
+    const int ch = 4 * (k % nq) + (threadIdx.x >> 6);
+    if (tile >= ntiles || ch >= nch) return;
+    const int lane = threadIdx.x & 63;
     const int64_t atom = (int64_t)tile * 64 + lane;
     const bool valid = atom < natoms;
     const cd *ut = reinterpret_cast<const cd *>(utile) + (size_t)tile * NH * 64;
     cd *zo = reinterpret_cast<cd *>(zout) + (size_t)atom * nz;
-    for (int r = zchunk[ch]; r < zchunk[ch + 1]; r++) {
+    const int r0 = __builtin_amdgcn_readfirstlane(zchunk[ch]);
+    const int r1 = __builtin_amdgcn_readfirstlane(zchunk[ch + 1]);
+    for (int r = r0; r < r1; r++) {
         const int *row = zrows + 12 * r;
-        const int zrow = row[0], l1 = row[1], l2 = row[2], l = row[3], sh = row[4], mb = row[5];
-        const int mb1lo = row[6], nbb = row[7], cgblk = row[8], h1 = row[9], h2 = row[10];
-        const int l2p = l2 + 1;
-        cd acc[NT];
-#pragma unroll
-        for (int t = 0; t < NT; t++) acc[t] = cmake(0.0, 0.0);
-        for (int ib = 0; ib < nbb; ib++) {
-            const int mb1 = mb1lo + ib, mb2 = mb + sh - mb1;
-            cd u1[TWOJ + 1], u2[TWOJ + 1];
-            load_urow<TWOJ>(ut, l1, mb1, h1, lane, u1);
-            load_urow<TWOJ>(ut, l2, mb2, h2, lane, u2);
-            const double cgb = cg[cgblk + mb1 * l2p + mb2];
-            static_for<TWOJ + 1>([&](auto A) {
-                constexpr int a = decltype(A)::value;
-                if (a <= l1) {
-                    static_for<TWOJ + 1>([&](auto C) {
-                        constexpr int c = decltype(C)::value;
-                        const int t = a + c;
-                        if (c <= l2 && t >= sh && t <= sh + l) {
-                            const double w = cgb * cg[cgblk + a * l2p + c];
-                            const cd pr = cmul(u1[a], u2[c]);
-                            acc[a + c].re += w * pr.re;
-                            acc[a + c].im += w * pr.im;
-                        }
-                    });
-                }
-            });
-        }
-        if (valid) {
-#pragma unroll
-            for (int t = 0; t < NT; t++)
-                if (t >= sh && t <= sh + l) zo[zrow + t - sh] = acc[t];
-        }
+        const int zrow0 = __builtin_amdgcn_readfirstlane(row[0]);
+        const int l1 = __builtin_amdgcn_readfirstlane(row[1]);
+        const int l2 = __builtin_amdgcn_readfirstlane(row[2]);
+        const int l = __builtin_amdgcn_readfirstlane(row[3]);
+        const int sh = __builtin_amdgcn_readfirstlane(row[4]);
+        const int mb = __builtin_amdgcn_readfirstlane(row[5]);
+        const int mb1lo = __builtin_amdgcn_readfirstlane(row[6]);
+        const int nbb = __builtin_amdgcn_readfirstlane(row[7]);
+        const int cgblk = __builtin_amdgcn_readfirstlane(row[8]);
+        const int h1 = __builtin_amdgcn_readfirstlane(row[9]);
+        const int h2 = __builtin_amdgcn_readfirstlane(row[10]);
+        zrow_dispatch<TWOJ, 0, 0>(l1, l2, ut, lane, cg, cgblk, l, sh, mb, mb1lo, nbb, h1, h2, zo, valid, zrow0);
     }
 }
 
@@ -635,10 +717,11 @@ __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int
     __shared__ double red[4][64][4];
     const int tid = threadIdx.x;
     const int wv = tid >> 6, lane = tid & 63;
+    const int wpb = blockDim.x >> 6;
     const int g = lane / G, mb = lane % G;
-    for (int k = tid; k < (TWOJ + 1) * (TWOJ + 1); k += 256) rpq[k] = rootpq_g[k];
+    for (int k = tid; k < (TWOJ + 1) * (TWOJ + 1); k += blockDim.x) rpq[k] = rootpq_g[k];
     __syncthreads();
-    const int64_t img = ((int64_t)blockIdx.x * 4 + wv) * NG + g;
+    const int64_t img = ((int64_t)blockIdx.x * wpb + wv) * NG + g;
     bool valid = (g < NG) && (img < nimg);
     double x = 0, yy = 0, zz = 1.0, r = 1.0, w = 0.0;
     int64_t i = 0;
@@ -1024,7 +1107,7 @@ extern "C" int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64
     const So4Dev *T = nullptr;
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * 16);
+    const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * g_tune[PXFF_TUNE_ACCUM_WAVES_PER_CU]);
     const double fcut0 = cutoff_f0_host(p->cutoff_id);
     PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, false, true><<<grid, 64, 0, s>>>(
                                     natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_site_w, nullptr, d_work_order,
@@ -1042,15 +1125,34 @@ extern "C" int pxff_so4_zb(const pxff_so4_params *p, int64_t natoms, const doubl
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
     const int ntiles = (int)((natoms + 63) / 64);
-    const int grid = 8 * ((ntiles + 7) / 8) * T->nzchunks;
-    PXFF_DISPATCH_TWOJ(p->twol, (so4_z_tile_kernel<TJ><<<grid, 64, 0, s>>>(natoms, ntiles, T->nzchunks, T->zchunk,
-                                                                          T->zrows, T->cg, d_utot, d_z, T->nz)));
+    const int nch = std::min(g_tune[PXFF_TUNE_Z_CHUNKS], T->nzrows);
+    const int *zch = nullptr;
+    if ((rc = get_zchunks(p->twol, p->normalize_u, nch, &zch))) return rc;
+    const int grid = 8 * ((ntiles + 7) / 8) * ((nch + 3) / 4);
+    const size_t lds = sizeof(double) * (size_t)T->ncg;
+    PXFF_DISPATCH_TWOJ(p->twol, {
+        auto kern = so4_z_tile_kernel<TJ>;
+        if (lds > 65536)
+            PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        kern<<<grid, 256, lds, s>>>(natoms, ntiles, nch, zch, T->zrows, T->cg, T->ncg, d_utot, d_z, T->nz);
+    });
     PXFF_LAUNCH_CHECK();
-    if (d_b) {
-        PXFF_DISPATCH_TWOJ(p->twol, (so4_b_kernel<TJ><<<div_up(natoms * T->nb, 256), 256, 0, s>>>(
-                                        natoms, d_utot, d_z, T->nz, T->nb, T->b_zoff, T->b_level, d_b)));
-        PXFF_LAUNCH_CHECK();
-    }
+    if (d_b) return pxff_so4_b(p, natoms, d_utot, d_z, d_b, stream);
+    return 0;
+}
+
+extern "C" int pxff_so4_b(const pxff_so4_params *p, int64_t natoms, const double *d_utot, const double *d_z,
+                          double *d_b, void *stream) {
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (natoms <= 0) return 0;
+    PXFF_REQUIRE(d_b != nullptr, "null B output");
+    const So4Dev *T = nullptr;
+    if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    PXFF_DISPATCH_TWOJ(p->twol, (so4_b_kernel<TJ><<<div_up(natoms * T->nb, 256), 256, 0, s>>>(
+                                    natoms, d_utot, d_z, T->nz, T->nb, T->b_zoff, T->b_level, d_b)));
+    PXFF_LAUNCH_CHECK();
     return 0;
 }
 
@@ -1080,8 +1182,9 @@ extern "C" int pxff_so4_deidrj(const pxff_so4_params *p, int64_t nimg, const int
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
     const int ng = 64 / (p->twol / 2 + 1);
-    const int grid = div_up(nimg, 4 * ng);
-    PXFF_DISPATCH_TWOJ(p->twol, (so4_deidrj_kernel<TJ><<<grid, 256, 0, s>>>(nimg, d_img_atom, d_img_j, d_img_r, d_atom_w,
+    const int wpb = g_tune[PXFF_TUNE_IMAGE_WAVES_PER_BLOCK];
+    const int grid = div_up(nimg, wpb * ng);
+    PXFF_DISPATCH_TWOJ(p->twol, (so4_deidrj_kernel<TJ><<<grid, 64 * wpb, 0, s>>>(nimg, d_img_atom, d_img_j, d_img_r, d_atom_w,
                                                                            d_y, T->rootpq, p->rcut, p->rfac0,
                                                                            p->cutoff_id, d_dedr)));
     PXFF_LAUNCH_CHECK();
@@ -1108,7 +1211,7 @@ extern "C" int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const 
     const So4Dev *T = nullptr;
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * 16);
+    const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * g_tune[PXFF_TUNE_ACCUM_WAVES_PER_CU]);
     PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, false><<<grid, 64, 0, s>>>(
                                     natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr, d_dr, nullptr,
                                     T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, 0.0, d_dutot)));
@@ -1169,4 +1272,17 @@ extern "C" int pxff_rdxdr_reduce(int64_t natoms, int nb, const int64_t *d_row_st
     rdxdr_reduce_kernel<<<div_up(natoms * nb, 256), 256, 0, s>>>(natoms, nb, d_row_start, d_rdxdr, d_out);
     PXFF_LAUNCH_CHECK();
     return 0;
+}
+
+extern "C" int pxff_tune_set(int knob, int value) {
+    PXFF_REQUIRE(knob >= 0 && knob < PXFF_TUNE_COUNT, "unknown tuning knob");
+    PXFF_REQUIRE(value >= g_tune_lo[knob] && value <= g_tune_hi[knob], "tuning value out of range");
+    if (knob == PXFF_TUNE_IMAGE_WAVES_PER_BLOCK) PXFF_REQUIRE(value == 1 || value == 2 || value == 4, "waves per block must be 1, 2 or 4");
+    g_tune[knob] = value;
+    return 0;
+}
+
+extern "C" int pxff_tune_get(int knob) {
+    if (knob < 0 || knob >= PXFF_TUNE_COUNT) return -1;
+    return g_tune[knob];
 }

@@ -62,9 +62,16 @@ def zb(nl, cfg, utot, want_b=True):
     p = nat.ptr
     dev = nl.pos.device
     z = _f64(nl.natoms * cfg.nz * 2, dev)
-    b = _f64(nl.natoms * cfg.nb, dev) if want_b else None
-    nat.check(nat.lib().pxff_so4_zb(cfg.params, nl.natoms, p(utot), p(z), p(b), nat.stream()), "pxff_so4_zb")
-    return z, (b[: nl.natoms * cfg.nb].view(nl.natoms, cfg.nb) if want_b else None)
+    nat.check(nat.lib().pxff_so4_zb(cfg.params, nl.natoms, p(utot), p(z), None, nat.stream()), "pxff_so4_zb")
+    return z, (bi(nl, cfg, utot, z) if want_b else None)
+
+
+def bi(nl, cfg, utot, z):
+    """B [N, nb] from U_tot and Z."""
+    p = nat.ptr
+    b = _f64(nl.natoms * cfg.nb, nl.pos.device)
+    nat.check(nat.lib().pxff_so4_b(cfg.params, nl.natoms, p(utot), p(z), p(b), nat.stream()), "pxff_so4_b")
+    return b[: nl.natoms * cfg.nb].view(nl.natoms, cfg.nb)
 
 
 def forward(nl, cfg, atom_w, site_w, work_order=None, img_r=None):

@@ -36,25 +36,25 @@ def diamond_si(reps=2, sigma=0.05, seed=0, a=5.43):
 
 
 def _random_positions(rng, cell, n, dmin, max_tries=20000):
+    """Uniform positions in the cell, rejecting any within dmin (minimum
+    image) of an accepted one."""
     inv = np.linalg.inv(cell)
-    pts = []
+    pts = np.zeros((n, 3))
+    k = 0
     tries = 0
-    while len(pts) < n:
+    while k < n:
         tries += 1
-        if tries > max_tries * n:
+        if tries > max_tries * max(n, 1):
             raise RuntimeError("could not place atoms at the requested minimum distance")
-        f = rng.random(3)
-        p = f @ cell
-        ok = True
-        for q in pts:
-            d = p - q
+        p = rng.random(3) @ cell
+        if k:
+            d = p - pts[:k]
             d = d - np.round(d @ inv) @ cell
-            if d @ d < dmin * dmin:
-                ok = False
-                break
-        if ok:
-            pts.append(p)
-    return np.array(pts)
+            if np.min(np.einsum("ij,ij->i", d, d)) < dmin * dmin:
+                continue
+        pts[k] = p
+        k += 1
+    return pts
 
 
 def random_sio_structure(rng, nmin=8, nmax=256, density=0.066, dmin=1.5):
@@ -76,40 +76,37 @@ def random_sio_batch(nstruct=1024, seed=0, **kw):
 
 
 def amorphous_sio2(n_si=4000, n_o=8000, seed=0, density_gcc=2.2, dmin=1.6):
+    """Random SiO2 at the given mass density with a minimum distance (cubic,
+    periodic); a cell grid accelerates the rejection test."""
     rng = np.random.default_rng(seed)
     mass = (n_si * 28.0855 + n_o * 15.999) / 6.02214076e23  # g
-    vol = mass / density_gcc * 1e24  # A^3
-    L = vol ** (1.0 / 3.0)
+    L = (mass / density_gcc * 1e24) ** (1.0 / 3.0)  # A
     n = n_si + n_o
-    cell = np.eye(3) * L
-    # grid-accelerated rejection sampling
     nb = max(1, int(L // dmin))
-    grid = {}
-    pts = []
-    while len(pts) < n:
-        p = rng.random(3) * L
-        c = tuple((p / L * nb).astype(int) % nb)
-        ok = True
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    for q in grid.get(((c[0] + dx) % nb, (c[1] + dy) % nb, (c[2] + dz) % nb), ()):
-                        d = p - pts[q]
-                        d -= np.round(d / L) * L
-                        if d @ d < dmin * dmin:
-                            ok = False
-                            break
-                    if not ok:
-                        break
-                if not ok:
-                    break
-            if not ok:
+    h = L / nb
+    cells = {}
+    pts = np.zeros((n, 3))
+    k = 0
+    offs = [(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)]
+    while k < n:
+        cand = rng.random((256, 3)) * L
+        for p in cand:
+            c = tuple(int(v) % nb for v in p // h)
+            near = []
+            for o in offs:
+                near.extend(cells.get(((c[0] + o[0]) % nb, (c[1] + o[1]) % nb, (c[2] + o[2]) % nb), ()))
+            if near:
+                d = p - pts[near]
+                d -= np.round(d / L) * L
+                if np.min(np.einsum("ij,ij->i", d, d)) < dmin * dmin:
+                    continue
+            cells.setdefault(c, []).append(k)
+            pts[k] = p
+            k += 1
+            if k == n:
                 break
-        if ok:
-            grid.setdefault(c, []).append(len(pts))
-            pts.append(p)
     numbers = [14] * n_si + [8] * n_o
-    return Structure(np.array(pts), cell, numbers=numbers, pbc=True)
+    return Structure(pts, np.eye(3) * L, numbers=numbers, pbc=True)
 
 
 def nacl_cluster(angle=1.0, axis="x"):
