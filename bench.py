@@ -106,6 +106,26 @@ class Timers:
         return {k: float(np.mean([s.elapsed_time(e) for s, e in v])) for k, v in self.acc.items()}
 
 
+ADJ_KERNELS = ("ui", "z", "bi", "yi", "deidrj", "gather")
+PLAIN_KERNELS = ("dbdr", "rows", "dxdr_contract")
+# Python wrapper -> the one kernel it launches (rocprof names)
+KERNEL_SYMBOL = {"ui": "so4_accum_kernel", "z": "so4_z_tile_kernel", "bi": "so4_b_kernel", "yi": "so4_yi_kernel",
+                 "deidrj": "so4_deidrj_kernel", "gather": "so4_gather_kernel", "dbdr": "so4_dbdr_kernel",
+                 "rows": "so4_rows_kernel", "dxdr_contract": "dxdr_contract_kernel"}
+
+
+def copy_calibration(nbytes=1 << 28, reps=3):
+    """A plain device copy of known size (for scaling FETCH_SIZE / WRITE_SIZE
+    counters to bytes): reads and writes nbytes per launch."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return nbytes
+
+
 def run_gpu(steps, warmup):
     import torch
     from pyxtal_ff_amd import adjoint, potential
@@ -121,6 +141,7 @@ def run_gpu(steps, warmup):
     w = torch.ones(n, dtype=torch.float64, device=dev)
     model = potential.NNPotential(["Si"], cfg.nb, (30, 30), seed=0).to(dev)
     species = potential.species_index(s.symbols, ["Si"], dev)
+    species_atom = potential.species_atom(s.symbols, ["Si"], dev)
 
     # structure metadata (cell, pbc, bin grid) is set up once, as in an MD
     # run; every step rebuilds the neighbour list from the HBM positions
@@ -129,44 +150,63 @@ def run_gpu(steps, warmup):
     def nlist():
         return builder.build(pos_dev)
 
-    # launch parameters picked on this workload (cached per 2J / size bucket)
+    # launch parameters picked on this workload
     from pyxtal_ff_amd import autotune
     nl0 = nlist()
     tuned = autotune.tune_so4(nl0, cfg, w, w, use_cache=False)
 
     def step_adjoint():
+        # adjoint kernels called directly, network forward + backward fused
         nl = nlist()
         st = adjoint.SO4State(nl, cfg, w, w)
-        out = potential.energy_forces_adjoint(nl, st, model, species)
-        return out
+        return potential.energy_forces_fused(nl, st, model, species_atom)
+
+    def step_adjoint_autograd():
+        # the same kernels behind torch.autograd, torch network
+        nl = nlist()
+        st = adjoint.SO4State(nl, cfg, w, w)
+        return potential.energy_forces_adjoint(nl, st, model, species)
 
     def step_plain():
         nl = nlist()
         return potential.energy_forces_plain(nl, cfg, w, w, model, species)
 
-    def timed(fn, k, wu):
-        for _ in range(wu):
-            fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(k):
-            out = fn()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / k, out
-
-    t_adj, out_a = timed(step_adjoint, steps, warmup)
-    t_pl, out_p = timed(step_plain, max(1, steps), max(1, warmup))
-    ferr = float(torch.linalg.norm(out_a["forces"] - out_p["forces"]) / torch.linalg.norm(out_p["forces"]))
-
-    # per-kernel timing (separate instrumented steps)
-    names = ["ui", "zb", "yi", "deidrj", "dbdr", "rows", "dxdr_contract"]
+    # per-kernel means from instrumented (untimed) steps; they also name the
+    # dominant kernel of the adjoint step
     tm = Timers()
+    names = ADJ_KERNELS + PLAIN_KERNELS
     tm.wrap(dso4, names)
     for _ in range(3):
         step_adjoint()
         step_plain()
     ktimes = tm.summary()
     tm.unwrap(dso4, names)
+    dom = max((k for k in ADJ_KERNELS if k in ktimes), key=ktimes.get)
+
+    def timed(fn, k, wu, on_start=None):
+        for _ in range(wu):
+            fn()
+        torch.cuda.synchronize()
+        if on_start:
+            on_start()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k, out
+
+    # timed adjoint steps; HIP events on the launch stream bracket the
+    # dominant kernel's launches inside the timed region
+    td = Timers()
+    td.wrap(dso4, [dom])
+    t_adj, out_a = timed(step_adjoint, steps, warmup, on_start=td.acc.clear)
+    dom_ms = td.summary()[dom]
+    td.unwrap(dso4, [dom])
+    t_pl, out_p = timed(step_plain, max(1, steps), max(1, warmup))
+    t_ag, out_g = timed(step_adjoint_autograd, max(1, steps), max(1, warmup))
+    ferr = float(torch.linalg.norm(out_a["forces"] - out_p["forces"]) / torch.linalg.norm(out_p["forces"]))
+    ferr_ag = float(torch.linalg.norm(out_g["forces"] - out_a["forces"]) / torch.linalg.norm(out_a["forces"]))
+
     nl = nlist()
     tnl = []
     for _ in range(3):
@@ -176,8 +216,9 @@ def run_gpu(steps, warmup):
         torch.cuda.synchronize()
         tnl.append(time.perf_counter() - t0)
     kmean = nl.nimg / n
-    return dict(n=n, t_adj=t_adj, t_plain=t_pl, ferr=ferr, ktimes=ktimes, t_nlist=float(np.median(tnl)),
-                nimg=nl.nimg, nrows=nl.nrows, kmean=kmean, tuned=tuned)
+    return dict(n=n, t_adj=t_adj, t_plain=t_pl, t_autograd=t_ag, ferr=ferr, ferr_autograd=ferr_ag, ktimes=ktimes,
+                dom=dom, dom_ms=dom_ms,
+                t_nlist=float(np.median(tnl)), nimg=nl.nimg, nrows=nl.nrows, kmean=kmean, tuned=tuned)
 
 
 def run_batched(steps=3):
@@ -265,7 +306,7 @@ def run_so3_sio2(steps=3):
 
 
 # ------------------------------------------------------------ CPU baseline
-def run_cpu_baseline(reps_sample=2):
+def run_cpu_baseline(reps_sample=5):
     """The oracle (C restatement of the reference's numba kernels: two passes,
     U recomputed per pair, materialised dB/dr) on one host core, on the same
     4096-atom structure, plus the NumPy force contraction."""
@@ -287,6 +328,21 @@ def run_cpu_baseline(reps_sample=2):
                        "median of %d runs (%.1f s each), 1 thread" % (reps_sample, t))
 
 
+def committed_traffic(symbol):
+    """HBM bytes per launch of a kernel on this workload, from the committed
+    rocprofv3 --pmc summary (profiles/*traffic*.md lines
+    'TRAFFIC <kernel> fetch=<bytes> write=<bytes>'), or None."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"\s*TRAFFIC\s+(\S+)\s+fetch=(\d+)\s+write=(\d+)", line)
+                if m and m.group(1) == symbol:
+                    return int(m.group(2)) + int(m.group(3))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,6 +350,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched / training / SO3 workloads")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 256 MiB device copy (scale reference for rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only (every box in this build has one GPU); "
@@ -303,20 +361,20 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        copy_calibration()
     r = run_gpu(args.steps, args.warmup)
     n = r["n"]
     adj_rate = n / r["t_adj"]
     plain_rate = n / r["t_plain"]
     fl = flops(8, r["kmean"])
     kt = r["ktimes"]
-    # dominant kernel of the adjoint step
-    adj_k = {k: kt[k] for k in ("ui", "zb", "yi", "deidrj") if k in kt}
-    dom = max(adj_k, key=adj_k.get)
-    alg = {"zb": fl["z_per_atom"] * n,
+    dom = r["dom"]
+    alg = {"z": fl["z_per_atom"] * n,
            "deidrj": (fl["du_per_nbr"] + fl["adj_contract_per_nbr"]) * r["nimg"],
            "ui": fl["u_per_nbr"] * r["nimg"],
-           "yi": 8 * so4_counts(8)["nhalf"] * n}[dom]
-    achieved = alg / (kt[dom] * 1e-3)
+           "yi": 8 * so4_counts(8)["nhalf"] * n}.get(dom)
+    achieved = alg / (r["dom_ms"] * 1e-3) if alg else None
     cpu = None if args.no_cpu else run_cpu_baseline()
     extra = {}
     if not args.no_extra:
@@ -332,12 +390,20 @@ def main():
         "warmup": args.warmup, "ms_per_step": r["t_adj"] * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "diamond Si 8x8x8 (4096 atoms), sigma 0.08 A, SNAP 2J=8, rc 4.9, E+F",
-                   "natoms": n, "images": r["nimg"], "seq_rows": r["nrows"], "path": "adjoint",
+                   "natoms": n, "images": r["nimg"], "seq_rows": r["nrows"],
+                   "path": "adjoint kernels called directly, network forward+backward fused",
                    "plain_dxdr_atom_env_per_s": plain_rate, "plain_ms_per_step": r["t_plain"] * 1e3,
                    "adjoint_over_plain": adj_rate / plain_rate, "force_rel_diff_adjoint_vs_plain": r["ferr"],
+                   "adjoint_autograd_ms_per_step": r["t_autograd"] * 1e3,
+                   "force_rel_diff_autograd_vs_fused": r["ferr_autograd"],
                    "nlist_ms": r["t_nlist"] * 1e3, "kernel_ms": kt, "tuned": r["tuned"], **extra},
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None},
+        "roofline": {"bound": "valu", "kernel": KERNEL_SYMBOL[dom],
+                     "achieved": achieved / 1e12 if achieved else None, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64 if achieved else None,
+                     "traffic": committed_traffic(KERNEL_SYMBOL[dom]),
+                     "kernel_ms_timed_region": r["dom_ms"],
+                     "flops_per_launch": alg,
+                     "note": "FP64 vector ALU roof (no MFMA; MI355X FP64 vector peak = FP64 matrix peak)"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

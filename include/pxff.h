@@ -36,11 +36,14 @@ extern "C" {
 /* Launch-parameter knobs (process-wide), set by the autotuner:
  *   PXFF_TUNE_Z_CHUNKS              Z rows split into this many wavefront chunks per 64-atom tile (1..256)
  *   PXFF_TUNE_ACCUM_WAVES_PER_CU    persistent grid of the per-centre kernels = CUs x this (1..256)
- *   PXFF_TUNE_IMAGE_WAVES_PER_BLOCK wavefronts per workgroup of the per-image adjoint kernel (1, 2, 4) */
+ *   PXFF_TUNE_IMAGE_WAVES_PER_BLOCK wavefronts per workgroup of the per-image adjoint kernel (1, 2, 4)
+ *   PXFF_TUNE_ACCUM_LDSADD          per-centre U_tot sums: 0 = per-group LDS partials, 1 = one LDS array
+ *                                   with LDS atomic adds (issued by one wavefront in a fixed order) */
 #define PXFF_TUNE_Z_CHUNKS 0
 #define PXFF_TUNE_ACCUM_WAVES_PER_CU 1
 #define PXFF_TUNE_IMAGE_WAVES_PER_BLOCK 2
-#define PXFF_TUNE_COUNT 3
+#define PXFF_TUNE_ACCUM_LDSADD 3
+#define PXFF_TUNE_COUNT 4
 int pxff_tune_set(int knob, int value);
 int pxff_tune_get(int knob); /* -1 for an unknown knob */
 
@@ -197,6 +200,31 @@ int pxff_dxdr_contract(int64_t natoms, int nb, const int64_t *d_row_start, const
  * Voigt (xx, yy, zz, xy, xz, yz) of the sum of the centre's rows. */
 int pxff_rdxdr_reduce(int64_t natoms, int nb, const int64_t *d_row_start, const double *d_rdxdr, double *d_out,
                       void *stream);
+
+/* Segmented sum over contiguous rows: out [nseg*ncol], out[s][c] = sum of
+ * in[r][c] for r in [d_seg_start[s], d_seg_start[s+1]) (d_seg_start [nseg+1]
+ * int64).  Per-structure energies / virials of a batch (the reference sums
+ * atoms per structure in neuralnetwork.py:603-677); deterministic order,
+ * no atomics. */
+int pxff_segment_sum(int64_t nseg, int ncol, const int64_t *d_seg_start, const double *d_in, double *d_out,
+                     void *stream);
+
+/* Per-element network energies and dE/dx (energy + force evaluation; no
+ * parameter gradients).  Replaces the forward of the reference's per-element
+ * networks and the descriptor gradient they feed into the force contraction
+ * (neuralnetwork.py:80-200 architecture, :484-497 / :657-677 use,
+ * :938-1029 drange scaling).  Atom i (element d_species[i] in [0, nelem))
+ * uses x' = (x_i - dmin_e) * scale_e (d_dmin, d_scale [nelem*nin]) and the
+ * layers of its element; layout per element (ints, stride 1 + 5*8):
+ * nlayers (1..8), then per layer {nin, nout, act, w_off, b_off}: W
+ * [nout][nin] row-major at d_params + w_off, bias at d_params + b_off, act
+ * 0 Linear, 1 Tanh, 2 Sigmoid, 3 ReLU, 4 Softplus, 5 ELU.  Widths <= 64,
+ * last width 1, nin <= 1024.  h_layout (host copy of d_layout) is
+ * validated before launch.  Outputs: d_e [natoms] atomic energies, d_beta
+ * [natoms*nin] = dE_i/dx_i.  An out-of-range species gives NaN. */
+int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const int64_t *d_species, const int *h_layout,
+                         const int *d_layout, const double *d_params, const double *d_dmin, const double *d_scale,
+                         const double *d_x, double *d_e, double *d_beta, void *stream);
 
 /* ------------------------------------------------------------------ *
  * SO(3) power spectrum (replaces SO3.calculate / compute_cs /          *

@@ -48,6 +48,8 @@ _SIGS = {
     "pxff_so4_ui": ([P, I64, P, P, P, P, P, P, P, P], I32),
     "pxff_so4_zb": ([P, I64, P, P, P, P], I32),
     "pxff_so4_b": ([P, I64, P, P, P, P], I32),
+    "pxff_segment_sum": ([I64, ctypes.c_int, P, P, P, P], I32),
+    "pxff_mlp_energy_grad": ([I64, ctypes.c_int, ctypes.c_int, P, P, P, P, P, P, P, P, P, P], I32),
     "pxff_so4_yi": ([P, I64, P, P, P, P], I32),
     "pxff_so4_deidrj": ([P, I64, P, P, P, P, P, P, P], I32),
     "pxff_so4_gather": ([I64, P, P, P, P, P, P, P], I32),

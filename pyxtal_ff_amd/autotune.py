@@ -3,6 +3,8 @@
 Knobs (``pxff_tune_set``, include/pxff.h):
   * Z-row chunks per 64-atom tile of the Z kernel (wavefronts per tile),
   * persistent-grid size of the per-centre kernels (wavefronts per CU),
+  * how the per-centre kernels sum over images (per-group LDS partials or
+    LDS atomic adds),
   * wavefronts per workgroup of the per-image adjoint kernel.
 Each knob is timed on the caller's own workload with HIP events recorded on
 the stream the kernels run on (torch's current stream), median of a few
@@ -17,9 +19,9 @@ import torch
 from . import _native as nat
 from . import so4 as dso4
 
-Z_CHUNKS, ACCUM_WAVES_PER_CU, IMAGE_WAVES_PER_BLOCK = 0, 1, 2
+Z_CHUNKS, ACCUM_WAVES_PER_CU, IMAGE_WAVES_PER_BLOCK, ACCUM_LDSADD = 0, 1, 2, 3
 CANDIDATES = {Z_CHUNKS: (4, 8, 16, 24, 32, 48, 64), ACCUM_WAVES_PER_CU: (4, 8, 16, 32, 64),
-              IMAGE_WAVES_PER_BLOCK: (1, 2, 4)}
+              IMAGE_WAVES_PER_BLOCK: (1, 2, 4), ACCUM_LDSADD: (0, 1)}
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "pyxtal_ff_amd", "tune.json")
 
 
@@ -79,23 +81,26 @@ def tune_so4(nl, cfg, atom_w, site_w, work_order=None, use_cache=True, verbose=F
     best = {}
     # U_tot (per-centre persistent grid)
     res = {}
-    for v in CANDIDATES[ACCUM_WAVES_PER_CU]:
-        set_knob(ACCUM_WAVES_PER_CU, v)
-        res[v] = _time(lambda: dso4.ui(nl, cfg, atom_w, site_w, work_order))
-    best[ACCUM_WAVES_PER_CU] = min(res, key=res.get)
+    for mode in CANDIDATES[ACCUM_LDSADD]:
+        set_knob(ACCUM_LDSADD, mode)
+        for v in CANDIDATES[ACCUM_WAVES_PER_CU]:
+            set_knob(ACCUM_WAVES_PER_CU, v)
+            res[(mode, v)] = _time(lambda: dso4.ui(nl, cfg, atom_w, site_w, work_order))
+    best[ACCUM_LDSADD], best[ACCUM_WAVES_PER_CU] = min(res, key=res.get)
+    set_knob(ACCUM_LDSADD, best[ACCUM_LDSADD])
     set_knob(ACCUM_WAVES_PER_CU, best[ACCUM_WAVES_PER_CU])
     if verbose:
-        print("accum waves/CU", res)
+        print("accum (ldsadd, waves/CU)", res)
     u = dso4.ui(nl, cfg, atom_w, site_w, work_order)
     res = {}
     for v in CANDIDATES[Z_CHUNKS]:
         set_knob(Z_CHUNKS, v)
-        res[v] = _time(lambda: dso4.zb(nl, cfg, u))
+        res[v] = _time(lambda: dso4.z(nl, cfg, u))
     best[Z_CHUNKS] = min(res, key=res.get)
     set_knob(Z_CHUNKS, best[Z_CHUNKS])
     if verbose:
         print("Z chunks", res)
-    z, _ = dso4.zb(nl, cfg, u)
+    z = dso4.z(nl, cfg, u)
     beta = torch.ones((nl.natoms, cfg.nb), dtype=torch.float64, device=nl.pos.device)
     y = dso4.yi(nl, cfg, z, beta)
     res = {}

@@ -11,11 +11,40 @@ of the concatenated atoms, and the image kernels are flat over all images.
 import numpy as np
 import torch
 
+from . import _native as nat
 from . import adjoint
 from . import so3 as dev_so3
 from . import so4 as dev_so4
 from .neighbors import NeighborListBuilder
 from .structure import as_arrays
+
+
+class SegmentSum(torch.autograd.Function):
+    """out[s] = sum of rows [seg[s], seg[s+1]) of v (pxff_segment_sum: one
+    workgroup per segment, fixed reduction order, no atomics); backward
+    broadcasts the segment gradient back to its rows."""
+
+    @staticmethod
+    def forward(ctx, v, seg, row_seg):
+        ctx.row_seg = row_seg
+        nseg = seg.numel() - 1
+        vv = v.contiguous()
+        ncol = int(np.prod(v.shape[1:])) if v.dim() > 1 else 1
+        out = torch.empty((nseg,) + tuple(v.shape[1:]), dtype=v.dtype, device=v.device)
+        nat.check(nat.lib().pxff_segment_sum(nseg, ncol, nat.ptr(seg), nat.ptr(vv), nat.ptr(out), nat.stream()),
+                  "pxff_segment_sum")
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        return g.index_select(0, ctx.row_seg), None, None
+
+
+def segment_sum(v, seg, row_seg):
+    """Sum of the rows of v over contiguous segments (differentiable)."""
+    if v.dtype != torch.float64:
+        raise TypeError("segment_sum expects float64")
+    return SegmentSum.apply(v, seg, row_seg)
 
 
 class StructureBatch:
@@ -41,13 +70,32 @@ class StructureBatch:
         # neighbour-count-sorted work list (heaviest centres first)
         k = nl.img_start[1:] - nl.img_start[:-1]
         self.work_order = torch.argsort(k, descending=True, stable=True)
-        self.img_struct = self.atom_struct.index_select(0, nl.img_atom[: nl.nimg])
+        self.atom_seg = self.builder.atom_start
         self.inv_vol_struct = torch.as_tensor(1.0 / self.volumes, dtype=torch.float64, device=dev)
+        self._img_cache = None
+
+    def _img_segments(self):
+        """(image segment starts [nstruct+1], structure of each image) of the
+        current list; images are grouped by centre, centres by structure."""
+        nl = self.nl
+        if self._img_cache is None or self._img_cache[0] is not nl:
+            seg = nl.img_start.index_select(0, self.atom_seg)
+            st = self.atom_struct.index_select(0, nl.img_atom[: nl.nimg])
+            self._img_cache = (nl, seg, st)
+        return self._img_cache[1], self._img_cache[2]
+
+    @property
+    def img_struct(self):
+        return self._img_segments()[1]
 
     def per_structure(self, v):
-        """Segment sum of a per-atom tensor [N, ...] into [nstruct, ...]."""
-        out = torch.zeros((self.nstruct,) + tuple(v.shape[1:]), dtype=v.dtype, device=v.device)
-        return out.index_add(0, self.atom_struct, v)
+        """Sum of a per-atom tensor [N, ...] over each structure -> [nstruct, ...]."""
+        return segment_sum(v, self.atom_seg, self.atom_struct)
+
+    def per_structure_images(self, v):
+        """Sum of a per-image tensor [nimg, ...] over each structure."""
+        seg, st = self._img_segments()
+        return segment_sum(v, seg, st)
 
 
 class BatchedSNAP:
@@ -111,8 +159,7 @@ def energy_forces_virial(desc, model, species, virial=True, create_graph=False):
         out["forces"] = -adjoint._Gather.apply(dedr, nl)
         if virial:
             w = (r.detach().unsqueeze(2) * dedr.unsqueeze(1)).reshape(k, 9)
-            out["virial"] = torch.zeros((b.nstruct, 9), dtype=dedr.dtype, device=dedr.device).index_add(
-                0, b.img_struct, w).view(b.nstruct, 3, 3)
+            out["virial"] = b.per_structure_images(w).view(b.nstruct, 3, 3)
         return out
     if virial:
         g, vir = dev_so4.gather(nl, dedr, virial=True, img_r=r.detach())

@@ -10,9 +10,15 @@ HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
 FLAGS=(-O3 -std=c++17 -fPIC --offload-arch=gfx950 -Xclang -target-feature -Xclang -packed-fp32-ops -Wall -Wno-unused-result)
 OBJS=()
 PIDS=()
-for src in pxff_api.cpp so4_tables.cpp nlist.hip so4_kernels.hip so3_kernels.hip; do
+for src in pxff_api.cpp so4_tables.cpp nlist.hip so4_z.hip so4_kernels.hip so3_kernels.hip reduce.hip mlp.hip; do
   [ -f "$HERE/$src" ] || continue
   obj="$HERE/_build/${src%.*}.o"
+  # PXFF_INCREMENTAL=1: keep objects newer than their source and every header
+  if [ "${PXFF_INCREMENTAL:-0}" = 1 ] && [ -f "$obj" ] && [ "$obj" -nt "$HERE/$src" ]; then
+    stale=0
+    for h in "$HERE"/*.h "$HERE/../../include/pxff.h"; do [ "$h" -nt "$obj" ] && stale=1; done
+    if [ $stale = 0 ]; then OBJS+=("$obj"); continue; fi
+  fi
   if [[ "$src" == *.hip ]]; then
     "$HIPCC" "${FLAGS[@]}" -c "$HERE/$src" -o "$obj" &
   else

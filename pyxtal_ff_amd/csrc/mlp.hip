@@ -1,0 +1,175 @@
+// Per-element network energies and their gradients with respect to the
+// descriptors, for energy + force evaluation (no parameter gradients).
+// Mirrors the reference's per-element feed-forward networks
+// (pyxtal_ff/models/neuralnetwork.py:80-200: nn.Linear layers with one
+// activation each, descriptors scaled by the element's drange,
+// neuralnetwork.py:938-1029) evaluated forward, then backward to dE/dx.
+// One wavefront (= one workgroup) per atom: lane n holds neuron n of a
+// layer (widths <= 64), layer inputs are broadcast from LDS, weights stream
+// from L2.
+#include <hip/hip_runtime.h>
+
+#include "pxff_common.h"
+
+namespace pxff {
+
+constexpr int MLP_MAXW = 64;     // hidden / output width limit (one lane per neuron)
+constexpr int MLP_MAXL = 8;      // layers per element
+constexpr int MLP_MAXIN = 1024;  // descriptor length limit
+
+__device__ inline void act_fdf(int id, double z, double &a, double &da) {
+    switch (id) {
+    case 1: {  // Tanh
+        const double t = tanh(z);
+        a = t;
+        da = 1.0 - t * t;
+        break;
+    }
+    case 2: {  // Sigmoid
+        const double s = 1.0 / (1.0 + exp(-z));
+        a = s;
+        da = s * (1.0 - s);
+        break;
+    }
+    case 3:  // ReLU
+        a = z > 0.0 ? z : 0.0;
+        da = z > 0.0 ? 1.0 : 0.0;
+        break;
+    case 4: {  // Softplus (beta 1, threshold 20, as torch.nn.Softplus)
+        if (z > 20.0) {
+            a = z;
+            da = 1.0;
+        } else {
+            a = log1p(exp(z));
+            da = 1.0 / (1.0 + exp(-z));
+        }
+        break;
+    }
+    case 5:  // ELU (alpha 1)
+        if (z > 0.0) {
+            a = z;
+            da = 1.0;
+        } else {
+            a = expm1(z);
+            da = a + 1.0;
+        }
+        break;
+    default:  // Linear
+        a = z;
+        da = 1.0;
+    }
+}
+
+// layout per element e (ints, at d_layout[e * (1 + 5 * MLP_MAXL)]):
+//   nlayers, then per layer {nin, nout, act, w_off, b_off} (W row-major
+//   [nout][nin] as torch.nn.Linear stores it, offsets into d_params).
+__global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int nin, int nelem,
+                                                              const int64_t *__restrict__ species,
+                                                               const int *__restrict__ layout,
+                                                               const double *__restrict__ params,
+                                                               const double *__restrict__ dmin,
+                                                               const double *__restrict__ scale,
+                                                               const double *__restrict__ x, double *__restrict__ e_out,
+                                                               double *__restrict__ beta) {
+    extern __shared__ double sm[];
+    const int lane = threadIdx.x;
+    // input [nin], then per layer the activation derivative [MLP_MAXW] (the
+    // pre-activation gradient after the backward) and the output [MLP_MAXW]
+    double *h0 = sm;
+    double *dact = h0 + nin;                    // [MLP_MAXL][MLP_MAXW]
+    double *hout = dact + MLP_MAXL * MLP_MAXW;  // [MLP_MAXL][MLP_MAXW]
+    const int64_t i = blockIdx.x;
+    const int64_t el64 = species[i];
+    if (el64 < 0 || el64 >= nelem) {  // no network for this atom: NaN results
+        if (lane == 0) e_out[i] = __builtin_nan("");
+        for (int j = lane; j < nin; j += 64) beta[(size_t)i * nin + j] = __builtin_nan("");
+        return;  // whole workgroup (one wavefront)
+    }
+    const int el = (int)el64;
+    const int *lay = layout + el * (1 + 5 * MLP_MAXL);
+    const int nl = lay[0];
+    const double *xi = x + (size_t)i * nin;
+    const double *dm = dmin + (size_t)el * nin, *sc = scale + (size_t)el * nin;
+    for (int j = lane; j < nin; j += 64) h0[j] = (xi[j] - dm[j]) * sc[j];
+    __syncthreads();
+    // forward
+    const double *hin = h0;
+    for (int k = 0; k < nl; k++) {
+        const int *L = lay + 1 + 5 * k;
+        const int ni = L[0], no = L[1], act = L[2];
+        const double *W = params + L[3], *b = params + L[4];
+        if (lane < no) {
+            double z = b[lane];
+            const double *wr = W + (size_t)lane * ni;
+            for (int j = 0; j < ni; j++) z += wr[j] * hin[j];
+            double a, da;
+            act_fdf(act, z, a, da);
+            hout[k * MLP_MAXW + lane] = a;
+            dact[k * MLP_MAXW + lane] = da;
+        }
+        __syncthreads();
+        hin = hout + k * MLP_MAXW;
+    }
+    if (lane == 0) e_out[i] = hin[0];
+    // backward: dact[k] becomes the pre-activation gradient of layer k; for
+    // the output layer (one neuron, dE/dE = 1) it already is act'
+    for (int k = nl - 1; k >= 1; k--) {
+        const int *L = lay + 1 + 5 * k;
+        const int ni = L[0], no = L[1];
+        const double *W = params + L[3];
+        // dE/dh_{k-1}[j] = sum_n W[n][j] g_pre[n]; then times act'_{k-1}
+        if (lane < ni) {
+            double g = 0.0;
+            for (int n = 0; n < no; n++) g += W[(size_t)n * ni + lane] * dact[k * MLP_MAXW + n];
+            dact[(k - 1) * MLP_MAXW + lane] *= g;
+        }
+        __syncthreads();
+    }
+    {
+        const int *L = lay + 1;
+        const int ni = L[0], no = L[1];
+        const double *W = params + L[3];
+        for (int j = lane; j < ni; j += 64) {
+            double g = 0.0;
+            for (int n = 0; n < no; n++) g += W[(size_t)n * ni + j] * dact[n];
+            beta[(size_t)i * nin + j] = g * sc[j];
+        }
+    }
+}
+
+}  // namespace pxff
+
+using namespace pxff;
+
+extern "C" int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const int64_t *d_species,
+                                    const int *h_layout, const int *d_layout, const double *d_params,
+                                    const double *d_dmin, const double *d_scale, const double *d_x, double *d_e,
+                                    double *d_beta, void *stream) {
+    PXFF_REQUIRE(nin > 0 && nin <= MLP_MAXIN, "descriptor length must be in 1..1024");
+    PXFF_REQUIRE(nelem > 0 && h_layout != nullptr, "need at least one element network");
+    for (int e = 0; e < nelem; e++) {
+        const int *lay = h_layout + e * (1 + 5 * MLP_MAXL);
+        const int nl = lay[0];
+        PXFF_REQUIRE(nl >= 1 && nl <= MLP_MAXL, "1..8 layers per element network");
+        int prev = nin;
+        for (int k = 0; k < nl; k++) {
+            const int *L = lay + 1 + 5 * k;
+            PXFF_REQUIRE(L[0] == prev, "layer input width does not match the previous layer");
+            PXFF_REQUIRE(L[1] >= 1 && L[1] <= MLP_MAXW, "layer widths must be in 1..64");
+            PXFF_REQUIRE(L[2] >= 0 && L[2] <= 5, "unknown activation id");
+            prev = L[1];
+        }
+        PXFF_REQUIRE(prev == 1, "the last layer must have one output (the atomic energy)");
+    }
+    if (natoms <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t lds = sizeof(double) * ((size_t)nin + 2 * MLP_MAXL * MLP_MAXW);
+    if (lds > 65536)
+        PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)mlp_energy_grad_kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    PXFF_REQUIRE(natoms <= 0x7fffffff, "too many atoms for one launch");
+    mlp_energy_grad_kernel<<<(unsigned)natoms, 64, lds, s>>>(natoms, nin, nelem, d_species, d_layout, d_params, d_dmin,
+                                                               d_scale, d_x, d_e, d_beta);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}

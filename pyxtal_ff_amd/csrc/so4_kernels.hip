@@ -28,25 +28,10 @@
 #include <vector>
 
 #include "pxff_common.h"
+#include "so4_dev.h"
 #include "so4_tables.h"
 
 namespace pxff {
-
-// ------------------------------------------------------------ device tables
-struct So4Dev {
-    int twol, nu, nhalf, nz, nb, ncg, nblk;
-    double *cg;
-    int *zsched;
-    int *blk_level_start, *blk_zoff, *blk_tb;
-    double *blk_tf;
-    int *b_zoff, *b_level;
-    double *rootpq, *lnorm;
-    int *hoff, *h_level, *h_diag;
-    double *h_weight;
-    int *idxu_block;
-    int nzrows, nzchunks;
-    int *zrows, *zchunk;
-};
 
 namespace {
 std::mutex g_tab_mutex;
@@ -86,7 +71,6 @@ int get_so4_tables(int twol, int norm, const So4Dev **out) {
     d.nblk = t.nblk;
     int rc = 0;
     if ((rc = upload(t.cglist, &d.cg))) return rc;
-    if ((rc = upload(t.zsched, &d.zsched))) return rc;
     if ((rc = upload(t.blk_level_start, &d.blk_level_start))) return rc;
     if ((rc = upload(t.blk_zoff, &d.blk_zoff))) return rc;
     if ((rc = upload(t.blk_tb, &d.blk_tb))) return rc;
@@ -100,66 +84,46 @@ int get_so4_tables(int twol, int norm, const So4Dev **out) {
     if ((rc = upload(t.h_diag, &d.h_diag))) return rc;
     if ((rc = upload(t.h_weight, &d.h_weight))) return rc;
     if ((rc = upload(t.idxu_block, &d.idxu_block))) return rc;
-    d.nzrows = t.nzrows;
-    d.nzchunks = 16;
-    if ((rc = upload(t.zrows, &d.zrows))) return rc;
-    if ((rc = upload(chunk_zrows(t, d.nzchunks), &d.zchunk))) return rc;
+    d.nzgroups = t.nzgroups;
+    if ((rc = upload(t.zgroups, &d.zgroups))) return rc;
+    // inverse of the block targets: the (block, factor) pairs of each b
+    std::vector<int> bsrc(3 * (size_t)t.nb, -1);
+    std::vector<double> bsrcf(3 * (size_t)t.nb, 0.0);
+    for (int k = 0; k < t.nblk; k++)
+        for (int q = 0; q < 3; q++) {
+            const int b = t.blk_tb[3 * k + q];
+            if (b < 0) continue;
+            int slot = 0;
+            while (slot < 3 && bsrc[3 * b + slot] >= 0) slot++;
+            if (slot == 3) {
+                set_error("more than three blocks target one bispectrum component");
+                return PXFF_EINVAL;
+            }
+            bsrc[3 * b + slot] = k;
+            bsrcf[3 * b + slot] = t.blk_tf[3 * k + q];
+        }
+    if ((rc = upload(bsrc, &d.b_src))) return rc;
+    if ((rc = upload(bsrcf, &d.b_srcf))) return rc;
+    // half-U element of each Z element: Z block (l1, l2, l) element loc pairs
+    // with U^l element loc (same (mb, ma), mb <= l/2)
+    std::vector<int> zh((size_t)t.nz, 0), zblk((size_t)t.nz, 0);
+    for (int k = 0; k < t.nblk; k++) {
+        int l = 0;
+        while (l + 1 < t.ldim && k >= t.blk_level_start[l + 1]) l++;
+        const int nloc = (l / 2 + 1) * (l + 1);
+        for (int q = 0; q < nloc; q++) {
+            zh[t.blk_zoff[k] + q] = t.hoff[l] + q;
+            zblk[t.blk_zoff[k] + q] = k;
+        }
+    }
+    if ((rc = upload(zh, &d.zh))) return rc;
+    if ((rc = upload(zblk, &d.zblk))) return rc;
+    std::vector<double> hscale((size_t)t.nhalf, 0.0);
+    for (int h = 0; h < t.nhalf; h++) hscale[h] = 2.0 * t.h_weight[h] * t.lnorm[t.h_level[h]];
+    if ((rc = upload(hscale, &d.hscale))) return rc;
     g_tabs[key] = d;
     *out = &g_tabs[key];
     return 0;
-}
-
-// ------------------------------------------------------------ tuning knobs
-// Process-wide launch parameters set by the autotuner (pxff_tune_set).
-static int g_tune[PXFF_TUNE_COUNT] = {16, 16, 4};
-static const int g_tune_lo[PXFF_TUNE_COUNT] = {1, 1, 1};
-static const int g_tune_hi[PXFF_TUNE_COUNT] = {256, 256, 4};
-
-// Z-row chunk tables per (device, 2J, normalisation, nchunks)
-static std::mutex g_chunk_mutex;
-static std::map<std::tuple<int, int, int>, int *> g_chunks;
-static int get_zchunks(int twol, int norm, int nch, const int **out) {
-    int dev = 0;
-    PXFF_HIP_CHECK(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(g_chunk_mutex);
-    auto key = std::make_tuple(dev, twol * 2 + (norm ? 1 : 0), nch);
-    auto it = g_chunks.find(key);
-    if (it != g_chunks.end()) {
-        *out = it->second;
-        return 0;
-    }
-    So4HostTables t;
-    if (!build_so4_tables(twol, norm, t)) {
-        set_error("unsupported 2J");
-        return PXFF_ENOTIMPL;
-    }
-    std::vector<int> c = chunk_zrows(t, nch);
-    int *d = nullptr;
-    PXFF_HIP_CHECK(hipMalloc((void **)&d, c.size() * sizeof(int)));
-    PXFF_HIP_CHECK(hipMemcpy(d, c.data(), c.size() * sizeof(int), hipMemcpyHostToDevice));
-    g_chunks[key] = d;
-    *out = d;
-    return 0;
-}
-
-// ------------------------------------------------------- compile-time sizes
-__host__ __device__ constexpr int nhalf_of(int twoj) {
-    int s = 0;
-    for (int l = 0; l <= twoj; l++) s += (l / 2 + 1) * (l + 1);
-    return s;
-}
-__host__ __device__ constexpr int nb_of(int twoj) {
-    int s = 0;
-    for (int l1 = 0; l1 <= twoj; l1++)
-        for (int l2 = 0; l2 <= l1; l2++)
-            for (int l = l1 - l2; l <= (twoj < l1 + l2 ? twoj : l1 + l2); l += 2)
-                if (l >= l1) s++;
-    return s;
-}
-__host__ __device__ constexpr int hoff_of(int l) {
-    int s = 0;
-    for (int k = 0; k < l; k++) s += (k / 2 + 1) * (k + 1);
-    return s;
 }
 
 // Cayley-Klein parameters of one image (compute_uarray_recursive,
@@ -294,7 +258,7 @@ __device__ inline void sweep_rows(int mb, const double *__restrict__ rpq, const 
 // image per round; per-group partial sums live in LDS and are reduced in a
 // fixed order at the end (deterministic).  With TANGENT the kernel instead
 // accumulates dU_tot . dr_e (forward-mode derivative of U_tot).
-template <int TWOJ, bool TANGENT, bool TILED>
+template <int TWOJ, bool TANGENT, bool TILED, bool LDSADD>
 __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
                                                         const int64_t *__restrict__ img_j,
                                                         const double *__restrict__ img_r,
@@ -307,7 +271,11 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
     constexpr int G = TWOJ / 2 + 1;
     constexpr int NG = 64 / G;
     constexpr int NH = nhalf_of(TWOJ);
-    __shared__ cd part[NG][NH];
+    // LDSADD: all groups add into one LDS array (ds_add_f64; one wavefront
+    // per centre issues every add, so the order is fixed); otherwise each
+    // group keeps its own partial array, summed in group order at the end.
+    constexpr int NP = LDSADD ? 1 : NG;
+    __shared__ cd part[NP][NH];
     __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
     __shared__ double lnorm[TWOJ + 1];
     const int lane = threadIdx.x;
@@ -318,7 +286,7 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
     for (int64_t c = blockIdx.x; c < natoms; c += gridDim.x) {
         const int64_t i = work_order ? work_order[c] : c;
         __syncthreads();
-        for (int k = lane; k < NG * NH; k += 64) (&part[0][0])[k] = cmake(0.0, 0.0);
+        for (int k = lane; k < NP * NH; k += 64) (&part[0][0])[k] = cmake(0.0, 0.0);
         __syncthreads();
         const int64_t s = img_start[i], e = img_start[i + 1];
         for (int64_t base = s; base < e; base += NG) {
@@ -345,15 +313,20 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
             cayley_klein(x, y, z, r, rfac0 * M_PI * r / rcut, TANGENT, ck);
             if (!TANGENT) {
                 const double sc = fc * w;
-                cd *prow = part[g < NG ? g : 0];
+                cd *prow = part[(!LDSADD && g < NG) ? g : 0];
                 sweep_rows<TWOJ, false>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
                     if (!valid || !half) return;
                     const int h0 = hoff_of(l) + mb * (l + 1);
 #pragma unroll
                     for (int m = 0; m <= TWOJ; m++) {
                         if (m > l) continue;
-                        cd v = prow[h0 + m];
-                        prow[h0 + m] = cmake(v.re + sc * u[m].re, v.im + sc * u[m].im);
+                        if (LDSADD) {
+                            atomicAdd(&prow[h0 + m].re, sc * u[m].re);
+                            atomicAdd(&prow[h0 + m].im, sc * u[m].im);
+                        } else {
+                            cd v = prow[h0 + m];
+                            prow[h0 + m] = cmake(v.re + sc * u[m].re, v.im + sc * u[m].im);
+                        }
                     }
                 });
             } else {
@@ -365,7 +338,7 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
                 }
                 // D . dr = fc (dU . dr) + dfc (rhat . dr) U
                 const double gu = dfc * (x * d0 + y * d1 + z * d2) / r;
-                cd *prow = part[g < NG ? g : 0];
+                cd *prow = part[(!LDSADD && g < NG) ? g : 0];
                 sweep_rows<TWOJ, true>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
                     if (!valid || !half) return;
                     const int h0 = hoff_of(l) + mb * (l + 1);
@@ -374,8 +347,13 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
                         if (m > l) continue;
                         double re = fc * (du[0][m].re * d0 + du[1][m].re * d1 + du[2][m].re * d2) + gu * u[m].re;
                         double im = fc * (du[0][m].im * d0 + du[1][m].im * d1 + du[2][m].im * d2) + gu * u[m].im;
-                        cd v = prow[h0 + m];
-                        prow[h0 + m] = cmake(v.re + w * re, v.im + w * im);
+                        if (LDSADD) {
+                            atomicAdd(&prow[h0 + m].re, w * re);
+                            atomicAdd(&prow[h0 + m].im, w * im);
+                        } else {
+                            cd v = prow[h0 + m];
+                            prow[h0 + m] = cmake(v.re + w * re, v.im + w * im);
+                        }
                     }
                 });
             }
@@ -383,7 +361,7 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
         __syncthreads();
         for (int h = lane; h < NH; h += 64) {
             cd acc = cmake(0.0, 0.0);
-            for (int q = 0; q < NG; q++) acc = cadd(acc, part[q][h]);
+            for (int q = 0; q < NP; q++) acc = cadd(acc, part[q][h]);
             // level / row / column of h
             int l = 0;
             while (l < TWOJ && h >= hoff_of(l + 1)) l++;
@@ -399,304 +377,96 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
     }
 }
 
-// ------------------------------------------------------------------- Z, B
-// One workgroup (256 threads) per atom, persistent over atoms.  The CG table
-// and the full U_tot (rebuilt from the half rows by inversion symmetry) live
-// in LDS; one thread per Z element runs compute_zi's double loop
-// (SNAP.py:1050-1098), elements scheduled in decreasing-cost order.  B then
-// follows compute_bi (SNAP.py:1101-1141).
-template <int TWOJ>
-__global__ __launch_bounds__(256) void so4_zb_kernel(int64_t natoms, const double *__restrict__ utot,
-                                                       const double *__restrict__ cg_g, int ncg,
-                                                       const int *__restrict__ zsched, int nz,
-                                                       const int *__restrict__ b_zoff, const int *__restrict__ b_level,
-                                                       int nb, double *__restrict__ zout, double *__restrict__ bout) {
-    constexpr int NH = nhalf_of(TWOJ);
-    constexpr int NU = (TWOJ + 1) * (TWOJ + 2) * (2 * TWOJ + 3) / 6;
-    extern __shared__ double smem[];
-    double *cg = smem;
-    cd *U = reinterpret_cast<cd *>(smem + ((ncg + 1) & ~1));
-    const int tid = threadIdx.x;
-    for (int k = tid; k < ncg; k += 256) cg[k] = cg_g[k];
-    for (int64_t i = blockIdx.x; i < natoms; i += gridDim.x) {
-        __syncthreads();
-        const cd *uh = reinterpret_cast<const cd *>(utot) + (size_t)i * NH;
-        for (int k = tid; k < NU; k += 256) {
-            int l = 0, base = 0;
-            while (l < TWOJ && k >= base + (l + 1) * (l + 1)) {
-                base += (l + 1) * (l + 1);
-                l++;
-            }
-            int loc = k - base;
-            int mb = loc / (l + 1), ma = loc % (l + 1);
-            cd v;
-            if (2 * mb <= l) {
-                v = uh[hoff_of(l) + mb * (l + 1) + ma];
-            } else {
-                cd o = uh[hoff_of(l) + (l - mb) * (l + 1) + (l - ma)];
-                double s = ((ma + mb) & 1) ? -1.0 : 1.0;
-                v = cmake(s * o.re, -s * o.im);
-            }
-            U[k] = v;
-        }
-        __syncthreads();
-        cd *zo = reinterpret_cast<cd *>(zout) + (size_t)i * nz;
-        for (int k = tid; k < nz; k += 256) {
-            const int *zs = zsched + 6 * k;
-            const int jjz = zs[0];
-            int u1 = zs[1], u2 = zs[2];
-            const int cga0 = zs[3];
-            int cgb = zs[4];
-            const int pk = zs[5];
-            const int na = pk & 15, nbb = (pk >> 4) & 15, l1p = (pk >> 8) & 15, l2p = (pk >> 12) & 15;
-            const int l2 = pk >> 16;
-            cd zsum = cmake(0.0, 0.0);
-            for (int ib = 0; ib < nbb; ib++) {
-                cd sa = cmake(0.0, 0.0);
-                int cga = cga0;
-                for (int ia = 0; ia < na; ia++) {
-                    cd p = cmul(U[u1 + ia], U[u2 - ia]);
-                    double c = cg[cga];
-                    sa.re += c * p.re;
-                    sa.im += c * p.im;
-                    cga += l2;
-                }
-                double cb = cg[cgb];
-                zsum.re += cb * sa.re;
-                zsum.im += cb * sa.im;
-                u1 += l1p;
-                u2 -= l2p;
-                cgb += l2;
-            }
-            zo[jjz] = zsum;
-        }
-        if (bout) {
-            __syncthreads();
-            for (int b = tid; b < nb; b += 256) {
-                const int l = b_level[b];
-                const cd *zb = zo + b_zoff[b];
-                const int ub = l * (l + 1) * (2 * l + 1) / 6;  // idxu_block[l]
-                double s = 0.0;
-                int llz = 0;
-                for (int mb = 0; 2 * mb < l; mb++)
-                    for (int ma = 0; ma <= l; ma++, llz++) s += cdotre(U[ub + llz], zb[llz]);
-                if (l % 2 == 0) {
-                    const int mb = l / 2;
-                    for (int ma = 0; ma < mb; ma++, llz++) s += cdotre(U[ub + llz], zb[llz]);
-                    s += 0.5 * cdotre(U[ub + llz], zb[llz]);
-                }
-                bout[(size_t)i * nb + b] = 2.0 * s;
-            }
-        }
-    }
-}
-
-// ------------------------------------------------- Z, atom per lane
-// compute_zi (SNAP.py:1050-1098) with one atom per lane: a wavefront owns a
-// tile of 64 atoms and a chunk of Z rows (block (l1, l2, l), row mb), so the
-// loop structure and the Clebsch-Gordan coefficients are the same for all
-// lanes (scalar loads, uniform branches).  For a row, every (mb1, mb2) pair
-// of the ib loop brings U^{l1} row mb1 and U^{l2} row mb2 into registers
-// (coalesced loads from the tile-major U_tot, rows beyond l/2 by inversion
-// symmetry), and all (ma1, ma2) products are accumulated by ma1 + ma2 =
-// ma + (l1 + l2 - l)/2 into registers with static indices:
-//   Z[ma][mb] = sum_{mb1+mb2=mb+s} C[mb1][mb2] sum_{ma1+ma2=ma+s} C[ma1][ma2] U1[mb1][ma1] U2[mb2][ma2].
-// Workgroup ids are mapped so that the chunks of one tile run on the same XCD
-// (workgroups are dispatched to the 8 XCDs round-robin), sharing its L2.
-// Compile-time loop: f(std::integral_constant<int, 0>) ... f(<N-1>).
-template <typename F, int... I>
-__device__ inline void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
-    (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, typename F>
-__device__ inline void static_for(F &&f) {
-    static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
-template <int TWOJ>
-__device__ inline void load_urow(const cd *__restrict__ ut, int lv, int mb, int hoffl, int lane, cd *u) {
-    const bool flip = 2 * mb > lv;
-    const int mrow = flip ? lv - mb : mb;
-    const int base = hoffl + mrow * (lv + 1);
-#pragma unroll
-    for (int ma = 0; ma <= TWOJ; ma++) {
-        if (ma <= lv) {
-            const int col = flip ? lv - ma : ma;
-            cd v = ut[(base + col) * 64 + lane];
-            if (flip) {
-                const double sg = ((ma + mb) & 1) ? -1.0 : 1.0;
-                v = cmake(sg * v.re, -sg * v.im);
-            }
-            u[ma] = v;
-        } else {
-            u[ma] = cmake(0.0, 0.0);
-        }
-    }
-}
-
-// One Z row (block (L1, L2, l), row mb) for the lane's atom, with the (l1,
-// l2) of the block known at compile time: the loads of U rows are exact,
-// and the pairs (a, c) of one t = a + c are branch-free (only the t range
-// [sh, sh+l], which depends on l, is tested -- one uniform branch per t).
-template <int TWOJ, int L1, int L2>
-__device__ __forceinline__ void zrow(const cd *__restrict__ ut, int lane, const double *__restrict__ cg, int cgblk, int l,
-                                  int sh, int mb, int mb1lo, int nbb, int h1, int h2, cd *__restrict__ zo, bool valid,
-                                  int zrow0) {
-    constexpr int NT = L1 + L2 + 1;
-    constexpr int L2P = L2 + 1;
-    cd acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = cmake(0.0, 0.0);
-    const double *cgr = cg + cgblk;
-    for (int ib = 0; ib < nbb; ib++) {
-        const int mb1 = mb1lo + ib, mb2 = mb + sh - mb1;
-        cd u1[L1 + 1], u2[L2 + 1];
-        load_urow<L1>(ut, L1, mb1, h1, lane, u1);
-        load_urow<L2>(ut, L2, mb2, h2, lane, u2);
-        const double cgb = cgr[mb1 * L2P + mb2];
-        // keep the CG reads of the pair loop inside the row loop: hoisted,
-        // (L1+1)(L2+1) doubles would not fit in registers beside acc and U
-        asm volatile("" ::: "memory");
-        static_for<NT>([&](auto T) {
-            constexpr int t = decltype(T)::value;
-            if (t >= sh && t <= sh + l) {
-                static_for<L1 + 1>([&](auto A) {
-                    constexpr int a = decltype(A)::value;
-                    constexpr int c = t - a;
-                    if constexpr (c >= 0 && c <= L2) {
-                        const double w = cgb * cgr[a * L2P + c];
-                        const cd pr = cmul(u1[a], u2[c]);
-                        acc[t].re += w * pr.re;
-                        acc[t].im += w * pr.im;
-                    }
-                });
-            }
-        });
-    }
-    if (valid) {
-#pragma unroll
-        for (int t = 0; t < NT; t++)
-            if (t >= sh && t <= sh + l) zo[zrow0 + t - sh] = acc[t];
-    }
-}
-
-// Uniform dispatch to zrow<TWOJ, l1, l2>.
-template <int TWOJ, int L1, int L2>
-__device__ inline void zrow_dispatch(int l1, int l2, const cd *ut, int lane, const double *cg, int cgblk, int l, int sh,
-                                     int mb, int mb1lo, int nbb, int h1, int h2, cd *zo, bool valid, int zrow0) {
-    if constexpr (L1 <= TWOJ) {
-        if (l1 == L1 && l2 == L2) {
-            zrow<TWOJ, L1, L2>(ut, lane, cg, cgblk, l, sh, mb, mb1lo, nbb, h1, h2, zo, valid, zrow0);
-            return;
-        }
-        if constexpr (L2 < L1)
-            zrow_dispatch<TWOJ, L1, L2 + 1>(l1, l2, ut, lane, cg, cgblk, l, sh, mb, mb1lo, nbb, h1, h2, zo, valid, zrow0);
-        else
-            zrow_dispatch<TWOJ, L1 + 1, 0>(l1, l2, ut, lane, cg, cgblk, l, sh, mb, mb1lo, nbb, h1, h2, zo, valid, zrow0);
-    }
-}
-
-template <int TWOJ>
-__global__ __launch_bounds__(256) void so4_z_tile_kernel(int64_t natoms, int ntiles, int nch, const int *__restrict__ zchunk,
-                                                          const int *__restrict__ zrows, const double *__restrict__ cg_g,
-                                                          int ncg, const double *__restrict__ utile,
-                                                          double *__restrict__ zout, int nz) {
-    constexpr int NH = nhalf_of(TWOJ);
-    extern __shared__ double cg[];  // the whole CG table; uniform-address reads broadcast
-    for (int k = threadIdx.x; k < ncg; k += blockDim.x) cg[k] = cg_g[k];
-    __syncthreads();
-    // workgroup = 4 wavefronts on 4 consecutive chunks of one tile
-    const int b = blockIdx.x;
-    const int nq = (nch + 3) / 4;
-    const int xcd = b & 7, k = b >> 3;
-    const int tile = xcd + 8 * (k / nq);
-    const int ch = 4 * (k % nq) + (threadIdx.x >> 6);
-    if (tile >= ntiles || ch >= nch) return;
-    const int lane = threadIdx.x & 63;
-    const int64_t atom = (int64_t)tile * 64 + lane;
-    const bool valid = atom < natoms;
-    const cd *ut = reinterpret_cast<const cd *>(utile) + (size_t)tile * NH * 64;
-    cd *zo = reinterpret_cast<cd *>(zout) + (size_t)atom * nz;
-    const int r0 = __builtin_amdgcn_readfirstlane(zchunk[ch]);
-    const int r1 = __builtin_amdgcn_readfirstlane(zchunk[ch + 1]);
-    for (int r = r0; r < r1; r++) {
-        const int *row = zrows + 12 * r;
-        const int zrow0 = __builtin_amdgcn_readfirstlane(row[0]);
-        const int l1 = __builtin_amdgcn_readfirstlane(row[1]);
-        const int l2 = __builtin_amdgcn_readfirstlane(row[2]);
-        const int l = __builtin_amdgcn_readfirstlane(row[3]);
-        const int sh = __builtin_amdgcn_readfirstlane(row[4]);
-        const int mb = __builtin_amdgcn_readfirstlane(row[5]);
-        const int mb1lo = __builtin_amdgcn_readfirstlane(row[6]);
-        const int nbb = __builtin_amdgcn_readfirstlane(row[7]);
-        const int cgblk = __builtin_amdgcn_readfirstlane(row[8]);
-        const int h1 = __builtin_amdgcn_readfirstlane(row[9]);
-        const int h2 = __builtin_amdgcn_readfirstlane(row[10]);
-        zrow_dispatch<TWOJ, 0, 0>(l1, l2, ut, lane, cg, cgblk, l, sh, mb, mb1lo, nbb, h1, h2, zo, valid, zrow0);
-    }
-}
-
 // B (compute_bi, SNAP.py:1101-1141) from the tile-major U_tot and Z: one
-// thread per (atom, b).
+// atom per lane (64-atom tiles, so the U_tot loads are coalesced), the
+// components b spread over the wavefronts of a tile (grid.y); loop bounds
+// are wave-uniform.
 template <int TWOJ>
-__global__ void so4_b_kernel(int64_t natoms, const double *__restrict__ utile, const double *__restrict__ z, int nz,
-                             int nb, const int *__restrict__ b_zoff, const int *__restrict__ b_level,
-                             double *__restrict__ bout) {
+__global__ __launch_bounds__(256) void so4_b_kernel(int64_t natoms, const double *__restrict__ utile,
+                                                     const double *__restrict__ z, int nz, int nb,
+                                                     const int *__restrict__ b_zoff, const int *__restrict__ b_level,
+                                                     double *__restrict__ bout) {
     constexpr int NH = nhalf_of(TWOJ);
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= natoms * nb) return;
-    const int64_t i = t / nb;
-    const int b = (int)(t - i * nb);
-    const cd *ut = reinterpret_cast<const cd *>(utile) + (size_t)(i >> 6) * NH * 64 + (i & 63);
-    const cd *zb = reinterpret_cast<const cd *>(z) + (size_t)i * nz + b_zoff[b];
-    const int l = b_level[b];
-    const int h0 = hoff_of(l);
-    double s = 0.0;
-    int llz = 0;
-    for (int mb = 0; 2 * mb < l; mb++)
-        for (int ma = 0; ma <= l; ma++, llz++) s += cdotre(ut[(h0 + llz) * 64], zb[llz]);
-    if (l % 2 == 0) {
-        const int mb = l / 2;
-        for (int ma = 0; ma < mb; ma++, llz++) s += cdotre(ut[(h0 + llz) * 64], zb[llz]);
-        s += 0.5 * cdotre(ut[(h0 + llz) * 64], zb[llz]);
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.y * 4 + (threadIdx.x >> 6), nw = gridDim.y * 4;
+    const int64_t tile = blockIdx.x;
+    const int64_t i = tile * 64 + lane;
+    if (i >= natoms) return;
+    const cd *ut = reinterpret_cast<const cd *>(utile) + (size_t)tile * NH * 64 + lane;
+    const cd *zi = reinterpret_cast<const cd *>(z) + (size_t)i * nz;
+    for (int b = w; b < nb; b += nw) {
+        const int l = __builtin_amdgcn_readfirstlane(b_level[b]);
+        const int z0 = __builtin_amdgcn_readfirstlane(b_zoff[b]);
+        const int h0 = hoff_of(l);
+        // rows mb < l/2 in full, then (l even) the middle row up to the
+        // diagonal, whose element counts half
+        const int nfull = ((l + 1) / 2) * (l + 1);
+        double s = 0.0;
+#pragma unroll 8
+        for (int q = 0; q < nfull; q++) s += cdotre(ut[(size_t)(h0 + q) * 64], zi[z0 + q]);
+        if ((l & 1) == 0) {
+            const int mb = l / 2;
+#pragma unroll 4
+            for (int q = nfull; q < nfull + mb; q++) s += cdotre(ut[(size_t)(h0 + q) * 64], zi[z0 + q]);
+            s += 0.5 * cdotre(ut[(size_t)(h0 + nfull + mb) * 64], zi[z0 + nfull + mb]);
+        }
+        bout[(size_t)i * nb + b] = 2.0 * s;
     }
-    bout[t] = 2.0 * s;
 }
 
 // ---------------------------------------------------------------------- Y
-// Y[h] = 2 hw(h) norm_l sum_{blocks at level l} c_blk Z[blk][loc(h)],
-// c_blk = sum_targets f beta[b].  One thread per (atom, half element).
-__global__ void so4_yi_kernel(int64_t natoms, int nhalf, int nz, int nb, const double *__restrict__ z,
-                              const double *__restrict__ beta, const int *__restrict__ hoff,
-                              const int *__restrict__ h_level, const double *__restrict__ h_weight,
-                              const double *__restrict__ lnorm, const int *__restrict__ blk_level_start,
-                              const int *__restrict__ blk_zoff, const int *__restrict__ blk_tb,
-                              const double *__restrict__ blk_tf, double *__restrict__ y) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= natoms * nhalf) return;
-    const int64_t i = t / nhalf;
-    const int h = (int)(t - i * nhalf);
-    const int l = h_level[h];
-    const int loc = h - hoff[l];
-    const cd *zi = reinterpret_cast<const cd *>(z) + (size_t)i * nz;
-    const double *bi = beta + (size_t)i * nb;
-    cd acc = cmake(0.0, 0.0);
-    const double hw = h_weight[h];
-    if (hw != 0.0) {
-        for (int k = blk_level_start[l]; k < blk_level_start[l + 1]; k++) {
-            double c = 0.0;
+// Y[h] = 2 hw(h) norm_l sum_{blocks k at level l} c_k Z[k][loc(h)],
+// c_k = sum_targets f beta[b].  One wavefront per atom (4 per workgroup):
+// the block coefficients c_k go to LDS, then the atom's Z row is streamed
+// once in order (coalesced 1 KB loads) and each element is added into its
+// half element's LDS accumulator (ds_add_f64; one wavefront issues all adds
+// of an atom, so their order is fixed), and the accumulators are scaled and
+// stored.
+__global__ __launch_bounds__(256) void so4_yi_kernel(int64_t natoms, int nhalf, int nz, int nb, int nblk,
+                                                      const double *__restrict__ z, const double *__restrict__ beta,
+                                                      const int *__restrict__ zblk, const int *__restrict__ zh,
+                                                      const double *__restrict__ hscale,
+                                                      const int *__restrict__ blk_tb,
+                                                      const double *__restrict__ blk_tf, double *__restrict__ y) {
+    extern __shared__ double sm[];  // per wavefront: c [nblk], Y accumulators [2 nhalf]
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + wv;
+    const bool valid = i < natoms;
+    double *c = sm + (size_t)wv * (nblk + 2 * nhalf);
+    double *ya = c + nblk;
+    if (valid) {
+        const double *bi = beta + (size_t)i * nb;
+        for (int k = lane; k < nblk; k += 64) {
+            double v = 0.0;
 #pragma unroll
-            for (int s = 0; s < 3; s++) {
-                int b = blk_tb[3 * k + s];
-                if (b >= 0) c += blk_tf[3 * k + s] * bi[b];
+            for (int t = 0; t < 3; t++) {
+                const int b = blk_tb[3 * k + t];
+                if (b >= 0) v += blk_tf[3 * k + t] * bi[b];
             }
-            cd v = zi[blk_zoff[k] + loc];
-            acc.re += c * v.re;
-            acc.im += c * v.im;
+            c[k] = v;
+        }
+        for (int k = lane; k < 2 * nhalf; k += 64) ya[k] = 0.0;
+    }
+    __syncthreads();
+    if (valid) {
+        const cd *zi = reinterpret_cast<const cd *>(z) + (size_t)i * nz;
+#pragma unroll 4
+        for (int e = lane; e < nz; e += 64) {
+            const cd v = zi[e];
+            const double ck = c[zblk[e]];
+            const int h = zh[e];
+            atomicAdd(&ya[2 * h], ck * v.re);
+            atomicAdd(&ya[2 * h + 1], ck * v.im);
         }
     }
-    const double f = 2.0 * hw * lnorm[l];
-    reinterpret_cast<cd *>(y)[(size_t)i * nhalf + h] = cmake(f * acc.re, f * acc.im);
+    __syncthreads();
+    if (valid) {
+        cd *yo = reinterpret_cast<cd *>(y) + (size_t)i * nhalf;
+        for (int h = lane; h < nhalf; h += 64) {
+            const double f = hscale[h];
+            yo[h] = cmake(f * ya[2 * h], f * ya[2 * h + 1]);
+        }
+    }
 }
 
 // -------------------------------------------------------- adjoint dE/dr
@@ -967,31 +737,50 @@ __global__ __launch_bounds__(64) void dxdr_contract_kernel(int64_t natoms, int n
 }
 
 // Adjoint path: dE/dR_i = sum_{e of i} (dE/dr_rev(e) - dE/dr_e); virial of
-// centre i = sum_e r_e (x) dE/dr_e.  One thread per atom.
-__global__ void so4_gather_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
-                                  const int64_t *__restrict__ img_rev, const double *__restrict__ img_r,
-                                  const double *__restrict__ dedr, double *__restrict__ grad,
-                                  double *__restrict__ virial) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natoms) return;
+// centre i = sum_e r_e (x) dE/dr_e.  One wavefront per atom (4 per
+// workgroup), lanes over the centre's images, then a butterfly reduction
+// across lanes (fixed order: deterministic, no atomics).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void so4_gather_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
+                                                          const int64_t *__restrict__ img_rev,
+                                                          const double *__restrict__ img_r,
+                                                          const double *__restrict__ dedr, double *__restrict__ grad,
+                                                          double *__restrict__ virial) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= natoms) return;  // whole wavefront
     double g[3] = {0, 0, 0};
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t e = img_start[i]; e < img_start[i + 1]; e++) {
+    for (int64_t e = img_start[i] + lane; e < img_start[i + 1]; e += 64) {
         const int64_t q = img_rev[e];
+        const double d[3] = {dedr[3 * e], dedr[3 * e + 1], dedr[3 * e + 2]};
 #pragma unroll
-        for (int k = 0; k < 3; k++) g[k] += dedr[3 * q + k] - dedr[3 * e + k];
+        for (int k = 0; k < 3; k++) g[k] += dedr[3 * q + k] - d[k];
         if (virial) {
 #pragma unroll
-            for (int a = 0; a < 3; a++)
+            for (int a = 0; a < 3; a++) {
+                const double ra = img_r[3 * e + a];
 #pragma unroll
-                for (int c = 0; c < 3; c++) v[3 * a + c] += img_r[3 * e + a] * dedr[3 * e + c];
+                for (int c = 0; c < 3; c++) v[3 * a + c] += ra * d[c];
+            }
         }
     }
 #pragma unroll
-    for (int k = 0; k < 3; k++) grad[3 * i + k] = g[k];
-    if (virial)
+    for (int k = 0; k < 3; k++) g[k] = wave_sum(g[k]);
+    if (virial) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[k] = wave_sum(v[k]);
+    }
+    if (lane < 3) grad[3 * i + lane] = lane == 0 ? g[0] : (lane == 1 ? g[1] : g[2]);
+    if (virial && lane == 0) {
 #pragma unroll
         for (int k = 0; k < 9; k++) virial[9 * i + k] = v[k];
+    }
 }
 
 // Per-atom rdxdr reduction (utilities/__init__.py:226-235).
@@ -1009,41 +798,56 @@ __global__ void rdxdr_reduce_kernel(int64_t natoms, int nb, const int64_t *__res
 }
 
 // ----------------------------------------------------------- tangent of B
-// dB_b = Re sum_targets f <dU^level, 2 hw Z^blk> with dU = dU_tot (u_norm
-// already applied by the accumulation kernel).  One thread per (atom, b).
-__global__ void so4_tangent_b_kernel(int64_t natoms, int nhalf, int nz, int nb, const double *__restrict__ dutot,
-                                     const double *__restrict__ z, const int *__restrict__ hoff,
-                                     const int *__restrict__ blk_level_start, const int *__restrict__ blk_zoff,
-                                     const int *__restrict__ blk_tb, const double *__restrict__ blk_tf, int ldim,
-                                     double *__restrict__ db) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= natoms * nb) return;
-    const int64_t i = t / nb;
-    const int b = (int)(t - i * nb);
-    const cd *du = reinterpret_cast<const cd *>(dutot) + (size_t)i * nhalf;
-    const cd *zi = reinterpret_cast<const cd *>(z) + (size_t)i * nz;
-    double acc = 0.0;
-    for (int l = 0; l < ldim; l++) {
-        for (int k = blk_level_start[l]; k < blk_level_start[l + 1]; k++) {
-            double f = 0.0;
-            for (int s = 0; s < 3; s++)
-                if (blk_tb[3 * k + s] == b) f += blk_tf[3 * k + s];
-            if (f == 0.0) continue;
-            const cd *zb = zi + blk_zoff[k];
-            const cd *ub = du + hoff[l];
-            double p = 0.0;
-            int loc = 0;
-            for (int hm = 0; 2 * hm < l; hm++)
-                for (int ma = 0; ma <= l; ma++, loc++) p += cdotre(ub[loc], zb[loc]);
-            if ((l & 1) == 0) {
-                const int hm = l / 2;
-                for (int ma = 0; ma < hm; ma++, loc++) p += cdotre(ub[loc], zb[loc]);
-                p += 0.5 * cdotre(ub[loc], zb[loc]);
-            }
-            acc += 2.0 * f * p;
+// dB_b = sum over the (block k, factor f) pairs targeting b of f p_k with
+// p_k = 2 Re sum_loc hw(loc) conj(dU^l[loc]) Z_k[loc] and dU = dU_tot (u_norm
+// already applied by the accumulation kernel).  One workgroup per atom
+// (persistent): dU into LDS, then the per-element products over the whole
+// Z row of the atom (coalesced reads), per-block sums, per-b gathers.
+__global__ __launch_bounds__(256) void so4_tangent_b_kernel(int64_t natoms, int nhalf, int nz, int nb, int nblk,
+                                                             int ldim, const double *__restrict__ dutot,
+                                                             const double *__restrict__ z, const int *__restrict__ zh,
+                                                             const double *__restrict__ h_weight,
+                                                             const int *__restrict__ blk_level_start,
+                                                             const int *__restrict__ blk_zoff,
+                                                             const int *__restrict__ b_src,
+                                                             const double *__restrict__ b_srcf,
+                                                             double *__restrict__ db) {
+    extern __shared__ double sm[];
+    cd *du = reinterpret_cast<cd *>(sm);  // [nhalf]
+    double *q = sm + 2 * nhalf;           // [nz]
+    double *pk = q + nz;                  // [nblk]
+    const int tid = threadIdx.x;
+    for (int64_t i = blockIdx.x; i < natoms; i += gridDim.x) {
+        const cd *dui = reinterpret_cast<const cd *>(dutot) + (size_t)i * nhalf;
+        const cd *zi = reinterpret_cast<const cd *>(z) + (size_t)i * nz;
+        for (int h = tid; h < nhalf; h += blockDim.x) du[h] = dui[h];
+        __syncthreads();
+        for (int e = tid; e < nz; e += blockDim.x) {
+            const int h = zh[e];
+            q[e] = 2.0 * h_weight[h] * cdotre(du[h], zi[e]);
         }
+        __syncthreads();
+        for (int k = tid; k < nblk; k += blockDim.x) {
+            int l = 0;
+            while (l + 1 < ldim && k >= blk_level_start[l + 1]) l++;
+            const int nloc = (l / 2 + 1) * (l + 1);
+            const int z0 = blk_zoff[k];
+            double p = 0.0;
+            for (int t = 0; t < nloc; t++) p += q[z0 + t];
+            pk[k] = p;
+        }
+        __syncthreads();
+        for (int b = tid; b < nb; b += blockDim.x) {
+            double acc = 0.0;
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                const int k = b_src[3 * b + t];
+                if (k >= 0) acc += b_srcf[3 * b + t] * pk[k];
+            }
+            db[(size_t)i * nb + b] = acc;
+        }
+        __syncthreads();
     }
-    db[t] = acc;
 }
 
 }  // namespace pxff
@@ -1051,18 +855,14 @@ __global__ void so4_tangent_b_kernel(int64_t natoms, int nhalf, int nz, int nb, 
 // =================================================================== C ABI
 using namespace pxff;
 
-#define PXFF_DISPATCH_TWOJ(twol, ...)                                             \
-    switch (twol) {                                                               \
-    case 0: { constexpr int TJ = 0; __VA_ARGS__; } break;                         \
-    case 2: { constexpr int TJ = 2; __VA_ARGS__; } break;                         \
-    case 4: { constexpr int TJ = 4; __VA_ARGS__; } break;                         \
-    case 6: { constexpr int TJ = 6; __VA_ARGS__; } break;                         \
-    case 8: { constexpr int TJ = 8; __VA_ARGS__; } break;                         \
-    case 10: { constexpr int TJ = 10; __VA_ARGS__; } break;                       \
-    default: set_error("unsupported 2J"); return PXFF_ENOTIMPL;                   \
-    }
+namespace pxff {
+// Process-wide launch parameters set by the autotuner (pxff_tune_set).
+static int g_tune[PXFF_TUNE_COUNT] = {16, 16, 4, 0};
+static const int g_tune_lo[PXFF_TUNE_COUNT] = {1, 1, 1, 0};
+static const int g_tune_hi[PXFF_TUNE_COUNT] = {256, 256, 4, 1};
+int tune_value(int knob) { return g_tune[knob]; }
 
-static int check_params(const pxff_so4_params *p) {
+int so4_check_params(const pxff_so4_params *p) {
     PXFF_REQUIRE(p != nullptr, "null params");
     if (p->twol < 0 || p->twol > 10 || (p->twol & 1)) {
         set_error("unsupported 2J=" + std::to_string(p->twol) + " (even 2J in 0..10 supported)");
@@ -1074,7 +874,7 @@ static int check_params(const pxff_so4_params *p) {
 }
 
 static int g_num_cus = 0;
-static int num_cus() {
+int num_cus() {
     if (g_num_cus == 0) {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess) return 256;
@@ -1084,6 +884,7 @@ static int num_cus() {
     }
     return g_num_cus;
 }
+}  // namespace pxff
 
 extern "C" int pxff_so4_sizes(int twol, int *nu, int *nhalf, int *nz, int *nb) {
     So4HostTables t;
@@ -1101,7 +902,7 @@ extern "C" int pxff_so4_sizes(int twol, int *nu, int *nhalf, int *nz, int *nb) {
 extern "C" int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start,
                            const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
                            const double *d_site_w, const int64_t *d_work_order, double *d_utot, void *stream) {
-    int rc = check_params(p);
+    int rc = so4_check_params(p);
     if (rc) return rc;
     if (natoms <= 0) return 0;
     const So4Dev *T = nullptr;
@@ -1109,65 +910,48 @@ extern "C" int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64
     hipStream_t s = (hipStream_t)stream;
     const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * g_tune[PXFF_TUNE_ACCUM_WAVES_PER_CU]);
     const double fcut0 = cutoff_f0_host(p->cutoff_id);
-    PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, false, true><<<grid, 64, 0, s>>>(
-                                    natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_site_w, nullptr, d_work_order,
-                                    T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, fcut0, d_utot)));
+    if (tune_value(PXFF_TUNE_ACCUM_LDSADD)) {
+        PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, false, true, true><<<grid, 64, 0, s>>>(
+                                        natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_site_w, nullptr,
+                                        d_work_order, T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, fcut0,
+                                        d_utot)));
+    } else {
+        PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, false, true, false><<<grid, 64, 0, s>>>(
+                                        natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_site_w, nullptr,
+                                        d_work_order, T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, fcut0,
+                                        d_utot)));
+    }
     PXFF_LAUNCH_CHECK();
-    return 0;
-}
-
-extern "C" int pxff_so4_zb(const pxff_so4_params *p, int64_t natoms, const double *d_utot, double *d_z, double *d_b,
-                           void *stream) {
-    int rc = check_params(p);
-    if (rc) return rc;
-    if (natoms <= 0) return 0;
-    const So4Dev *T = nullptr;
-    if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    const int ntiles = (int)((natoms + 63) / 64);
-    const int nch = std::min(g_tune[PXFF_TUNE_Z_CHUNKS], T->nzrows);
-    const int *zch = nullptr;
-    if ((rc = get_zchunks(p->twol, p->normalize_u, nch, &zch))) return rc;
-    const int grid = 8 * ((ntiles + 7) / 8) * ((nch + 3) / 4);
-    const size_t lds = sizeof(double) * (size_t)T->ncg;
-    PXFF_DISPATCH_TWOJ(p->twol, {
-        auto kern = so4_z_tile_kernel<TJ>;
-        if (lds > 65536)
-            PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        kern<<<grid, 256, lds, s>>>(natoms, ntiles, nch, zch, T->zrows, T->cg, T->ncg, d_utot, d_z, T->nz);
-    });
-    PXFF_LAUNCH_CHECK();
-    if (d_b) return pxff_so4_b(p, natoms, d_utot, d_z, d_b, stream);
     return 0;
 }
 
 extern "C" int pxff_so4_b(const pxff_so4_params *p, int64_t natoms, const double *d_utot, const double *d_z,
                           double *d_b, void *stream) {
-    int rc = check_params(p);
+    int rc = so4_check_params(p);
     if (rc) return rc;
     if (natoms <= 0) return 0;
     PXFF_REQUIRE(d_b != nullptr, "null B output");
     const So4Dev *T = nullptr;
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    PXFF_DISPATCH_TWOJ(p->twol, (so4_b_kernel<TJ><<<div_up(natoms * T->nb, 256), 256, 0, s>>>(
-                                    natoms, d_utot, d_z, T->nz, T->nb, T->b_zoff, T->b_level, d_b)));
+    const dim3 grid((unsigned)div_up(natoms, 64), (unsigned)div_up(T->nb, 16));
+    PXFF_DISPATCH_TWOJ(p->twol, (so4_b_kernel<TJ><<<grid, 256, 0, s>>>(natoms, d_utot, d_z, T->nz, T->nb, T->b_zoff,
+                                                                       T->b_level, d_b)));
     PXFF_LAUNCH_CHECK();
     return 0;
 }
 
 extern "C" int pxff_so4_yi(const pxff_so4_params *p, int64_t natoms, const double *d_z, const double *d_beta,
                            double *d_y, void *stream) {
-    int rc = check_params(p);
+    int rc = so4_check_params(p);
     if (rc) return rc;
     if (natoms <= 0) return 0;
     const So4Dev *T = nullptr;
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    const int64_t n = natoms * T->nhalf;
-    so4_yi_kernel<<<div_up(n, 256), 256, 0, s>>>(natoms, T->nhalf, T->nz, T->nb, d_z, d_beta, T->hoff, T->h_level,
-                                                 T->h_weight, T->lnorm, T->blk_level_start, T->blk_zoff, T->blk_tb,
-                                                 T->blk_tf, d_y);
+    const size_t lds = sizeof(double) * 4 * ((size_t)T->nblk + 2 * (size_t)T->nhalf);
+    so4_yi_kernel<<<div_up(natoms, 4), 256, lds, s>>>(natoms, T->nhalf, T->nz, T->nb, T->nblk, d_z, d_beta, T->zblk,
+                                                      T->zh, T->hscale, T->blk_tb, T->blk_tf, d_y);
     PXFF_LAUNCH_CHECK();
     return 0;
 }
@@ -1175,7 +959,7 @@ extern "C" int pxff_so4_yi(const pxff_so4_params *p, int64_t natoms, const doubl
 extern "C" int pxff_so4_deidrj(const pxff_so4_params *p, int64_t nimg, const int64_t *d_img_atom,
                                const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
                                const double *d_y, double *d_dedr, void *stream) {
-    int rc = check_params(p);
+    int rc = so4_check_params(p);
     if (rc) return rc;
     if (nimg <= 0) return 0;
     const So4Dev *T = nullptr;
@@ -1196,8 +980,8 @@ extern "C" int pxff_so4_gather(int64_t natoms, const int64_t *d_img_start, const
                                void *stream) {
     if (natoms <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
-    so4_gather_kernel<<<div_up(natoms, 256), 256, 0, s>>>(natoms, d_img_start, d_img_rev, d_img_r, d_dedr, d_grad,
-                                                          d_virial);
+    so4_gather_kernel<<<div_up(natoms, 4), 256, 0, s>>>(natoms, d_img_start, d_img_rev, d_img_r, d_dedr, d_grad,
+                                                        d_virial);
     PXFF_LAUNCH_CHECK();
     return 0;
 }
@@ -1205,20 +989,31 @@ extern "C" int pxff_so4_gather(int64_t natoms, const int64_t *d_img_start, const
 extern "C" int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start,
                                 const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
                                 const double *d_dr, const double *d_z, double *d_dutot, double *d_db, void *stream) {
-    int rc = check_params(p);
+    int rc = so4_check_params(p);
     if (rc) return rc;
     if (natoms <= 0) return 0;
     const So4Dev *T = nullptr;
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
     const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * g_tune[PXFF_TUNE_ACCUM_WAVES_PER_CU]);
-    PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, false><<<grid, 64, 0, s>>>(
-                                    natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr, d_dr, nullptr,
-                                    T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, 0.0, d_dutot)));
+    if (tune_value(PXFF_TUNE_ACCUM_LDSADD)) {
+        PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, false, true><<<grid, 64, 0, s>>>(
+                                        natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr, d_dr, nullptr,
+                                        T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, 0.0, d_dutot)));
+    } else {
+        PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, false, false><<<grid, 64, 0, s>>>(
+                                        natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr, d_dr, nullptr,
+                                        T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, 0.0, d_dutot)));
+    }
     PXFF_LAUNCH_CHECK();
-    so4_tangent_b_kernel<<<div_up(natoms * T->nb, 256), 256, 0, s>>>(natoms, T->nhalf, T->nz, T->nb, d_dutot, d_z,
-                                                                     T->hoff, T->blk_level_start, T->blk_zoff,
-                                                                     T->blk_tb, T->blk_tf, p->twol + 1, d_db);
+    const size_t lds = sizeof(double) * (2 * (size_t)T->nhalf + T->nz + T->nblk);
+    const int gridb = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * 8);
+    if (lds > 65536)
+        PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)so4_tangent_b_kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    so4_tangent_b_kernel<<<gridb, 256, lds, s>>>(natoms, T->nhalf, T->nz, T->nb, T->nblk, p->twol + 1, d_dutot, d_z,
+                                                 T->zh, T->h_weight, T->blk_level_start, T->blk_zoff, T->b_src,
+                                                 T->b_srcf, d_db);
     PXFF_LAUNCH_CHECK();
     return 0;
 }
@@ -1226,7 +1021,7 @@ extern "C" int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const 
 extern "C" int pxff_so4_dbdr(const pxff_so4_params *p, int64_t nimg, const int64_t *d_img_atom,
                              const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w, const double *d_z,
                              double *d_dbdr, void *stream) {
-    int rc = check_params(p);
+    int rc = so4_check_params(p);
     if (rc) return rc;
     if (nimg <= 0) return 0;
     const So4Dev *T = nullptr;

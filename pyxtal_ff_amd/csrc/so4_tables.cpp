@@ -236,24 +236,41 @@ bool build_so4_tables(int twol, int normalize_u, So4HostTables &t) {
     }
     for (int l = 0; l < ldim; l++) t.blk_level_start[l + 1] += t.blk_level_start[l];
 
-    // Z rows for the atom-per-lane kernel, in idxz order
+    // Z row groups for the atom-per-lane kernel.  Row (l1, l2, l, mb) sums
+    // over the pairs (mb1, mb2) with mb1 + mb2 = K = mb + (l1 + l2 - l)/2;
+    // all rows of one (l1, l2) with the same K share those pairs (and so
+    // the products U^{l1}[mb1][ma1] U^{l2}[mb2][ma2]), one row per l at
+    // most.  A group is (l1, l2, K) with the rows of its active l.
     for (int l1 = 0; l1 < ldim; l1++)
-        for (int l2 = 0; l2 <= l1; l2++)
-            for (int l = l1 - l2; l <= std::min(twol, l1 + l2); l += 2) {
-                const int s2 = (l1 + l2 - l) / 2;
-                for (int mb = 0; 2 * mb <= l; mb++) {
-                    const int lo = std::max(0, mb + s2 - l2), hi = std::min(l1, mb + s2);
-                    int pairs = 0;
-                    for (int a = 0; a <= l1; a++)
-                        for (int b = 0; b <= l2; b++)
-                            if (a + b >= s2 && a + b <= s2 + l) pairs++;
-                    const int row[12] = {t.idxz_block[I3(l1, l2, l)] + mb * (l + 1), l1, l2, l, s2, mb, lo, hi - lo + 1,
-                                         t.idxcg_block[I3(l1, l2, l)], t.hoff[l1], t.hoff[l2],
-                                         (hi - lo + 1) * (pairs + l1 + l2 + 2) + (l + 1)};
-                    t.zrows.insert(t.zrows.end(), row, row + 12);
-                    t.nzrows++;
+        for (int l2 = 0; l2 <= l1; l2++) {
+            const int lmin = l1 - l2, lmax = std::min(twol, l1 + l2);
+            for (int K = 0; K <= l1 + l2; K++) {
+                std::vector<int> g(ZG_STRIDE, -1);
+                int mask = 0, cost = 0;
+                const int lo = std::max(0, K - l2), hi = std::min(l1, K);
+                if (hi < lo) continue;
+                for (int l = lmin, j = 0; l <= lmax; l += 2, j++) {
+                    const int s2 = (l1 + l2 - l) / 2, mb = K - s2;
+                    g[ZG_CGBLK + j] = t.idxcg_block[I3(l1, l2, l)];
+                    if (mb < 0 || 2 * mb > l) continue;
+                    mask |= 1 << j;
+                    g[ZG_ZROW + j] = t.idxz_block[I3(l1, l2, l)] + mb * (l + 1);
+                    cost += (l + 1) * (hi - lo + 1) * (l1 + 1);
                 }
+                if (!mask) continue;
+                g[0] = l1;
+                g[1] = l2;
+                g[2] = K;
+                g[3] = lo;
+                g[4] = hi - lo + 1;
+                g[5] = t.hoff[l1];
+                g[6] = t.hoff[l2];
+                g[7] = mask;
+                g[8] = cost + (hi - lo + 1) * (l1 + l2 + 2) * 4;
+                t.zgroups.insert(t.zgroups.end(), g.begin(), g.end());
+                t.nzgroups++;
             }
+        }
 
     // init_rootpqarray, SNAP.py:861-868
     t.rootpq.assign(ldim * ldim, 0.0);
@@ -262,18 +279,18 @@ bool build_so4_tables(int twol, int normalize_u, So4HostTables &t) {
     return true;
 }
 
-std::vector<int> chunk_zrows(const So4HostTables &t, int nchunks) {
+std::vector<int> chunk_zgroups(const So4HostTables &t, int nchunks) {
     long total = 0;
-    for (int r = 0; r < t.nzrows; r++) total += t.zrows[12 * r + 11];
+    for (int r = 0; r < t.nzgroups; r++) total += t.zgroups[ZG_STRIDE * r + 8];
     std::vector<int> start(1, 0);
     long acc = 0;
-    for (int r = 0; r < t.nzrows; r++) {
-        acc += t.zrows[12 * r + 11];
-        if ((int)start.size() < nchunks && acc * nchunks >= total * (long)start.size() && r + 1 < t.nzrows)
+    for (int r = 0; r < t.nzgroups; r++) {
+        acc += t.zgroups[ZG_STRIDE * r + 8];
+        if ((int)start.size() < nchunks && acc * nchunks >= total * (long)start.size() && r + 1 < t.nzgroups)
             start.push_back(r + 1);
     }
-    while ((int)start.size() < nchunks) start.push_back(t.nzrows);
-    start.push_back(t.nzrows);
+    while ((int)start.size() < nchunks) start.push_back(t.nzgroups);
+    start.push_back(t.nzgroups);
     return start;
 }
 

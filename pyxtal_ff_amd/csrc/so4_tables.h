@@ -37,14 +37,21 @@ struct So4HostTables {
     std::vector<int> h_level;            // [nhalf] level of each half element
     std::vector<double> h_weight;        // [nhalf] half weight (1, 0.5 or 0)
     std::vector<int> h_diag;             // [nhalf] 1 if mb == ma
-    // Z rows (block, mb) for the atom-per-lane kernel: per row
-    // {zrow, l1, l2, l, s, mb, mb1lo, nbb, cgblock, hoff[l1], hoff[l2], cost}
-    std::vector<int> zrows;              // [nzrows*12]
-    int nzrows = 0;
+    // Z row groups (l1, l2, K) for the atom-per-lane kernel, ZG_STRIDE ints
+    // each: {l1, l2, K, mb1lo, nbb, hoff[l1], hoff[l2], active-l mask, cost,
+    // Z offset of the row of each l (ZG_ZROW + j, -1 if inactive), CG block
+    // of each l (ZG_CGBLK + j)}, l = l1 - l2 + 2 j
+    std::vector<int> zgroups;            // [nzgroups*ZG_STRIDE]
+    int nzgroups = 0;
 };
 
-// Split the Z rows into nchunks contiguous chunks of about equal cost.
-std::vector<int> chunk_zrows(const So4HostTables &t, int nchunks);
+constexpr int ZG_MAXL = 6;  // at most six l per (l1, l2) for 2J <= 10
+constexpr int ZG_ZROW = 9;
+constexpr int ZG_CGBLK = ZG_ZROW + ZG_MAXL;
+constexpr int ZG_STRIDE = 24;
+
+// Split the Z row groups into nchunks contiguous chunks of about equal cost.
+std::vector<int> chunk_zgroups(const So4HostTables &t, int nchunks);
 
 // Build the tables.  Returns false for unsupported 2J.
 bool build_so4_tables(int twol, int normalize_u, So4HostTables &t);

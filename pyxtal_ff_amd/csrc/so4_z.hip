@@ -1,0 +1,247 @@
+// Z of the SO(4)/SNAP bispectrum (compute_zi, SNAP.py:1050-1098) for gfx950,
+// one atom per lane.  Kept in its own translation unit: the kernel is
+// instantiated for every (l1, l2) pair of every supported 2J.
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "pxff_common.h"
+#include "so4_dev.h"
+#include "so4_tables.h"
+
+namespace pxff {
+
+// Z-group chunk tables per (device, 2J, normalisation, nchunks)
+static std::mutex g_chunk_mutex;
+static std::map<std::tuple<int, int, int>, int *> g_chunks;
+static int get_zchunks(int twol, int norm, int nch, const int **out) {
+    int dev = 0;
+    PXFF_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_chunk_mutex);
+    auto key = std::make_tuple(dev, twol * 2 + (norm ? 1 : 0), nch);
+    auto it = g_chunks.find(key);
+    if (it != g_chunks.end()) {
+        *out = it->second;
+        return 0;
+    }
+    So4HostTables t;
+    if (!build_so4_tables(twol, norm, t)) {
+        set_error("unsupported 2J");
+        return PXFF_ENOTIMPL;
+    }
+    std::vector<int> c = chunk_zgroups(t, nch);
+    int *d = nullptr;
+    PXFF_HIP_CHECK(hipMalloc((void **)&d, c.size() * sizeof(int)));
+    PXFF_HIP_CHECK(hipMemcpy(d, c.data(), c.size() * sizeof(int), hipMemcpyHostToDevice));
+    g_chunks[key] = d;
+    *out = d;
+    return 0;
+}
+
+
+// ------------------------------------------------- Z, atom per lane
+// compute_zi (SNAP.py:1050-1098) with one atom per lane: a wavefront owns a
+// tile of 64 atoms and a chunk of Z row groups, so the loop structure and
+// the Clebsch-Gordan coefficients are the same for all lanes (uniform
+// branches, broadcast LDS reads).  Row (l1, l2, l, mb) is
+//   Z[ma] = sum_{mb1+mb2=K} C[mb1][mb2] sum_{ma1+ma2=ma+s} C[ma1][ma2] U1[mb1][ma1] U2[mb2][ma2],
+// s = (l1 + l2 - l)/2, K = mb + s.  The rows of one (l1, l2) with the same
+// K (one per l) share every (mb1, mb2) pair and hence every product
+// U1[mb1][ma1] U2[mb2][ma2]: a group loads U^{l1} row mb1 and U^{l2} row
+// mb2 once per pair (coalesced loads from the tile-major U_tot, rows beyond
+// l/2 by inversion symmetry), forms each product once and adds it, with
+// that l's coefficients, into the register accumulators of every l whose
+// ma range holds ma1 + ma2.  Accumulators of l without a row in the group
+// get a zero coefficient (uniform predication instead of branches).
+// Workgroup ids are mapped so that the chunks of one tile run on the same
+// XCD (workgroups are dispatched to the 8 XCDs round-robin), sharing its L2.
+// Compile-time loop: f(std::integral_constant<int, 0>) ... f(<N-1>).
+template <typename F, int... I>
+__device__ inline void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ inline void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+template <int TWOJ>
+__device__ inline void load_urow(const cd *__restrict__ ut, int lv, int mb, int hoffl, int lane, cd *u) {
+    const bool flip = 2 * mb > lv;
+    const int mrow = flip ? lv - mb : mb;
+    const int base = hoffl + mrow * (lv + 1);
+#pragma unroll
+    for (int ma = 0; ma <= TWOJ; ma++) {
+        if (ma <= lv) {
+            const int col = flip ? lv - ma : ma;
+            cd v = ut[(base + col) * 64 + lane];
+            if (flip) {
+                const double sg = ((ma + mb) & 1) ? -1.0 : 1.0;
+                v = cmake(sg * v.re, -sg * v.im);
+            }
+            u[ma] = v;
+        } else {
+            u[ma] = cmake(0.0, 0.0);
+        }
+    }
+}
+
+// The l of (L1, L2): LMIN + 2 j, j < nl_of; accumulator offset of l_j.
+__host__ __device__ constexpr int zg_lmin(int L1, int L2) { return L1 - L2; }
+__host__ __device__ constexpr int zg_nl(int TWOJ, int L1, int L2) {
+    return ((TWOJ < L1 + L2 ? TWOJ : L1 + L2) - (L1 - L2)) / 2 + 1;
+}
+__host__ __device__ constexpr int zg_accoff(int L1, int L2, int j) {
+    int o = 0;
+    for (int q = 0; q < j; q++) o += L1 - L2 + 2 * q + 1;
+    return o;
+}
+
+template <int TWOJ, int L1, int L2>
+__device__ __forceinline__ void zgroup(const cd *__restrict__ ut, int lane, const double *__restrict__ cg,
+                                       const int *__restrict__ g, cd *__restrict__ zo, bool valid) {
+    constexpr int NL = zg_nl(TWOJ, L1, L2);
+    constexpr int NACC = zg_accoff(L1, L2, NL);
+    constexpr int L2P = L2 + 1;
+    const int K = __builtin_amdgcn_readfirstlane(g[2]);
+    const int mb1lo = __builtin_amdgcn_readfirstlane(g[3]);
+    const int nbb = __builtin_amdgcn_readfirstlane(g[4]);
+    const int h1 = __builtin_amdgcn_readfirstlane(g[5]);
+    const int h2 = __builtin_amdgcn_readfirstlane(g[6]);
+    const int mask = __builtin_amdgcn_readfirstlane(g[7]);
+    int cgblk[NL];
+#pragma unroll
+    for (int j = 0; j < NL; j++) cgblk[j] = __builtin_amdgcn_readfirstlane(g[ZG_CGBLK + j]);
+    cd acc[NACC];
+#pragma unroll
+    for (int t = 0; t < NACC; t++) acc[t] = cmake(0.0, 0.0);
+    for (int ib = 0; ib < nbb; ib++) {
+        const int mb1 = mb1lo + ib, mb2 = K - mb1;
+        cd u1[L1 + 1], u2[L2 + 1];
+        load_urow<L1>(ut, L1, mb1, h1, lane, u1);
+        load_urow<L2>(ut, L2, mb2, h2, lane, u2);
+        double cgb[NL];
+#pragma unroll
+        for (int j = 0; j < NL; j++) cgb[j] = ((mask >> j) & 1) ? cg[cgblk[j] + mb1 * L2P + mb2] : 0.0;
+        // keep the CG reads of the product loop inside the pair loop:
+        // hoisted, they would not fit in registers beside acc and U
+        asm volatile("" ::: "memory");
+        static_for<L1 + 1>([&](auto A) {
+            constexpr int a = decltype(A)::value;
+            static_for<L2 + 1>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                constexpr int t = a + c;
+                // does any l take ma1 + ma2 = t?  (t in [s_l, s_l + l])
+                constexpr bool used = (t >= (L1 + L2 - (NL - 1) * 2 - (L1 - L2)) / 2) &&
+                                      (t <= (L1 + L2 + (L1 - L2) + 2 * (NL - 1)) / 2);
+                if constexpr (used) {
+                    const cd pr = cmul(u1[a], u2[c]);
+                    static_for<NL>([&](auto J) {
+                        constexpr int j = decltype(J)::value;
+                        constexpr int l = L1 - L2 + 2 * j;
+                        constexpr int s = (L1 + L2 - l) / 2;
+                        if constexpr (t >= s && t <= s + l) {
+                            constexpr int o = zg_accoff(L1, L2, j) + t - s;
+                            const double w = cgb[j] * cg[cgblk[j] + a * L2P + c];
+                            acc[o].re += w * pr.re;
+                            acc[o].im += w * pr.im;
+                        }
+                    });
+                }
+            });
+        });
+    }
+    if (valid) {
+        static_for<NL>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            constexpr int l = L1 - L2 + 2 * j;
+            if ((mask >> j) & 1) {
+                const int zrow0 = __builtin_amdgcn_readfirstlane(g[ZG_ZROW + j]);
+#pragma unroll
+                for (int m = 0; m <= l; m++) zo[zrow0 + m] = acc[zg_accoff(L1, L2, j) + m];
+            }
+        });
+    }
+}
+
+// Uniform dispatch to zgroup<TWOJ, l1, l2>.
+template <int TWOJ, int L1, int L2>
+__device__ inline void zgroup_dispatch(int l1, int l2, const cd *ut, int lane, const double *cg, const int *g, cd *zo,
+                                       bool valid) {
+    if constexpr (L1 <= TWOJ) {
+        if (l1 == L1 && l2 == L2) {
+            zgroup<TWOJ, L1, L2>(ut, lane, cg, g, zo, valid);
+            return;
+        }
+        if constexpr (L2 < L1)
+            zgroup_dispatch<TWOJ, L1, L2 + 1>(l1, l2, ut, lane, cg, g, zo, valid);
+        else
+            zgroup_dispatch<TWOJ, L1 + 1, 0>(l1, l2, ut, lane, cg, g, zo, valid);
+    }
+}
+
+template <int TWOJ>
+__global__ __launch_bounds__(256) void so4_z_tile_kernel(int64_t natoms, int ntiles, int nch, const int *__restrict__ zchunk,
+                                                          const int *__restrict__ zgroups, const double *__restrict__ cg_g,
+                                                          int ncg, const double *__restrict__ utile,
+                                                          double *__restrict__ zout, int nz) {
+    constexpr int NH = nhalf_of(TWOJ);
+    extern __shared__ double cg[];  // the whole CG table; uniform-address reads broadcast
+    for (int k = threadIdx.x; k < ncg; k += blockDim.x) cg[k] = cg_g[k];
+    __syncthreads();
+    // workgroup = 4 wavefronts on 4 consecutive chunks of one tile
+    const int b = blockIdx.x;
+    const int nq = (nch + 3) / 4;
+    const int xcd = b & 7, k = b >> 3;
+    const int tile = xcd + 8 * (k / nq);
+    const int ch = 4 * (k % nq) + (threadIdx.x >> 6);
+    if (tile >= ntiles || ch >= nch) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t atom = (int64_t)tile * 64 + lane;
+    const bool valid = atom < natoms;
+    const cd *ut = reinterpret_cast<const cd *>(utile) + (size_t)tile * NH * 64;
+    cd *zo = reinterpret_cast<cd *>(zout) + (size_t)atom * nz;
+    const int r0 = __builtin_amdgcn_readfirstlane(zchunk[ch]);
+    const int r1 = __builtin_amdgcn_readfirstlane(zchunk[ch + 1]);
+    for (int r = r0; r < r1; r++) {
+        const int *g = zgroups + ZG_STRIDE * r;
+        const int l1 = __builtin_amdgcn_readfirstlane(g[0]);
+        const int l2 = __builtin_amdgcn_readfirstlane(g[1]);
+        zgroup_dispatch<TWOJ, 0, 0>(l1, l2, ut, lane, cg, g, zo, valid);
+    }
+}
+
+}  // namespace pxff
+
+// =================================================================== C ABI
+using namespace pxff;
+
+extern "C" int pxff_so4_zb(const pxff_so4_params *p, int64_t natoms, const double *d_utot, double *d_z, double *d_b,
+                           void *stream) {
+    int rc = so4_check_params(p);
+    if (rc) return rc;
+    if (natoms <= 0) return 0;
+    const So4Dev *T = nullptr;
+    if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const int ntiles = (int)((natoms + 63) / 64);
+    const int nch = std::min(tune_value(PXFF_TUNE_Z_CHUNKS), T->nzgroups);
+    const int *zch = nullptr;
+    if ((rc = get_zchunks(p->twol, p->normalize_u, nch, &zch))) return rc;
+    const int grid = 8 * ((ntiles + 7) / 8) * ((nch + 3) / 4);
+    const size_t lds = sizeof(double) * (size_t)T->ncg;
+    PXFF_DISPATCH_TWOJ(p->twol, {
+        auto kern = so4_z_tile_kernel<TJ>;
+        if (lds > 65536)
+            PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        kern<<<grid, 256, lds, s>>>(natoms, ntiles, nch, zch, T->zgroups, T->cg, T->ncg, d_utot, d_z, T->nz);
+    });
+    PXFF_LAUNCH_CHECK();
+    if (d_b) return pxff_so4_b(p, natoms, d_utot, d_z, d_b, stream);
+    return 0;
+}
+

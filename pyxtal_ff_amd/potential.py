@@ -7,7 +7,11 @@ energy / force / stress assembly (``calculate_properties``,
 ``neuralnetwork.py:579-678``), computed through either
 
   * the adjoint path: F = -dE/dR by ``torch.autograd`` through
-    ``adjoint.bispectrum`` (never materialises dB/dR), or
+    ``adjoint.bispectrum`` (never materialises dB/dR; differentiable in the
+    network parameters, for training), or
+  * the fused adjoint path: the same kernels called directly, with the
+    networks evaluated forward and backward to dE/dB by one kernel
+    (``pxff_mlp_energy_grad``) -- energy + force evaluation, or
   * the plain path: the reference's materialised dxdr [npairs, nb, 3] and the
     contraction F_m = -sum_rows dE/dx_i . dxdr[(i, m)].
 """
@@ -20,11 +24,28 @@ from . import so4 as dev_so4
 EV2GPA = 160.21766208
 
 
+# activations of the reference's networks (neuralnetwork.py:105-109) that
+# the fused kernel (pxff_mlp_energy_grad) evaluates, with its ids
+ACTIVATIONS = {"Linear": 0, "Tanh": 1, "Sigmoid": 2, "ReLU": 3, "Softplus": 4, "ELU": 5}
+_TORCH_ACT = {"Linear": lambda v: v, "Tanh": torch.tanh, "Sigmoid": torch.sigmoid,
+              "ReLU": torch.relu, "Softplus": torch.nn.functional.softplus, "ELU": torch.nn.functional.elu}
+
+
 class ElementMLP(torch.nn.Module):
-    def __init__(self, nin, hidden=(30, 30), seed=0):
+    """nn.Linear layers, one activation after each hidden layer, a linear
+    output layer of width 1 (neuralnetwork.py:186-200)."""
+
+    def __init__(self, nin, hidden=(30, 30), seed=0, activation="Tanh"):
         super().__init__()
         g = torch.Generator().manual_seed(int(seed))
         dims = [nin] + list(hidden) + [1]
+        acts = [activation] * len(hidden) if isinstance(activation, str) else list(activation)
+        if len(acts) != len(hidden):
+            raise ValueError("one activation per hidden layer")
+        for a in acts:
+            if a not in ACTIVATIONS:
+                raise NotImplementedError(f"activation {a} (available: {sorted(ACTIVATIONS)})")
+        self.acts = acts + ["Linear"]
         self.layers = torch.nn.ModuleList()
         for a, b in zip(dims[:-1], dims[1:]):
             lin = torch.nn.Linear(a, b, dtype=torch.float64)
@@ -35,10 +56,8 @@ class ElementMLP(torch.nn.Module):
             self.layers.append(lin)
 
     def forward(self, x):
-        for k, lin in enumerate(self.layers):
-            x = lin(x)
-            if k < len(self.layers) - 1:
-                x = torch.tanh(x)
+        for lin, act in zip(self.layers, self.acts):
+            x = _TORCH_ACT[act](lin(x))
         return x.squeeze(-1)
 
 
@@ -46,11 +65,13 @@ class NNPotential(torch.nn.Module):
     """Per-element MLPs; atoms of element e use models[e] on
     (x - dmin_e) / (dmax_e - dmin_e)."""
 
-    def __init__(self, elements, nin, hidden=(30, 30), seed=0, drange=None):
+    def __init__(self, elements, nin, hidden=(30, 30), seed=0, drange=None, activation="Tanh"):
         super().__init__()
         self.elements = list(elements)
+        self.nin = int(nin)
         self.models = torch.nn.ModuleDict(
-            {e: ElementMLP(nin, hidden, seed + 17 * k) for k, e in enumerate(self.elements)})
+            {e: ElementMLP(nin, hidden, seed + 17 * k, activation) for k, e in enumerate(self.elements)})
+        self._packed = None
         for e in self.elements:
             lo = torch.zeros(nin, dtype=torch.float64)
             sc = torch.ones(nin, dtype=torch.float64)
@@ -69,6 +90,60 @@ class NNPotential(torch.nn.Module):
             xe = (xe - getattr(self, "dmin_" + e)) * getattr(self, "scale_" + e)
             out = out.index_add(0, idx, self.models[e](xe))
         return out
+
+    def packed(self):
+        """Parameters in the fused kernel's layout: (host layout int32,
+        device layout, params, dmin [nelem, nin], scale [nelem, nin]).
+        Rebuilt when any parameter or buffer changed in place (optimiser
+        steps bump tensor versions)."""
+        ts = list(self.parameters()) + list(self.buffers())
+        key = tuple((t.data_ptr(), t._version) for t in ts)
+        if self._packed is not None and self._packed[0] == key:
+            return self._packed[1]
+        stride = 1 + 5 * 8
+        lay = np.zeros((len(self.elements), stride), dtype=np.int32)
+        chunks, off = [], 0
+        for ei, e in enumerate(self.elements):
+            m = self.models[e]
+            if len(m.layers) > 8:
+                raise NotImplementedError("the fused network kernel takes at most 8 layers")
+            lay[ei, 0] = len(m.layers)
+            for k, (lin, act) in enumerate(zip(m.layers, m.acts)):
+                w = lin.weight.detach().reshape(-1)
+                b = lin.bias.detach().reshape(-1)
+                lay[ei, 1 + 5 * k: 6 + 5 * k] = [lin.in_features, lin.out_features, ACTIVATIONS[act], off,
+                                                 off + w.numel()]
+                chunks += [w, b]
+                off += w.numel() + b.numel()
+        params = torch.cat(chunks).contiguous()
+        dev = params.device
+        dmin = torch.stack([getattr(self, "dmin_" + e) for e in self.elements]).to(dev).contiguous()
+        scale = torch.stack([getattr(self, "scale_" + e) for e in self.elements]).to(dev).contiguous()
+        out = (lay, torch.as_tensor(lay, device=dev), params, dmin, scale)
+        self._packed = (key, out)
+        return out
+
+    def energy_grad(self, x, species_atom):
+        """Atomic energies [N] and dE/dx [N, nin] by the fused kernel (no
+        autograd; parameters are constants).  species_atom: int64 [N],
+        index into ``elements``."""
+        from . import _native as nat
+        lay, dlay, params, dmin, scale = self.packed()
+        n = x.shape[0]
+        x = x.contiguous()
+        e = torch.empty(max(n, 1), dtype=torch.float64, device=x.device)
+        beta = torch.empty((max(n, 1), self.nin), dtype=torch.float64, device=x.device)
+        nat.check(nat.lib().pxff_mlp_energy_grad(
+            n, self.nin, len(self.elements), nat.ptr(species_atom), lay.ctypes.data_as(nat.P), nat.ptr(dlay),
+            nat.ptr(params), nat.ptr(dmin), nat.ptr(scale), nat.ptr(x), nat.ptr(e), nat.ptr(beta), nat.stream()),
+            "pxff_mlp_energy_grad")
+        return e[:n], beta[:n]
+
+
+def species_atom(symbols, elements, device):
+    """Per-atom index into ``elements`` (int64 device tensor)."""
+    pos = {e: k for k, e in enumerate(elements)}
+    return torch.as_tensor(np.array([pos[s] for s in symbols], dtype=np.int64), device=device)
 
 
 def species_index(symbols, elements, device):
@@ -96,6 +171,27 @@ def energy_forces_adjoint(nl, state, model, species, stress=False):
     out = {"energy": e.detach(), "forces": f, "atom_energies": e_atom.detach()}
     if stress:
         out["virial"] = grads[1]
+    return out
+
+
+def energy_forces_fused(nl, state, model, species_atom, virial=False):
+    """E, F (and the per-centre virial sum) through the adjoint kernels with
+    no autograd: U_tot -> Z -> B -> network energies and dE/dB (fused
+    kernel) -> Y -> dE/dr per image -> forces by the deterministic gather.
+    Same quantities as ``energy_forces_adjoint``; the network parameters are
+    constants here (energy + force evaluation, e.g. MD or the calculator)."""
+    cfg = state.cfg
+    b, z, _ = dev_so4.forward(nl, cfg, state.atom_w, state.site_w, state.work_order)
+    e_atom, beta = model.energy_grad(b, species_atom)
+    y = dev_so4.yi(nl, cfg, z, beta)
+    dedr = dev_so4.deidrj(nl, cfg, state.atom_w, y)
+    out = {"energy": e_atom.sum(), "atom_energies": e_atom}
+    if virial:
+        g, vir = dev_so4.gather(nl, dedr, virial=True)
+        out["virial"] = vir.sum(0)
+    else:
+        g = dev_so4.gather(nl, dedr)
+    out["forces"] = -g
     return out
 
 

@@ -57,13 +57,18 @@ def ui(nl, cfg, atom_w, site_w, work_order=None, img_r=None):
     return out
 
 
+def z(nl, cfg, utot):
+    """Z [N, nz, 2] (flat) from the tile-major U_tot (one kernel launch)."""
+    p = nat.ptr
+    zz = _f64(nl.natoms * cfg.nz * 2, nl.pos.device)
+    nat.check(nat.lib().pxff_so4_zb(cfg.params, nl.natoms, p(utot), p(zz), None, nat.stream()), "pxff_so4_zb")
+    return zz
+
+
 def zb(nl, cfg, utot, want_b=True):
     """Z [N, nz, 2] and B [N, nb]."""
-    p = nat.ptr
-    dev = nl.pos.device
-    z = _f64(nl.natoms * cfg.nz * 2, dev)
-    nat.check(nat.lib().pxff_so4_zb(cfg.params, nl.natoms, p(utot), p(z), None, nat.stream()), "pxff_so4_zb")
-    return z, (bi(nl, cfg, utot, z) if want_b else None)
+    zz = z(nl, cfg, utot)
+    return zz, (bi(nl, cfg, utot, zz) if want_b else None)
 
 
 def bi(nl, cfg, utot, z):

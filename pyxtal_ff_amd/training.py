@@ -10,10 +10,10 @@ any order.  Structures are fixed during training, so the neighbour list, Z
 network, the adjoint sweep (dE/dr per image, recomputed every step), its
 tangent, and the optimiser.
 
-Reductions avoid multi-block ``torch.sum`` on large tensors (which records
-memset nodes into captured graphs): per-atom terms are first index_add-ed
-into per-structure buckets, and only those (one value per structure) are
-summed.
+Per-structure sums (energies, force errors, virials) are segment sums over
+the contiguous atoms / images of each structure (``pxff_segment_sum``:
+fixed order, no atomics, no memset nodes in a captured graph); only the
+per-structure values (one per structure) go through torch reductions.
 """
 import torch
 
@@ -100,7 +100,7 @@ class ForceTrainer:
         dedr = self._dedr(beta.contiguous())
         forces = -adjoint._Gather.apply(dedr, nl)
         w = self.r.unsqueeze(2) * dedr.unsqueeze(1)  # r_e (x) dE/dr_e
-        W = torch.zeros((b.nstruct, 3, 3), dtype=torch.float64, device=b.device).index_add(0, b.img_struct, w)
+        W = b.per_structure_images(w.reshape(-1, 9)).view(b.nstruct, 3, 3)
         # Voigt components by a device index (no host-to-device copy in a captured step)
         S = -W.view(b.nstruct, 9).index_select(1, self._voigt) * b.inv_vol_struct[:, None]
         E = b.per_structure(e_atom)

@@ -273,3 +273,50 @@ def test_diamond4096_full_parity(oracle_mod):
     r = oracle_mod.snap_calculate(s.positions, s.cell, s.pbc, s.symbols, {"Si": 1.0}, lmax=4, rcut=4.9,
                                   derivative=True, stress=True)
     check_layout(g, r)
+
+
+def test_accumulation_modes_agree_and_reproduce():
+    """Both U_tot summation modes (per-group LDS partials, LDS atomic adds
+    issued by one wavefront) give the oracle's B to 1e-10; each mode is
+    bitwise reproducible run to run (forces too)."""
+    from pyxtal_ff_amd import autotune
+    s = workloads.diamond_si(reps=3, sigma=0.08, seed=4)
+    nl, cfg, aw, sw, model, species, state = _engine(s, weights={"Si": 1.0})
+    old = autotune.get_knob(autotune.ACCUM_LDSADD)
+    try:
+        res = {}
+        for mode in (0, 1):
+            autotune.set_knob(autotune.ACCUM_LDSADD, mode)
+            runs = []
+            for _ in range(2):
+                b, _, _ = dso4.forward(nl, cfg, aw, sw)
+                f = potential.energy_forces_adjoint(nl, state, model, species)["forces"]
+                runs.append((b.cpu().numpy(), f.cpu().numpy()))
+            _print("mode", mode, "rerun max |dB|", np.abs(runs[0][0] - runs[1][0]).max())
+            assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1]), mode
+            res[mode] = runs[0]
+        assert rel_err(res[1][0], res[0][0]) < 1e-13
+        assert rel_err(res[1][1], res[0][1]) < 1e-12
+    finally:
+        autotune.set_knob(autotune.ACCUM_LDSADD, old)
+
+
+@pytest.mark.parametrize("act", ["Tanh", "Sigmoid", "Softplus", "ELU", "Linear"])
+def test_fused_network_path_matches_autograd(act):
+    """Energies, forces and virial of the fused path (network forward and
+    backward in one kernel, adjoint kernels called directly) equal the
+    autograd adjoint path with the torch network (two elements, drange
+    scaling)."""
+    rng = np.random.default_rng(21)
+    s = workloads.random_sio_structure(rng, 20, 40)
+    nl, cfg, aw, sw, _, species, state = _engine(s, weights={"Si": 1.0, "O": 0.8})
+    x = dso4.forward(nl, cfg, aw, sw)[0].cpu().numpy()
+    dr = {e: np.stack([x.min(0) - 0.1, x.max(0) + 0.1], 1) for e in ("O", "Si")}
+    model = potential.NNPotential(["O", "Si"], cfg.nb, (24, 16), seed=3, drange=dr, activation=act).cuda()
+    sp_idx = potential.species_index(s.symbols, ["O", "Si"], "cuda")
+    sp_atom = potential.species_atom(s.symbols, ["O", "Si"], "cuda")
+    a = potential.energy_forces_adjoint(nl, state, model, sp_idx, stress=True)
+    f = potential.energy_forces_fused(nl, state, model, sp_atom, virial=True)
+    assert rel_err(f["atom_energies"].cpu().numpy(), a["atom_energies"].cpu().numpy()) < 1e-13
+    assert rel_err(f["forces"].cpu().numpy(), a["forces"].cpu().numpy()) < 1e-12
+    assert rel_err(f["virial"].cpu().numpy(), a["virial"].cpu().numpy()) < 1e-12
