@@ -38,7 +38,8 @@ extern "C" {
  *   PXFF_TUNE_ACCUM_WAVES_PER_CU    persistent grid of the per-centre kernels = CUs x this (1..256)
  *   PXFF_TUNE_IMAGE_WAVES_PER_BLOCK wavefronts per workgroup of the per-image adjoint kernel (1, 2, 4)
  *   PXFF_TUNE_ACCUM_LDSADD          per-centre U_tot sums: 0 = per-group LDS partials, 1 = one LDS array
- *                                   with LDS atomic adds (issued by one wavefront in a fixed order) */
+ *                                   with LDS atomic adds (issued by one wavefront in a fixed order),
+ *                                   2 = two groups per partial array, added in two ordered passes */
 #define PXFF_TUNE_Z_CHUNKS 0
 #define PXFF_TUNE_ACCUM_WAVES_PER_CU 1
 #define PXFF_TUNE_IMAGE_WAVES_PER_BLOCK 2
@@ -89,8 +90,10 @@ int pxff_nlist_fill(const double *d_pos, int64_t natoms, const int64_t *d_atom_s
 
 /* Phase 3: reverse images.  d_img_rev [nimg] int64 = index of image
  * (j, i, -T) for image (i, j, T) -- the list is exactly symmetric, which
- * the deterministic force gather relies on.  A missing reverse (cannot
- * happen for this construction) sets d_flag[0] (int32, zeroed by the call). */
+ * the deterministic force gather relies on.  nimg may be an upper bound
+ * (the image capacity): images at or beyond d_img_start[natoms] are
+ * skipped.  A missing reverse (cannot happen for this construction) sets
+ * d_flag[0] (int32, zeroed by the call). */
 int pxff_nlist_rev(int64_t natoms, int64_t nimg, const int64_t *d_img_start, const int64_t *d_img_atom,
                    const int64_t *d_img_j, const int32_t *d_img_T, int64_t *d_img_rev, int32_t *d_flag,
                    void *stream);
@@ -108,6 +111,14 @@ int pxff_nlist_rows(int64_t natoms, int64_t nimg, int64_t nrows, const int64_t *
 
 /* Exclusive scan: d_out [n+1], d_out[n] = total. */
 int pxff_scan_i64(const int64_t *d_in, int64_t n, int64_t *d_out, void *stream);
+
+/* Exclusive scan clamped to cap (every d_out[k] <= cap) with the unclamped
+ * total in d_total[0]: image offsets for arrays of cap entries, sized before
+ * the total is known on the host (no synchronisation between the count and
+ * fill phases).  A total above cap means the list was cut; the caller
+ * re-runs the scan and fill phases with a larger capacity. */
+int pxff_scan_i64_capped(const int64_t *d_in, int64_t n, int64_t cap, int64_t *d_out, int64_t *d_total,
+                         void *stream);
 
 /* ------------------------------------------------------------------ *
  * SO(4) / SNAP bispectrum (replaces get_bispectrum_components,         *
@@ -131,7 +142,7 @@ int pxff_so4_sizes(int twol, int *nu, int *nhalf, int *nz, int *nb);
 /* U_tot (half rows, self term and u_norm included) per centre, stored
  * tile-major for the Z kernel: element h of atom i at
  * [(i/64)*nhalf + h]*64 + i%64 (complex); d_utot must hold ceil(natoms/64)*64
- * atoms and be zeroed by the caller (padding lanes are read).
+ * atoms (the padding lanes of the last tile are set to zero).
  *   d_img_start [natoms+1], d_img_j [nimg], d_img_r [nimg*3]; d_atom_w
  *   [natoms] = neighbour weight of each atom (pair weight of image = w[j]);
  *   d_site_w [natoms] = self weight.  d_work_order [natoms] int64 or NULL:

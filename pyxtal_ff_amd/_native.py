@@ -38,6 +38,7 @@ _SIGS = {
     "pxff_last_error": ([ctypes.c_char_p, ctypes.c_size_t], I32),
     "pxff_version": ([], ctypes.c_char_p),
     "pxff_scan_i64": ([P, I64, P, P], I32),
+    "pxff_scan_i64_capped": ([P, I64, I64, P, P, P], I32),
     "pxff_tune_set": ([ctypes.c_int, ctypes.c_int], I32),
     "pxff_tune_get": ([ctypes.c_int], I32),
     "pxff_nlist_count": ([P, I64, P, P, P, P, P, P, I64, F64, P, P, P, P, P, P, P], I32),

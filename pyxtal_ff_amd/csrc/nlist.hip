@@ -170,11 +170,12 @@ __global__ void nl_fill_kernel(int64_t natoms, const double *__restrict__ pos, c
                                double *__restrict__ img_r, int64_t *__restrict__ row_count, int64_t min_count) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natoms) return;
-    const int64_t s0 = img_start[i];
-    if (img_start[i + 1] - s0 < min_count) return;
+    const int64_t s0 = img_start[i], send = img_start[i + 1];
+    if (send - s0 < min_count) return;
     int64_t c = s0;
     nl_enumerate(i, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap, bin_atoms,
                  [&](int64_t j, int T0, int T1, int T2, const double *r) {
+                     if (c >= send) return;  // range clamped to the image capacity
                      img_j[c] = j;
                      img_T[3 * c] = T0;
                      img_T[3 * c + 1] = T1;
@@ -258,7 +259,7 @@ __global__ void nl_rev_kernel(int64_t natoms, int64_t nimg, const int64_t *__res
                               const int32_t *__restrict__ img_T, int64_t *__restrict__ img_rev,
                               int32_t *__restrict__ flag) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nimg) return;
+    if (e >= nimg || e >= img_start[natoms]) return;
     const int64_t i = img_atom[e], j = img_j[e];
     const int32_t T[3] = {-img_T[3 * e], -img_T[3 * e + 1], -img_T[3 * e + 2]};
     int64_t lo = img_start[j], hi = img_start[j + 1];
@@ -464,6 +465,7 @@ __global__ __launch_bounds__(64) void nl_fill_wave_kernel(
                 if (o.j == h.j) first = false;
             }
         }
+        if (rank >= k) continue;  // the range was clamped to the image capacity
         const int64_t d = s0 + rank;
         img_atom[d] = i;
         img_j[d] = h.j;
@@ -483,9 +485,12 @@ __global__ __launch_bounds__(64) void nl_fill_wave_kernel(
 
 // ------------------------------------------------------------ device scan
 // Exclusive scan by one workgroup of 1024 threads (chunked).  out has n+1
-// entries; out[n] = total.
+// entries; out[n] = total.  With cap >= 0 every output is clamped to cap
+// (so ranges read through it stay inside arrays of cap entries) and the
+// unclamped total goes to total_out[0].
 template <typename T>
-__global__ __launch_bounds__(1024) void scan_kernel(int64_t n, const T *__restrict__ in, T *__restrict__ out) {
+__global__ __launch_bounds__(1024) void scan_kernel(int64_t n, const T *__restrict__ in, T *__restrict__ out,
+                                                    T cap = -1, T *__restrict__ total_out = nullptr) {
     __shared__ T part[1024];
     const int tid = threadIdx.x;
     const int64_t chunk = (n + 1023) / 1024;
@@ -503,10 +508,14 @@ __global__ __launch_bounds__(1024) void scan_kernel(int64_t n, const T *__restri
     T run = (tid > 0) ? part[tid - 1] : 0;
     for (int64_t k = b; k < e; k++) {
         T v = in[k];
-        out[k] = run;
+        out[k] = (cap >= 0 && run > cap) ? cap : run;
         run += v;
     }
-    if (tid == 1023) out[n] = part[1023];
+    if (tid == 1023) {
+        const T tot = part[1023];
+        out[n] = (cap >= 0 && tot > cap) ? cap : tot;
+        if (total_out) total_out[0] = tot;
+    }
 }
 
 }  // namespace pxff
@@ -516,6 +525,15 @@ using namespace pxff;
 extern "C" int pxff_scan_i64(const int64_t *d_in, int64_t n, int64_t *d_out, void *stream) {
     PXFF_REQUIRE(n >= 0, "n must be >= 0");
     scan_kernel<int64_t><<<1, 1024, 0, (hipStream_t)stream>>>(n, d_in, d_out);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_scan_i64_capped(const int64_t *d_in, int64_t n, int64_t cap, int64_t *d_out, int64_t *d_total,
+                                    void *stream) {
+    PXFF_REQUIRE(n >= 0 && cap >= 0, "n and cap must be >= 0");
+    PXFF_REQUIRE(d_total != nullptr, "null total");
+    scan_kernel<int64_t><<<1, 1024, 0, (hipStream_t)stream>>>(n, d_in, d_out, cap, d_total);
     PXFF_LAUNCH_CHECK();
     return 0;
 }

@@ -258,7 +258,7 @@ __device__ inline void sweep_rows(int mb, const double *__restrict__ rpq, const 
 // image per round; per-group partial sums live in LDS and are reduced in a
 // fixed order at the end (deterministic).  With TANGENT the kernel instead
 // accumulates dU_tot . dr_e (forward-mode derivative of U_tot).
-template <int TWOJ, bool TANGENT, bool TILED, bool LDSADD>
+template <int TWOJ, bool TANGENT, bool TILED, int MODE>
 __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
                                                         const int64_t *__restrict__ img_j,
                                                         const double *__restrict__ img_r,
@@ -271,10 +271,14 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
     constexpr int G = TWOJ / 2 + 1;
     constexpr int NG = 64 / G;
     constexpr int NH = nhalf_of(TWOJ);
-    // LDSADD: all groups add into one LDS array (ds_add_f64; one wavefront
-    // per centre issues every add, so the order is fixed); otherwise each
-    // group keeps its own partial array, summed in group order at the end.
-    constexpr int NP = LDSADD ? 1 : NG;
+    // MODE 0: each group keeps its own partial array, summed in group order
+    // at the end.  MODE 1: all groups add into one LDS array (ds_add_f64;
+    // one wavefront per centre issues every add, so the order is fixed).
+    // MODE 2: groups g and g + NP share partial array g; the first half of
+    // the groups adds, then the second (two masked passes of one wavefront,
+    // in program order): half the LDS, so twice the resident wavefronts.
+    constexpr bool LDSADD = MODE == 1;
+    constexpr int NP = MODE == 1 ? 1 : (MODE == 2 ? (NG + 1) / 2 : NG);
     __shared__ cd part[NP][NH];
     __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
     __shared__ double lnorm[TWOJ + 1];
@@ -282,6 +286,12 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
     const int g = lane / G, mb = lane % G;
     for (int k = lane; k < (TWOJ + 1) * (TWOJ + 1); k += 64) rpq[k] = rootpq_g[k];
     for (int k = lane; k <= TWOJ; k += 64) lnorm[k] = lnorm_g[k];
+    if (TILED && blockIdx.x == 0 && (natoms & 63)) {
+        // padding lanes of the last tile (read by the Z kernel) are zero
+        cd *last = reinterpret_cast<cd *>(out) + (size_t)(natoms >> 6) * NH * 64;
+        for (int k = lane; k < NH * 64; k += 64)
+            if ((k & 63) >= (natoms & 63)) last[k] = cmake(0.0, 0.0);
+    }
 
     for (int64_t c = blockIdx.x; c < natoms; c += gridDim.x) {
         const int64_t i = work_order ? work_order[c] : c;
@@ -313,20 +323,28 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
             cayley_klein(x, y, z, r, rfac0 * M_PI * r / rcut, TANGENT, ck);
             if (!TANGENT) {
                 const double sc = fc * w;
-                cd *prow = part[(!LDSADD && g < NG) ? g : 0];
+                const int pidx = g < NG ? g % NP : 0;  // index the __shared__ array itself: LDS atomics
                 sweep_rows<TWOJ, false>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
-                    if (!valid || !half) return;
+                    const bool act = valid && half;
                     const int h0 = hoff_of(l) + mb * (l + 1);
 #pragma unroll
-                    for (int m = 0; m <= TWOJ; m++) {
-                        if (m > l) continue;
-                        if (LDSADD) {
-                            atomicAdd(&prow[h0 + m].re, sc * u[m].re);
-                            atomicAdd(&prow[h0 + m].im, sc * u[m].im);
-                        } else {
-                            cd v = prow[h0 + m];
-                            prow[h0 + m] = cmake(v.re + sc * u[m].re, v.im + sc * u[m].im);
+                    for (int pass = 0; pass < (MODE == 2 ? 2 : 1); pass++) {
+                        // MODE 2: the two groups sharing an array add in
+                        // separate passes, ordered by workgroup barriers
+                        if (act && (MODE != 2 || (g >= NP) == (pass == 1))) {
+#pragma unroll
+                            for (int m = 0; m <= TWOJ; m++) {
+                                if (m > l) continue;
+                                if (LDSADD) {
+                                    atomicAdd(&part[pidx][h0 + m].re, sc * u[m].re);
+                                    atomicAdd(&part[pidx][h0 + m].im, sc * u[m].im);
+                                } else {
+                                    cd v = part[pidx][h0 + m];
+                                    part[pidx][h0 + m] = cmake(v.re + sc * u[m].re, v.im + sc * u[m].im);
+                                }
+                            }
                         }
+                        if (MODE == 2) __syncthreads();
                     }
                 });
             } else {
@@ -338,22 +356,28 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
                 }
                 // D . dr = fc (dU . dr) + dfc (rhat . dr) U
                 const double gu = dfc * (x * d0 + y * d1 + z * d2) / r;
-                cd *prow = part[(!LDSADD && g < NG) ? g : 0];
+                const int pidx = g < NG ? g % NP : 0;  // index the __shared__ array itself: LDS atomics
                 sweep_rows<TWOJ, true>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
-                    if (!valid || !half) return;
+                    const bool act = valid && half;
                     const int h0 = hoff_of(l) + mb * (l + 1);
 #pragma unroll
-                    for (int m = 0; m <= TWOJ; m++) {
-                        if (m > l) continue;
-                        double re = fc * (du[0][m].re * d0 + du[1][m].re * d1 + du[2][m].re * d2) + gu * u[m].re;
-                        double im = fc * (du[0][m].im * d0 + du[1][m].im * d1 + du[2][m].im * d2) + gu * u[m].im;
-                        if (LDSADD) {
-                            atomicAdd(&prow[h0 + m].re, w * re);
-                            atomicAdd(&prow[h0 + m].im, w * im);
-                        } else {
-                            cd v = prow[h0 + m];
-                            prow[h0 + m] = cmake(v.re + w * re, v.im + w * im);
+                    for (int pass = 0; pass < (MODE == 2 ? 2 : 1); pass++) {
+                        if (act && (MODE != 2 || (g >= NP) == (pass == 1))) {
+#pragma unroll
+                            for (int m = 0; m <= TWOJ; m++) {
+                                if (m > l) continue;
+                                double re = fc * (du[0][m].re * d0 + du[1][m].re * d1 + du[2][m].re * d2) + gu * u[m].re;
+                                double im = fc * (du[0][m].im * d0 + du[1][m].im * d1 + du[2][m].im * d2) + gu * u[m].im;
+                                if (LDSADD) {
+                                    atomicAdd(&part[pidx][h0 + m].re, w * re);
+                                    atomicAdd(&part[pidx][h0 + m].im, w * im);
+                                } else {
+                                    cd v = part[pidx][h0 + m];
+                                    part[pidx][h0 + m] = cmake(v.re + w * re, v.im + w * im);
+                                }
+                            }
                         }
+                        if (MODE == 2) __syncthreads();
                     }
                 });
             }
@@ -859,7 +883,7 @@ namespace pxff {
 // Process-wide launch parameters set by the autotuner (pxff_tune_set).
 static int g_tune[PXFF_TUNE_COUNT] = {16, 16, 4, 0};
 static const int g_tune_lo[PXFF_TUNE_COUNT] = {1, 1, 1, 0};
-static const int g_tune_hi[PXFF_TUNE_COUNT] = {256, 256, 4, 1};
+static const int g_tune_hi[PXFF_TUNE_COUNT] = {256, 256, 4, 2};
 int tune_value(int knob) { return g_tune[knob]; }
 
 int so4_check_params(const pxff_so4_params *p) {
@@ -910,17 +934,17 @@ extern "C" int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64
     hipStream_t s = (hipStream_t)stream;
     const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * g_tune[PXFF_TUNE_ACCUM_WAVES_PER_CU]);
     const double fcut0 = cutoff_f0_host(p->cutoff_id);
-    if (tune_value(PXFF_TUNE_ACCUM_LDSADD)) {
-        PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, false, true, true><<<grid, 64, 0, s>>>(
-                                        natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_site_w, nullptr,
-                                        d_work_order, T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, fcut0,
-                                        d_utot)));
-    } else {
-        PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, false, true, false><<<grid, 64, 0, s>>>(
-                                        natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_site_w, nullptr,
-                                        d_work_order, T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, fcut0,
-                                        d_utot)));
+#define PXFF_ACCUM_UI(MODE)                                                                              \
+    PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, false, true, MODE><<<grid, 64, 0, s>>>(               \
+                                    natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_site_w, nullptr,       \
+                                    d_work_order, T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, fcut0, \
+                                    d_utot)))
+    switch (tune_value(PXFF_TUNE_ACCUM_LDSADD)) {
+    case 1: PXFF_ACCUM_UI(1); break;
+    case 2: PXFF_ACCUM_UI(2); break;
+    default: PXFF_ACCUM_UI(0); break;
     }
+#undef PXFF_ACCUM_UI
     PXFF_LAUNCH_CHECK();
     return 0;
 }
@@ -996,15 +1020,16 @@ extern "C" int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const 
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
     const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * g_tune[PXFF_TUNE_ACCUM_WAVES_PER_CU]);
-    if (tune_value(PXFF_TUNE_ACCUM_LDSADD)) {
-        PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, false, true><<<grid, 64, 0, s>>>(
-                                        natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr, d_dr, nullptr,
-                                        T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, 0.0, d_dutot)));
-    } else {
-        PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, false, false><<<grid, 64, 0, s>>>(
-                                        natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr, d_dr, nullptr,
-                                        T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, 0.0, d_dutot)));
+#define PXFF_ACCUM_TAN(MODE)                                                                      \
+    PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, false, MODE><<<grid, 64, 0, s>>>(        \
+                                    natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr, d_dr, nullptr, \
+                                    T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, 0.0, d_dutot)))
+    switch (tune_value(PXFF_TUNE_ACCUM_LDSADD)) {
+    case 1: PXFF_ACCUM_TAN(1); break;
+    case 2: PXFF_ACCUM_TAN(2); break;
+    default: PXFF_ACCUM_TAN(0); break;
     }
+#undef PXFF_ACCUM_TAN
     PXFF_LAUNCH_CHECK();
     const size_t lds = sizeof(double) * (2 * (size_t)T->nhalf + T->nz + T->nblk);
     const int gridb = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * 8);

@@ -85,7 +85,12 @@ class NeighborListBuilder:
         nb = max(self.nbins, 1)
         self._ws = dict(bin_count=torch.empty(nb, **i32), bin_start=torch.empty(nb + 1, **i32),
                         atom_bin=torch.empty(max(n, 1), **i32), wrap=torch.empty(max(n, 1) * 3, **i32),
-                        bin_atoms=torch.empty(max(n, 1), **i32), img_count=torch.empty(max(n, 1), **i64))
+                        bin_atoms=torch.empty(max(n, 1), **i32))
+
+        # image capacity of the next build: the arrays are sized before the
+        # image count is known on the host (no synchronisation in build);
+        # it grows when a build overflows it
+        self.img_capacity = max(1024, 48 * n)
 
     def build(self, pos):
         """A DeviceNeighborList of float64 device positions pos [natoms, 3]."""
@@ -129,23 +134,49 @@ class DeviceNeighborList:
         self._build(b.nbins, f64, i64, i32)
 
     def _build(self, nbins, f64, i64, i32):
+        # per-build counts (a later build must not overwrite this list's)
+        self._img_count = torch.empty(max(self.natoms, 1), **i64)
+        self._count()
+        self.generation = 0
+        self._fill(self.builder.img_capacity)
+
+    def _count(self):
+        """Bin the atoms (into the builder's shared workspace) and count the
+        images of every centre."""
+        ws = self.builder._ws
+        p = nat.ptr
+        nat.check(nat.lib().pxff_nlist_count(
+            p(self.pos), self.natoms, p(self.atom_struct), p(self.atom_start), p(self.cells), p(self.inv_cells),
+            p(self.pbc), p(self.geom), max(self.builder.nbins, 1), self.rc, p(ws["bin_count"]), p(ws["bin_start"]),
+            p(ws["atom_bin"]), p(ws["wrap"]), p(ws["bin_atoms"]), p(self._img_count), nat.stream()),
+            "pxff_nlist_count")
+
+    def _fill(self, cap):
+        """Scan (clamped to cap), fill and reverse phases into arrays of cap
+        images.  The unclamped total is copied to pinned host memory
+        asynchronously; ``nimg`` waits for it."""
         n = self.natoms
+        dev = self.pos.device
+        i64 = dict(dtype=torch.int64, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        f64 = dict(dtype=torch.float64, device=dev)
         s = nat.stream()
         L = nat.lib()
-        ws = self.builder._ws
-        bin_count, bin_start, atom_bin = ws["bin_count"], ws["bin_start"], ws["atom_bin"]
-        wrap, bin_atoms, img_count = ws["wrap"], ws["bin_atoms"], ws["img_count"]
         p = nat.ptr
-        nat.check(L.pxff_nlist_count(p(self.pos), n, p(self.atom_struct), p(self.atom_start), p(self.cells),
-                                     p(self.inv_cells), p(self.pbc), p(self.geom), max(nbins, 1), self.rc,
-                                     p(bin_count), p(bin_start), p(atom_bin), p(wrap), p(bin_atoms), p(img_count), s),
-                  "pxff_nlist_count")
+        ws = self.builder._ws
+        bin_start, atom_bin, wrap, bin_atoms = ws["bin_start"], ws["atom_bin"], ws["wrap"], ws["bin_atoms"]
+        self._cap = int(cap)
+        m = max(self._cap, 1)
         img_start = torch.empty(n + 1, **i64)
-        nat.check(L.pxff_scan_i64(p(img_count), n, p(img_start), s), "pxff_scan_i64")
-        nimg = int(img_start[n].item())  # the one host synchronisation: sizes the image arrays
-        self.nimg = nimg
+        total = torch.empty(1, **i64)
+        nat.check(L.pxff_scan_i64_capped(p(self._img_count), n, self._cap, p(img_start), p(total), s),
+                  "pxff_scan_i64_capped")
+        self._total_host = torch.empty(1, dtype=torch.int64, pin_memory=True)
+        self._total_host.copy_(total, non_blocking=True)
+        self._total_ev = torch.cuda.Event()
+        self._total_ev.record()
+        self._nimg = None
         self.img_start = img_start
-        m = max(nimg, 1)
         self.img_atom = torch.empty(m, **i64)
         self.img_j = torch.empty(m, **i64)
         self.img_T = torch.empty(m * 3, **i32)
@@ -156,10 +187,35 @@ class DeviceNeighborList:
                                     p(bin_atoms), p(img_start), p(self.img_atom), p(self.img_j), p(self.img_T),
                                     p(self.img_r), p(self._row_count), s), "pxff_nlist_fill")
         self.img_rev = torch.empty(m, **i64)
-        self._flag = torch.zeros(2, **i32)
-        nat.check(L.pxff_nlist_rev(n, nimg, p(img_start), p(self.img_atom), p(self.img_j), p(self.img_T),
+        self._flag = torch.empty(2, **i32)
+        nat.check(L.pxff_nlist_rev(n, self._cap, p(img_start), p(self.img_atom), p(self.img_j), p(self.img_T),
                                    p(self.img_rev), p(self._flag), s), "pxff_nlist_rev")
         self._rows_built = False
+        self._shift = None
+
+    @property
+    def nimg(self):
+        """Number of images (waits for the count copied back by the build).
+        If the build's image capacity was too small, the scan / fill /
+        reverse phases are redone with a larger one first, and
+        ``generation`` increases: work launched on this list before reading
+        nimg used a cut list and must be redone."""
+        if self._nimg is None:
+            self._total_ev.synchronize()
+            total = int(self._total_host[0])
+            if total > self._cap:
+                b = self.builder
+                b.img_capacity = max(b.img_capacity, total + total // 4 + 64)
+                self._count()  # the shared bins may hold a later build's atoms
+                self._fill(b.img_capacity)
+                self.generation += 1
+                self._total_ev.synchronize()
+            self._nimg = total
+        return self._nimg
+
+    @property
+    def img_capacity(self):
+        return self._cap
 
     def verify(self):
         """Raise if the reverse-image / reverse-row search failed (reads a

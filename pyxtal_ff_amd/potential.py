@@ -181,9 +181,15 @@ def energy_forces_fused(nl, state, model, species_atom, virial=False):
     Same quantities as ``energy_forces_adjoint``; the network parameters are
     constants here (energy + force evaluation, e.g. MD or the calculator)."""
     cfg = state.cfg
+    gen = nl.generation
     b, z, _ = dev_so4.forward(nl, cfg, state.atom_w, state.site_w, state.work_order)
     e_atom, beta = model.energy_grad(b, species_atom)
     y = dev_so4.yi(nl, cfg, z, beta)
+    # the first use of the host-side image count (the build did not wait
+    # for it); if the list had to be rebuilt, everything above used a cut list
+    nl.nimg
+    if nl.generation != gen:
+        return energy_forces_fused(nl, state, model, species_atom, virial)
     dedr = dev_so4.deidrj(nl, cfg, state.atom_w, y)
     out = {"energy": e_atom.sum(), "atom_energies": e_atom}
     if virial:

@@ -50,7 +50,8 @@ def ui(nl, cfg, atom_w, site_w, work_order=None, img_r=None):
     pairs), padding lanes zero."""
     p = nat.ptr
     ntiles = (nl.natoms + 63) // 64
-    out = torch.zeros(max(ntiles * 64 * cfg.nhalf * 2, 1), dtype=torch.float64, device=nl.pos.device)
+    # every element, padding lanes of the last tile included, is written
+    out = torch.empty(max(ntiles * 64 * cfg.nhalf * 2, 1), dtype=torch.float64, device=nl.pos.device)
     r = nl.img_r if img_r is None else img_r.contiguous()
     nat.check(nat.lib().pxff_so4_ui(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r), p(atom_w),
                                     p(site_w), p(work_order), p(out), nat.stream()), "pxff_so4_ui")
