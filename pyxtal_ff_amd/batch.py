@@ -102,12 +102,18 @@ class BatchedSNAP:
     """SNAP bispectrum (SNAP.py conventions) of a StructureBatch."""
 
     def __init__(self, batch, weights, lmax=4, rcut=4.9, rfac0=0.99363, cutoff="cosine", normalize_U=False,
-                 store_z=True):
+                 store_z=True, site_weights=None):
+        """weights: per-element neighbour weight (SNAP.py), or None for the
+        atomic numbers (SO4.py); site_weights: per-element self weight, 1.0
+        by default (SNAP.py), or "Z" for the atomic numbers (SO4.py)."""
         self.batch = batch
         self.cfg = dev_so4.So4Config(int(round(2 * lmax)), rcut, rfac0, cutoff, normalize_U)
         dev = batch.device
-        self.atom_w = torch.as_tensor([weights[s] for s in batch.symbols], dtype=torch.float64, device=dev)
-        self.site_w = torch.ones(batch.natoms, dtype=torch.float64, device=dev)
+        z = np.asarray(batch.numbers, dtype=np.float64)
+        aw = z if weights is None else np.array([weights[s] for s in batch.symbols], dtype=np.float64)
+        sw = z if site_weights == "Z" else np.ones(batch.natoms)
+        self.atom_w = torch.as_tensor(aw, dtype=torch.float64, device=dev)
+        self.site_w = torch.as_tensor(sw, dtype=torch.float64, device=dev)
         self.state = adjoint.SO4State(batch.nl, self.cfg, self.atom_w, self.site_w, store_z=store_z,
                                       work_order=batch.work_order)
 
@@ -154,7 +160,7 @@ def energy_forces_virial(desc, model, species, virial=True, create_graph=False):
     e_atom = model.atom_energies(x, species)
     e_struct = b.per_structure(e_atom)
     (dedr,) = torch.autograd.grad(e_struct.sum(), [r], create_graph=create_graph)
-    out = {"energy": e_struct, "x": x}
+    out = {"energy": e_struct, "x": x, "atom_energies": e_atom.detach()}
     if create_graph:
         out["forces"] = -adjoint._Gather.apply(dedr, nl)
         if virial:

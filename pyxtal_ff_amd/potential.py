@@ -35,17 +35,17 @@ class ElementMLP(torch.nn.Module):
     """nn.Linear layers, one activation after each hidden layer, a linear
     output layer of width 1 (neuralnetwork.py:186-200)."""
 
-    def __init__(self, nin, hidden=(30, 30), seed=0, activation="Tanh"):
+    def __init__(self, nin, hidden=(30, 30), seed=0, activation="Tanh", out_activation="Linear"):
         super().__init__()
         g = torch.Generator().manual_seed(int(seed))
         dims = [nin] + list(hidden) + [1]
         acts = [activation] * len(hidden) if isinstance(activation, str) else list(activation)
         if len(acts) != len(hidden):
             raise ValueError("one activation per hidden layer")
-        for a in acts:
+        for a in acts + [out_activation]:
             if a not in ACTIVATIONS:
                 raise NotImplementedError(f"activation {a} (available: {sorted(ACTIVATIONS)})")
-        self.acts = acts + ["Linear"]
+        self.acts = acts + [out_activation]
         self.layers = torch.nn.ModuleList()
         for a, b in zip(dims[:-1], dims[1:]):
             lin = torch.nn.Linear(a, b, dtype=torch.float64)
@@ -65,22 +65,25 @@ class NNPotential(torch.nn.Module):
     """Per-element MLPs; atoms of element e use models[e] on
     (x - dmin_e) / (dmax_e - dmin_e)."""
 
-    def __init__(self, elements, nin, hidden=(30, 30), seed=0, drange=None, activation="Tanh"):
+    def __init__(self, elements, nin, hidden=(30, 30), seed=0, drange=None, activation="Tanh",
+                 out_activation="Linear"):
         super().__init__()
         self.elements = list(elements)
         self.nin = int(nin)
         self.models = torch.nn.ModuleDict(
-            {e: ElementMLP(nin, hidden, seed + 17 * k, activation) for k, e in enumerate(self.elements)})
+            {e: ElementMLP(nin, hidden, seed + 17 * k, activation, out_activation)
+             for k, e in enumerate(self.elements)})
         self._packed = None
         for e in self.elements:
             lo = torch.zeros(nin, dtype=torch.float64)
-            sc = torch.ones(nin, dtype=torch.float64)
+            rng = torch.ones(nin, dtype=torch.float64)
             if drange is not None and e in drange:
                 d = np.asarray(drange[e], dtype=np.float64)
                 lo = torch.as_tensor(d[:, 0])
-                sc = torch.as_tensor(1.0 / (d[:, 1] - d[:, 0]))
+                rng = torch.as_tensor(d[:, 1] - d[:, 0])
             self.register_buffer("dmin_" + e, lo)
-            self.register_buffer("scale_" + e, sc)
+            self.register_buffer("range_" + e, rng)  # dmax - dmin (Scale1 of the LAMMPS export)
+            self.register_buffer("scale_" + e, 1.0 / rng)
 
     def atom_energies(self, x, species):
         """x [N, nb]; species: list of (element, index tensor)."""
