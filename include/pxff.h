@@ -260,10 +260,13 @@ typedef struct {
 } pxff_so3_params;
 
 /* Per-centre expansion ctot [natoms*csz*2] (sum over the centre's images);
- * with d_dr [nimg*3] non-NULL, its tangent sum_e dc_e/dr_e . dr_e. */
+ * with d_dr [nimg*3] non-NULL, its tangent sum_e dc_e/dr_e . dr_e.
+ * d_radial (plain expansion only; may be NULL): the radial integrals of
+ * every image, [nimg][lmax+1][2][nmax] (I_nl, dI_nl/dR), for the adjoint
+ * and plain per-image kernels to reuse. */
 int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_j,
                   const double *d_img_r, const int64_t *d_atom_z, const double *d_G, const double *d_rq,
-                  const double *d_lnorm, const double *d_dr, double *d_out, void *stream);
+                  const double *d_lnorm, const double *d_dr, double *d_out, double *d_radial, void *stream);
 
 /* Power spectrum x [natoms*ncoef]; with d_dctot non-NULL the tangent of x. */
 int pxff_so3_power(const pxff_so3_params *p, int64_t natoms, const double *d_ctot, const double *d_dctot,
@@ -273,16 +276,19 @@ int pxff_so3_power(const pxff_so3_params *p, int64_t natoms, const double *d_cto
 int pxff_so3_adjoint_a(const pxff_so3_params *p, int64_t natoms, const double *d_ctot, const double *d_beta,
                        double *d_a, void *stream);
 
-/* Adjoint: dE/dr per image [nimg*3] (expansion recomputed per image). */
+/* Adjoint: dE/dr per image [nimg*3] (angular part and expansion recomputed
+ * per image; radial integrals from d_radial when not NULL, else recomputed). */
 int pxff_so3_deidrj(const pxff_so3_params *p, int64_t nimg, const int64_t *d_img_atom, const int64_t *d_img_j,
                     const double *d_img_r, const int64_t *d_atom_z, const double *d_G, const double *d_rq,
-                    const double *d_lnorm, const double *d_a, double *d_dedr, void *stream);
+                    const double *d_lnorm, const double *d_a, const double *d_radial, double *d_dedr,
+                    void *stream);
 
 /* Plain path: dP/dr per image [nimg*ncoef*3] (then pxff_so4_rows builds the
  * reference's dxdr / rdxdr rows, whose row rules SO3.py:243-265 shares). */
 int pxff_so3_dpdr(const pxff_so3_params *p, int64_t nimg, const int64_t *d_img_atom, const int64_t *d_img_j,
                   const double *d_img_r, const int64_t *d_atom_z, const double *d_G, const double *d_rq,
-                  const double *d_lnorm, const double *d_ctot, double *d_dpdr, void *stream);
+                  const double *d_lnorm, const double *d_ctot, const double *d_radial, double *d_dpdr,
+                  void *stream);
 
 #ifdef __cplusplus
 }

@@ -21,36 +21,50 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
+#include <set>
+#include <vector>
 
 #include "pxff_common.h"
 
 namespace pxff {
 
 constexpr int SO3_LMAXC = 32;  // SO3.py:95-101 allows lmax <= 32
+constexpr int SO3_KMAX = 400;  // series terms
+
+// Reciprocals of the series: g_so3_rk[n][k] = 1 / (k (2n + 2k + 1)) and
+// g_so3_rodd[k] = 1 / (2k + 1), filled once per device (so3_series_init),
+// so the series below multiplies instead of dividing.
+__device__ double g_so3_rk[(SO3_LMAXC + 2) * SO3_KMAX];
+__device__ double g_so3_rodd[SO3_LMAXC + 3];
 
 // i_l(x) and i_{l-1}(x) (i_1 when l == 0: needed for i_0' = i_1).
 __device__ inline void sph_bessel_i_pair(int l, double x, double &il, double &ilm1) {
     const double thr = 16.0 + 0.5 * l * (l + 1);
     if (x < thr) {
         // i_n(x) = x^n / (2n+1)!! sum_k (x^2/2)^k / (k! (2n+3)(2n+5)...(2n+2k+1))
+        // both series in one loop (two independent product chains), four
+        // terms per convergence test; stopping when both last terms are
+        // below 1e-17 of their sums
         const int n2 = (l == 0) ? 1 : l - 1;
-        double v[2];
-        const int ns[2] = {l, n2};
         const double h = 0.5 * x * x;
-        for (int s = 0; s < 2; s++) {
-            const int n = ns[s];
-            double pre = 1.0;
-            for (int k = 1; k <= n; k++) pre *= x / (2 * k + 1);
-            double t = 1.0, sum = 1.0;
-            for (int k = 1; k < 400; k++) {
-                t *= h / (k * (2.0 * n + 2 * k + 1));
-                sum += t;
-                if (t < 1e-17 * sum) break;
+        double pre1 = 1.0, pre2 = 1.0;
+        for (int k = 1; k <= l; k++) pre1 *= x * g_so3_rodd[k];
+        for (int k = 1; k <= n2; k++) pre2 *= x * g_so3_rodd[k];
+        const double *rk1 = g_so3_rk + l * SO3_KMAX, *rk2 = g_so3_rk + n2 * SO3_KMAX;
+        double t1 = 1.0, s1 = 1.0, t2 = 1.0, s2 = 1.0;
+        for (int k = 1; k + 3 < SO3_KMAX; k += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                t1 *= h * rk1[k + u];
+                s1 += t1;
+                t2 *= h * rk2[k + u];
+                s2 += t2;
             }
-            v[s] = pre * sum;
+            if (t1 < 1e-17 * s1 && t2 < 1e-17 * s2) break;
         }
-        il = v[0];
-        ilm1 = v[1];
+        il = pre1 * s1;
+        ilm1 = pre2 * s2;
         return;
     }
     const double e = exp(x), ei = 1.0 / e;
@@ -170,7 +184,8 @@ struct So3Args {
 
 // Radial part of lane (image, l): I[n], dI[n] for n < NMAX.
 template <int NMAX>
-__device__ inline void radial(const So3Args &a, int l, double R, const double *Gs, const double *rqs, double *I,
+__device__ inline void radial(const So3Args &a, int l, double R, const double *__restrict__ Gs,
+                              const double *__restrict__ rqs, double *I,
                               double *dI, bool want_d) {
 #pragma unroll
     for (int n = 0; n < NMAX; n++) {
@@ -289,6 +304,35 @@ __device__ inline void image_factors(const So3Args &a, const So3Img &m, int l, i
     }
 }
 
+// ------------------------------------------------------------ radial
+// rad[e][l][0..NMAX) = I_nl(R_e), rad[e][l][NMAX..2NMAX) = dI_nl/dR: one
+// thread per (image, l), no shared memory (the quadrature tables are read
+// through wave-uniform addresses), so the quadrature -- the bulk of the
+// expansion's work -- runs at full occupancy.
+// Grid-stride over (image, l); the image count is read from the list
+// (img_start[natoms]), so the launch needs no host-side count.
+template <int NMAX>
+__global__ __launch_bounds__(256) void so3_radial_kernel(So3Args a, int64_t natoms,
+                                                         const int64_t *__restrict__ img_start,
+                                                         const double *__restrict__ img_r, double *__restrict__ rad) {
+    const int L = a.lmax + 1;
+    const int64_t total = img_start[natoms] * L;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = t / L;
+        const int l = (int)(t - e * L);
+        const double x = img_r[3 * e], y = img_r[3 * e + 1], z = img_r[3 * e + 2];
+        const double R = sqrt(x * x + y * y + z * z);
+        double I[NMAX], dI[NMAX];
+        radial<NMAX>(a, l, R, a.G, a.rq, I, dI, true);
+        double *ro = rad + (size_t)t * 2 * NMAX;
+#pragma unroll
+        for (int n = 0; n < NMAX; n++) {
+            ro[n] = I[n];
+            ro[NMAX + n] = dI[n];
+        }
+    }
+}
+
 // ------------------------------------------------------------ accumulate
 // ctot[i][n][l][m] = sum_e c_e  (MODE 0)  or its tangent
 // sum_e sum_k dr_e,k dc_e/dr_k (MODE 1).  One workgroup (64 threads) per
@@ -301,7 +345,9 @@ __global__ __launch_bounds__(64) void so3_accum_kernel(So3Args a, int64_t natoms
                                                         const int64_t *__restrict__ img_j,
                                                         const double *__restrict__ img_r,
                                                         const int64_t *__restrict__ atom_z,
-                                                        const double *__restrict__ dr, double *__restrict__ out) {
+                                                        const double *__restrict__ dr, double *__restrict__ out,
+                                                        const double *__restrict__ rad_in) {
+    constexpr bool TAN = MODE == SO3_TANGENT;
     extern __shared__ double smem[];
     const int L = a.lmax + 1;
     const int LP = a.lmax + 1;
@@ -309,19 +355,18 @@ __global__ __launch_bounds__(64) void so3_accum_kernel(So3Args a, int64_t natoms
     const int M2 = 2 * a.lmax + 1;
     const int ysz = (a.lmax + 2) * (2 * LP + 1);
     const int csz = NMAX * L * M2;
-    double *Gs = smem;                           // [NMAX*nq]
-    double *rqs = Gs + NMAX * a.nq;              // [nq]
-    double *lnorm = rqs + a.nq;                  // [L]
-    double *Ir = lnorm + L;                      // [NG][NMAX][L]
-    double *dIr = Ir + NG * NMAX * L;            // [NG][NMAX][L]
-    cd *Yt = reinterpret_cast<cd *>(dIr + NG * NMAX * L + ((NMAX * a.nq + a.nq + L) & 1));  // [NG][ysz]
-    cd *X = Yt + NG * ysz;                       // [NG][L][M2]
-    cd *Z2 = X + NG * L * M2;                    // [NG][L][M2]
-    cd *acc = Z2 + NG * L * M2;                  // [csz]
+    // LDS: the radial quadrature tables stay in global memory (their
+    // addresses are wave-uniform); the tangent-only arrays exist only in
+    // the tangent instantiation
+    double *lnorm = smem;                                    // [L]
+    double *Ir = lnorm + L;                                  // [NG][NMAX][L]
+    double *dIr = Ir + NG * NMAX * L;                        // [NG][NMAX][L] (tangent)
+    cd *Yt = reinterpret_cast<cd *>(dIr + (TAN ? NG * NMAX * L : 0) + ((L + NG * NMAX * L * (TAN ? 2 : 1)) & 1));
+    cd *X = Yt + NG * ysz;                                   // [NG][L][M2]
+    cd *Z2 = X + NG * L * M2;                                // [NG][L][M2] (tangent)
+    cd *acc = Z2 + (TAN ? NG * L * M2 : 0);                  // [csz]
     const int lane = threadIdx.x;
     const int g = lane / L, l = lane % L;
-    for (int k = lane; k < NMAX * a.nq; k += 64) Gs[k] = a.G[k];
-    for (int k = lane; k < a.nq; k += 64) rqs[k] = a.rq[k];
     for (int k = lane; k < L; k += 64) lnorm[k] = a.lnorm[k];
     So3Args al = a;
     al.lnorm = lnorm;
@@ -336,18 +381,27 @@ __global__ __launch_bounds__(64) void so3_accum_kernel(So3Args a, int64_t natoms
             load_image(al, valid, e, c, img_j, img_r, atom_z, m);
             double I[NMAX], dI[NMAX];
             if (valid) {
-                radial<NMAX>(al, l, m.R, Gs, rqs, I, dI, MODE == SO3_TANGENT);
+                if (rad_in) {  // from so3_radial_kernel
+                    const double *ri = rad_in + ((size_t)e * L + l) * 2 * NMAX;
+#pragma unroll
+                    for (int n = 0; n < NMAX; n++) {
+                        I[n] = ri[n];
+                        dI[n] = ri[NMAX + n];
+                    }
+                } else {
+                    radial<NMAX>(al, l, m.R, a.G, a.rq, I, dI, TAN);
+                }
                 fill_ylm(m, l, L, a.lmax, Yt + g * ysz, LP);
 #pragma unroll
                 for (int n = 0; n < NMAX; n++) {
                     Ir[(g * NMAX + n) * L + l] = I[n];
-                    dIr[(g * NMAX + n) * L + l] = dI[n];
+                    if (TAN) dIr[(g * NMAX + n) * L + l] = dI[n];
                 }
             }
             __syncthreads();
             if (valid) {
                 double d[3] = {0, 0, 0};
-                if (MODE == SO3_TANGENT) {
+                if (TAN) {
                     d[0] = dr[3 * e];
                     d[1] = dr[3 * e + 1];
                     d[2] = dr[3 * e + 2];
@@ -357,9 +411,8 @@ __global__ __launch_bounds__(64) void so3_accum_kernel(So3Args a, int64_t natoms
                     cd P, U[3], V[3];
                     image_factors(al, m, l, mm, Y, LP, P, U, V);
                     const int o = (g * L + l) * M2 + a.lmax + mm;
-                    if (MODE == SO3_CTOT) {
+                    if (!TAN) {
                         X[o] = P;
-                        Z2[o] = cmake(0, 0);
                     } else {
                         X[o] = cadd(cadd(cscale(U[0], d[0]), cscale(U[1], d[1])), cscale(U[2], d[2]));
                         Z2[o] = cadd(cadd(cscale(V[0], d[0]), cscale(V[1], d[1])), cscale(V[2], d[2]));
@@ -376,7 +429,7 @@ __global__ __launch_bounds__(64) void so3_accum_kernel(So3Args a, int64_t natoms
                     const int o = (gg * L + ll) * M2 + mi;
                     const double iv = Ir[(gg * NMAX + n) * L + ll];
                     s = cadd(s, cscale(X[o], iv));
-                    if (MODE == SO3_TANGENT) s = cadd(s, cscale(Z2[o], dIr[(gg * NMAX + n) * L + ll]));
+                    if (TAN) s = cadd(s, cscale(Z2[o], dIr[(gg * NMAX + n) * L + ll]));
                 }
                 acc[k] = s;
             }
@@ -397,7 +450,8 @@ __global__ __launch_bounds__(64) void so3_image_kernel(So3Args a, int64_t nimg, 
                                                         const int64_t *__restrict__ img_j,
                                                         const double *__restrict__ img_r,
                                                         const int64_t *__restrict__ atom_z,
-                                                        const double *__restrict__ cin, double *__restrict__ out) {
+                                                        const double *__restrict__ cin, double *__restrict__ out,
+                                                        const double *__restrict__ rad_in) {
     extern __shared__ double smem[];
     const int L = a.lmax + 1;
     const int LP = a.lmax + 1;
@@ -405,15 +459,11 @@ __global__ __launch_bounds__(64) void so3_image_kernel(So3Args a, int64_t nimg, 
     const int M2 = 2 * a.lmax + 1;
     const int ysz = (a.lmax + 2) * (2 * LP + 1);
     const int csz = NMAX * L * M2;
-    double *Gs = smem;
-    double *rqs = Gs + NMAX * a.nq;
-    double *lnorm = rqs + a.nq;
+    double *lnorm = smem;
     double *red = lnorm + L;  // [64][3]
-    cd *Yt = reinterpret_cast<cd *>(red + 64 * 3 + ((NMAX * a.nq + a.nq + L) & 1));
+    cd *Yt = reinterpret_cast<cd *>(red + 64 * 3 + (L & 1));
     const int lane = threadIdx.x;
     const int g = lane / L, l = lane % L;
-    for (int k = lane; k < NMAX * a.nq; k += 64) Gs[k] = a.G[k];
-    for (int k = lane; k < a.nq; k += 64) rqs[k] = a.rq[k];
     for (int k = lane; k < L; k += 64) lnorm[k] = a.lnorm[k];
     So3Args al = a;
     al.lnorm = lnorm;
@@ -424,7 +474,16 @@ __global__ __launch_bounds__(64) void so3_image_kernel(So3Args a, int64_t nimg, 
     load_image(al, valid, e, valid ? img_atom[e] : 0, img_j, img_r, atom_z, m);
     double I[NMAX], dI[NMAX];
     if (valid) {
-        radial<NMAX>(al, l, m.R, Gs, rqs, I, dI, true);
+        if (rad_in) {  // radial integrals stored by the forward expansion
+            const double *ri = rad_in + ((size_t)e * L + l) * 2 * NMAX;
+#pragma unroll
+            for (int n = 0; n < NMAX; n++) {
+                I[n] = ri[n];
+                dI[n] = ri[NMAX + n];
+            }
+        } else {
+            radial<NMAX>(al, l, m.R, a.G, a.rq, I, dI, true);
+        }
         fill_ylm(m, l, L, a.lmax, Yt + g * ysz, LP);
     }
     __syncthreads();
@@ -554,6 +613,8 @@ __global__ void so3_adj_a_kernel(int64_t natoms, int nmax, int lmax, const doubl
 using namespace pxff;
 
 namespace {
+int so3_series_init();
+
 int so3_check(const pxff_so3_params *p) {
     PXFF_REQUIRE(p != nullptr, "null params");
     if (p->nmax < 1 || p->nmax > 11 || p->lmax < 0 || p->lmax > SO3_LMAXC) {
@@ -567,7 +628,7 @@ int so3_check(const pxff_so3_params *p) {
         set_error("SO3: nmax*(lmax+1)*(2lmax+1) > 4096 is not supported on the device path");
         return PXFF_ENOTIMPL;
     }
-    return 0;
+    return so3_series_init();
 }
 
 So3Args make_args(const pxff_so3_params *p, const double *G, const double *rq, const double *lnorm) {
@@ -585,19 +646,38 @@ So3Args make_args(const pxff_so3_params *p, const double *G, const double *rq, c
     return a;
 }
 
-size_t accum_lds(const So3Args &a, int nmax) {
+size_t accum_lds(const So3Args &a, int nmax, bool tangent) {
     const int L = a.lmax + 1, NG = 64 / L, M2 = 2 * a.lmax + 1, LP = L;
     const int ysz = (a.lmax + 2) * (2 * LP + 1);
-    size_t d = (size_t)nmax * a.nq + a.nq + L + 2 * (size_t)NG * nmax * L + 1;
-    d += 2 * ((size_t)NG * ysz + 2 * (size_t)NG * L * M2 + (size_t)nmax * L * M2);
+    size_t d = (size_t)L + (tangent ? 2 : 1) * (size_t)NG * nmax * L + 1;
+    d += 2 * ((size_t)NG * ysz + (tangent ? 2 : 1) * (size_t)NG * L * M2 + (size_t)nmax * L * M2);
     return d * sizeof(double);
 }
 
 size_t image_lds(const So3Args &a, int nmax) {
     const int L = a.lmax + 1, NG = 64 / L, LP = L;
     const int ysz = (a.lmax + 2) * (2 * LP + 1);
-    size_t d = (size_t)nmax * a.nq + a.nq + L + 64 * 3 + 1 + 2 * (size_t)NG * ysz;
+    (void)nmax;
+    size_t d = (size_t)L + 64 * 3 + 1 + 2 * (size_t)NG * ysz;
     return d * sizeof(double);
+}
+
+// Fill the series reciprocal tables of the current device (once).
+std::mutex g_series_mutex;
+std::set<int> g_series_done;
+int so3_series_init() {
+    int dev = 0;
+    PXFF_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_series_mutex);
+    if (g_series_done.count(dev)) return 0;
+    std::vector<double> rk((SO3_LMAXC + 2) * SO3_KMAX, 0.0), rodd(SO3_LMAXC + 3, 0.0);
+    for (int n = 0; n < SO3_LMAXC + 2; n++)
+        for (int k = 1; k < SO3_KMAX; k++) rk[n * SO3_KMAX + k] = 1.0 / (k * (2.0 * n + 2 * k + 1));
+    for (int k = 0; k < SO3_LMAXC + 3; k++) rodd[k] = 1.0 / (2 * k + 1);
+    PXFF_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_so3_rk), rk.data(), rk.size() * sizeof(double)));
+    PXFF_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_so3_rodd), rodd.data(), rodd.size() * sizeof(double)));
+    g_series_done.insert(dev);
+    return 0;
 }
 
 int g_cus = 0;
@@ -638,26 +718,32 @@ static int launch_set_lds(K kern, size_t lds) {
 extern "C" int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start,
                              const int64_t *d_img_j, const double *d_img_r, const int64_t *d_atom_z,
                              const double *d_G, const double *d_rq, const double *d_lnorm, const double *d_dr,
-                             double *d_out, void *stream) {
+                             double *d_out, double *d_radial, void *stream) {
     int rc = so3_check(p);
     if (rc) return rc;
     if (natoms <= 0) return 0;
+    PXFF_REQUIRE(!(d_dr && d_radial), "SO3: the radial integrals are stored by the plain expansion only");
     So3Args a = make_args(p, d_G, d_rq, d_lnorm);
     hipStream_t s = (hipStream_t)stream;
-    const size_t lds = accum_lds(a, p->nmax);
+    const size_t lds = accum_lds(a, p->nmax, d_dr != nullptr);
     PXFF_REQUIRE(lds <= 160 * 1024, "SO3: LDS need exceeds 160 KB");
     const int grid = (int)std::min<int64_t>(natoms, (int64_t)cus() * 8);
     if (d_dr) {
         SO3_DISPATCH_NMAX(p->nmax, {
             auto k = so3_accum_kernel<NM, SO3_TANGENT>;
             if ((rc = launch_set_lds(k, lds))) return rc;
-            k<<<grid, 64, lds, s>>>(a, natoms, d_img_start, d_img_j, d_img_r, d_atom_z, d_dr, d_out);
+            k<<<grid, 64, lds, s>>>(a, natoms, d_img_start, d_img_j, d_img_r, d_atom_z, d_dr, d_out, nullptr);
         });
     } else {
+        if (d_radial) {
+            SO3_DISPATCH_NMAX(p->nmax, (so3_radial_kernel<NM><<<cus() * 16, 256, 0, s>>>(a, natoms, d_img_start,
+                                                                                          d_img_r, d_radial)));
+            PXFF_LAUNCH_CHECK();
+        }
         SO3_DISPATCH_NMAX(p->nmax, {
             auto k = so3_accum_kernel<NM, SO3_CTOT>;
             if ((rc = launch_set_lds(k, lds))) return rc;
-            k<<<grid, 64, lds, s>>>(a, natoms, d_img_start, d_img_j, d_img_r, d_atom_z, nullptr, d_out);
+            k<<<grid, 64, lds, s>>>(a, natoms, d_img_start, d_img_j, d_img_r, d_atom_z, nullptr, d_out, d_radial);
         });
     }
     PXFF_LAUNCH_CHECK();
@@ -691,7 +777,7 @@ extern "C" int pxff_so3_adjoint_a(const pxff_so3_params *p, int64_t natoms, cons
 static int so3_image_launch(const pxff_so3_params *p, int mode, int64_t nimg, const int64_t *d_img_atom,
                             const int64_t *d_img_j, const double *d_img_r, const int64_t *d_atom_z, const double *d_G,
                             const double *d_rq, const double *d_lnorm, const double *d_cin, double *d_out,
-                            void *stream) {
+                            const double *d_radial, void *stream) {
     int rc = so3_check(p);
     if (rc) return rc;
     if (nimg <= 0) return 0;
@@ -705,13 +791,13 @@ static int so3_image_launch(const pxff_so3_params *p, int mode, int64_t nimg, co
         SO3_DISPATCH_NMAX(p->nmax, {
             auto k = so3_image_kernel<NM, SO3_ADJOINT>;
             if ((rc = launch_set_lds(k, lds))) return rc;
-            k<<<grid, 64, lds, s>>>(a, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_cin, d_out);
+            k<<<grid, 64, lds, s>>>(a, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_cin, d_out, d_radial);
         });
     } else {
         SO3_DISPATCH_NMAX(p->nmax, {
             auto k = so3_image_kernel<NM, SO3_PLAIN>;
             if ((rc = launch_set_lds(k, lds))) return rc;
-            k<<<grid, 64, lds, s>>>(a, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_cin, d_out);
+            k<<<grid, 64, lds, s>>>(a, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_cin, d_out, d_radial);
         });
     }
     PXFF_LAUNCH_CHECK();
@@ -721,15 +807,15 @@ static int so3_image_launch(const pxff_so3_params *p, int mode, int64_t nimg, co
 extern "C" int pxff_so3_deidrj(const pxff_so3_params *p, int64_t nimg, const int64_t *d_img_atom,
                                const int64_t *d_img_j, const double *d_img_r, const int64_t *d_atom_z,
                                const double *d_G, const double *d_rq, const double *d_lnorm, const double *d_a,
-                               double *d_dedr, void *stream) {
+                               const double *d_radial, double *d_dedr, void *stream) {
     return so3_image_launch(p, SO3_ADJOINT, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_G, d_rq, d_lnorm, d_a,
-                            d_dedr, stream);
+                            d_dedr, d_radial, stream);
 }
 
 extern "C" int pxff_so3_dpdr(const pxff_so3_params *p, int64_t nimg, const int64_t *d_img_atom,
                              const int64_t *d_img_j, const double *d_img_r, const int64_t *d_atom_z,
                              const double *d_G, const double *d_rq, const double *d_lnorm, const double *d_ctot,
-                             double *d_dpdr, void *stream) {
+                             const double *d_radial, double *d_dpdr, void *stream) {
     return so3_image_launch(p, SO3_PLAIN, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_G, d_rq, d_lnorm, d_ctot,
-                            d_dpdr, stream);
+                            d_dpdr, d_radial, stream);
 }

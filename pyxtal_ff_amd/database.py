@@ -29,9 +29,9 @@ def _plain_arrays(desc):
     """x, dxdr rows and per-atom Voigt rdxdr of a batch (device arrays)."""
     nl = desc.batch.nl
     if isinstance(desc, bt.BatchedSO3):
-        x, ct = dso3.forward(nl, desc.cfg, desc.state.atom_z)
+        x, ct, rad = dso3.forward(nl, desc.cfg, desc.state.atom_z, radial=True)
         nf = desc.cfg.ncoef
-        dx, rd = dso4.rows(nl, nf, dso3.dpdr(nl, desc.cfg, desc.state.atom_z, ct), stress=True)
+        dx, rd = dso4.rows(nl, nf, dso3.dpdr(nl, desc.cfg, desc.state.atom_z, ct, radial=rad), stress=True)
     else:
         x, z, _ = dso4.forward(nl, desc.cfg, desc.atom_w, desc.site_w)
         nf = desc.cfg.nb

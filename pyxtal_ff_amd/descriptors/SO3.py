@@ -166,13 +166,16 @@ class SO3:
         cfg = dev_so3.So3Config(self.nmax, self.lmax, self.rcut, self.alpha, self._cutoff_function, self.weight_on)
         d = nl.pos.device
         z = torch.as_tensor(numbers, dtype=torch.int64, device=d)
-        x, ct = dev_so3.forward(nl, cfg, z)
+        if self.derivative:
+            x, ct, rad = dev_so3.forward(nl, cfg, z, radial=True)
+        else:
+            x, ct = dev_so3.forward(nl, cfg, z)
         x = x.clone()
         if k < n:
             x[k:] = 0.0  # the reference leaves rows of atoms outside atom_ids at zero
         out = {"x": x}
         if self.derivative:
-            dp = dev_so3.dpdr(nl, cfg, z, ct)
+            dp = dev_so3.dpdr(nl, cfg, z, ct, radial=rad)
             dx, rd = dev_so4.rows(nl, cfg.ncoef, dp, stress=self.stress)
             nr = int(nl.row_start[k].item())
             out["dxdr"] = dx[:nr]

@@ -82,17 +82,20 @@ def _f64(n, dev):
     return torch.empty(max(int(n), 1), dtype=torch.float64, device=dev)
 
 
-def ctot(nl, cfg, atom_z, img_r=None, dr=None):
+def ctot(nl, cfg, atom_z, img_r=None, dr=None, radial=False):
     """Per-centre expansion [N, nmax, lmax+1, 2lmax+1] complex (as float64
-    pairs); with dr, its tangent along the per-image displacements."""
+    pairs); with dr, its tangent along the per-image displacements.  With
+    radial=True (plain expansion only) also the per-image radial integrals
+    [nimg, lmax+1, 2, nmax] for the backward: returns (ctot, radial)."""
     p = nat.ptr
     out = _f64(nl.natoms * cfg.csz * 2, nl.pos.device)
     r = nl.img_r if img_r is None else img_r.contiguous()
     d = None if dr is None else dr.contiguous()
+    rad = _f64(nl.img_capacity * (cfg.lmax + 1) * 2 * cfg.nmax, nl.pos.device) if radial else None
     nat.check(nat.lib().pxff_so3_ctot(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r), p(atom_z),
-                                      p(cfg.G), p(cfg.rq), p(cfg.lnorm), p(d), p(out), nat.stream()),
+                                      p(cfg.G), p(cfg.rq), p(cfg.lnorm), p(d), p(out), p(rad), nat.stream()),
               "pxff_so3_ctot")
-    return out
+    return (out, rad) if radial else out
 
 
 def power(nl, cfg, ct, dct=None):
@@ -102,7 +105,11 @@ def power(nl, cfg, ct, dct=None):
     return x[: nl.natoms * cfg.ncoef].view(nl.natoms, cfg.ncoef)
 
 
-def forward(nl, cfg, atom_z, img_r=None):
+def forward(nl, cfg, atom_z, img_r=None, radial=False):
+    """(x, ctot) or, with radial=True, (x, ctot, radial integrals)."""
+    if radial:
+        ct, rad = ctot(nl, cfg, atom_z, img_r, radial=True)
+        return power(nl, cfg, ct), ct, rad
     ct = ctot(nl, cfg, atom_z, img_r)
     return power(nl, cfg, ct), ct
 
@@ -116,63 +123,75 @@ def adjoint_a(nl, cfg, ct, beta):
     return a
 
 
-def deidrj(nl, cfg, atom_z, a, img_r=None):
+def deidrj(nl, cfg, atom_z, a, img_r=None, radial=None):
+    """dE/dr per image [nimg, 3]; radial: the forward's radial integrals (or
+    None to recompute them)."""
     p = nat.ptr
     out = _f64(nl.nimg * 3, nl.pos.device)
     r = nl.img_r if img_r is None else img_r.contiguous()
     nat.check(nat.lib().pxff_so3_deidrj(cfg.params, nl.nimg, p(nl.img_atom), p(nl.img_j), p(r), p(atom_z), p(cfg.G),
-                                        p(cfg.rq), p(cfg.lnorm), p(a), p(out), nat.stream()), "pxff_so3_deidrj")
+                                        p(cfg.rq), p(cfg.lnorm), p(a), p(radial), p(out), nat.stream()),
+              "pxff_so3_deidrj")
     return out[: nl.nimg * 3].view(nl.nimg, 3)
 
 
-def dpdr(nl, cfg, atom_z, ct, img_r=None):
+def dpdr(nl, cfg, atom_z, ct, img_r=None, radial=None):
     """Materialised dP/dr per image [nimg, ncoef, 3]."""
     p = nat.ptr
     out = _f64(nl.nimg * cfg.ncoef * 3, nl.pos.device)
     r = nl.img_r if img_r is None else img_r.contiguous()
     nat.check(nat.lib().pxff_so3_dpdr(cfg.params, nl.nimg, p(nl.img_atom), p(nl.img_j), p(r), p(atom_z), p(cfg.G),
-                                      p(cfg.rq), p(cfg.lnorm), p(ct), p(out), nat.stream()), "pxff_so3_dpdr")
+                                      p(cfg.rq), p(cfg.lnorm), p(ct), p(radial), p(out), nat.stream()),
+              "pxff_so3_dpdr")
     return out
 
 
 class SO3State:
-    def __init__(self, nl, cfg, atom_z):
+    """Neighbour list, config and atomic numbers; store_radial keeps the
+    per-image radial integrals (2 nmax (lmax+1) doubles per image) from the
+    forward for the backward instead of recomputing the quadrature."""
+
+    def __init__(self, nl, cfg, atom_z, store_radial=True):
         self.nl, self.cfg, self.atom_z = nl, cfg, atom_z
+        self.store_radial = store_radial
 
 
 class _PFromR(torch.autograd.Function):
     """x [N, ncoef] as a function of the separations r [nimg, 3]; backward =
-    adjoint (A = S(dE/dx) ctot, one expansion sweep per image), double
-    backward = tangent of x.  Per-image expansions are recomputed, never
-    stored."""
+    adjoint (A = S(dE/dx) ctot, one sweep of the angular part per image),
+    double backward = tangent of x.  Per-image expansions are recomputed,
+    never stored; the radial integrals are kept when store_radial."""
 
     @staticmethod
     def forward(ctx, r, state):
-        x, ct = forward(state.nl, state.cfg, state.atom_z, img_r=r)
-        ctx.state, ctx.ct = state, ct
+        if state.store_radial:
+            x, ct, rad = forward(state.nl, state.cfg, state.atom_z, img_r=r, radial=True)
+        else:
+            (x, ct), rad = forward(state.nl, state.cfg, state.atom_z, img_r=r), None
+        ctx.state, ctx.ct, ctx.rad = state, ct, rad
         ctx.save_for_backward(r)
         return x
 
     @staticmethod
     def backward(ctx, beta):
         (r,) = ctx.saved_tensors
-        return _SO3DEdR.apply(beta.contiguous(), r, ctx.ct, ctx.state), None
+        return _SO3DEdR.apply(beta.contiguous(), r, ctx.ct, ctx.state, ctx.rad), None
 
 
 class _SO3DEdR(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, beta, r, ct, state):
+    def forward(ctx, beta, r, ct, state, rad=None):
         a = adjoint_a(state.nl, state.cfg, ct, beta)
         ctx.state, ctx.ct = state, ct
         ctx.save_for_backward(r)
-        return deidrj(state.nl, state.cfg, state.atom_z, a, img_r=r)
+        return deidrj(state.nl, state.cfg, state.atom_z, a, img_r=r, radial=rad)
 
     @staticmethod
     def backward(ctx, g):
         (r,) = ctx.saved_tensors
         st = ctx.state
         dct = ctot(st.nl, st.cfg, st.atom_z, img_r=r, dr=g.contiguous())
-        return power(st.nl, st.cfg, ctx.ct, dct), None, None, None
+        return power(st.nl, st.cfg, ctx.ct, dct), None, None, None, None
 
 
 def power_spectrum(r, state):

@@ -34,8 +34,9 @@ def teacher_labels(desc, teacher, species, plain=True):
         dx, rd = dso4.rows(nl, desc.cfg.nb, dso4.dbdr(nl, desc.cfg, desc.atom_w, z), stress=True)
         nf = desc.cfg.nb
     elif plain:
-        x, ct = dso3.forward(nl, desc.cfg, desc.state.atom_z)
-        dx, rd = dso4.rows(nl, desc.cfg.ncoef, dso3.dpdr(nl, desc.cfg, desc.state.atom_z, ct), stress=True)
+        x, ct, rad = dso3.forward(nl, desc.cfg, desc.state.atom_z, radial=True)
+        dx, rd = dso4.rows(nl, desc.cfg.ncoef, dso3.dpdr(nl, desc.cfg, desc.state.atom_z, ct, radial=rad),
+                           stress=True)
         nf = desc.cfg.ncoef
     if plain:
         xx = x.detach().requires_grad_(True)
@@ -73,7 +74,8 @@ class ForceTrainer:
             _, self.z, _ = dso4.forward(nl, desc.cfg, desc.atom_w, desc.site_w, desc.batch.work_order)
         else:
             from . import so3 as dso3
-            _, self.ct = dso3.forward(nl, desc.cfg, desc.state.atom_z)
+            # fixed structures: the radial integrals are computed once too
+            _, self.ct, self.rad = dso3.forward(nl, desc.cfg, desc.state.atom_z, radial=True)
         self.n_struct = torch.as_tensor(b.counts, dtype=torch.float64, device=b.device)
         self._voigt = torch.as_tensor([3 * i + j for i, j in zip(*VOIGT)], dtype=torch.int64, device=b.device)
         # Adam keeps its step count on the device when the step is captured;
@@ -88,7 +90,7 @@ class ForceTrainer:
         if hasattr(self, "z"):
             return adjoint._DEdR.apply(beta, self.r, self.z, self.desc.state)
         from .so3 import _SO3DEdR
-        return _SO3DEdR.apply(beta, self.r, self.ct, self.desc.state)
+        return _SO3DEdR.apply(beta, self.r, self.ct, self.desc.state, self.rad)
 
     def loss(self):
         b = self.batch

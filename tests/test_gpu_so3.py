@@ -111,6 +111,20 @@ def test_so3_adjoint_matches_plain():
         assert rel_err(fa.cpu().numpy(), fp.cpu().numpy()) < 1e-9
 
 
+def test_so3_stored_radial_equals_recomputed():
+    """Adjoint forces with the forward's stored radial integrals equal those
+    with the integrals recomputed in the backward (same quadrature, same
+    operations: bitwise)."""
+    rng = np.random.default_rng(5)
+    s = workloads.random_sio_structure(rng, 20, 40)
+    nl, cfg, z, model, species, st = _so3_engine(s)
+    out = []
+    for store in (True, False):
+        st2 = dso3.SO3State(nl, cfg, z, store_radial=store)
+        out.append(_adjoint_forces(nl, st2, model, species)[1].cpu().numpy())
+    assert np.array_equal(out[0], out[1])
+
+
 def test_so3_double_backward_tangent_fd():
     s = workloads.diamond_si(reps=2, sigma=0.08, seed=3)
     nl, cfg, z, model, species, st = _so3_engine(s)
