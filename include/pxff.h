@@ -226,10 +226,11 @@ int pxff_segment_sum(int64_t nseg, int ncol, const int64_t *d_seg_start, const d
  * (neuralnetwork.py:80-200 architecture, :484-497 / :657-677 use,
  * :938-1029 drange scaling).  Atom i (element d_species[i] in [0, nelem))
  * uses x' = (x_i - dmin_e) * scale_e (d_dmin, d_scale [nelem*nin]) and the
- * layers of its element; layout per element (ints, stride 1 + 5*8):
- * nlayers (1..8), then per layer {nin, nout, act, w_off, b_off}: W
- * [nout][nin] row-major at d_params + w_off, bias at d_params + b_off, act
- * 0 Linear, 1 Tanh, 2 Sigmoid, 3 ReLU, 4 Softplus, 5 ELU.  Widths <= 64,
+ * layers of its element; layout per element (ints, stride 1 + 6*8):
+ * nlayers (1..8), then per layer {nin, nout, act, w_off, b_off, wt_off}: W
+ * [nout][nin] row-major at d_params + w_off, bias at d_params + b_off, the
+ * transpose W^T [nin][nout] at d_params + wt_off, act 0 Linear, 1 Tanh,
+ * 2 Sigmoid, 3 ReLU, 4 Softplus, 5 ELU.  Widths <= 64,
  * last width 1, nin <= 1024.  h_layout (host copy of d_layout) is
  * validated before launch.  Outputs: d_e [natoms] atomic energies, d_beta
  * [natoms*nin] = dE_i/dx_i.  An out-of-range species gives NaN. */

@@ -16,6 +16,8 @@ namespace pxff {
 constexpr int MLP_MAXW = 64;     // hidden / output width limit (one lane per neuron)
 constexpr int MLP_MAXL = 8;      // layers per element
 constexpr int MLP_MAXIN = 1024;  // descriptor length limit
+constexpr int MLP_LSTRIDE = 6;   // ints per layer in the layout
+constexpr int MLP_STRIDE = 1 + MLP_LSTRIDE * MLP_MAXL;
 
 __device__ inline void act_fdf(int id, double z, double &a, double &da) {
     switch (id) {
@@ -60,9 +62,12 @@ __device__ inline void act_fdf(int id, double z, double &a, double &da) {
     }
 }
 
-// layout per element e (ints, at d_layout[e * (1 + 5 * MLP_MAXL)]):
-//   nlayers, then per layer {nin, nout, act, w_off, b_off} (W row-major
-//   [nout][nin] as torch.nn.Linear stores it, offsets into d_params).
+// layout per element e (ints, at d_layout[e * MLP_STRIDE]):
+//   nlayers, then per layer {nin, nout, act, w_off, b_off, wt_off}: W
+//   row-major [nout][nin] as torch.nn.Linear stores it (read by the
+//   backward: lanes over inputs, consecutive addresses) and its transpose
+//   [nin][nout] (read by the forward: lanes over outputs), offsets into
+//   d_params.
 __global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int nin, int nelem,
                                                               const int64_t *__restrict__ species,
                                                                const int *__restrict__ layout,
@@ -86,7 +91,7 @@ __global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int
         return;  // whole workgroup (one wavefront)
     }
     const int el = (int)el64;
-    const int *lay = layout + el * (1 + 5 * MLP_MAXL);
+    const int *lay = layout + el * MLP_STRIDE;
     const int nl = lay[0];
     const double *xi = x + (size_t)i * nin;
     const double *dm = dmin + (size_t)el * nin, *sc = scale + (size_t)el * nin;
@@ -95,13 +100,13 @@ __global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int
     // forward
     const double *hin = h0;
     for (int k = 0; k < nl; k++) {
-        const int *L = lay + 1 + 5 * k;
+        const int *L = lay + 1 + MLP_LSTRIDE * k;
         const int ni = L[0], no = L[1], act = L[2];
-        const double *W = params + L[3], *b = params + L[4];
+        const double *WT = params + L[5], *b = params + L[4];
         if (lane < no) {
             double z = b[lane];
-            const double *wr = W + (size_t)lane * ni;
-            for (int j = 0; j < ni; j++) z += wr[j] * hin[j];
+#pragma unroll 8
+            for (int j = 0; j < ni; j++) z += WT[(size_t)j * no + lane] * hin[j];
             double a, da;
             act_fdf(act, z, a, da);
             hout[k * MLP_MAXW + lane] = a;
@@ -114,12 +119,13 @@ __global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int
     // backward: dact[k] becomes the pre-activation gradient of layer k; for
     // the output layer (one neuron, dE/dE = 1) it already is act'
     for (int k = nl - 1; k >= 1; k--) {
-        const int *L = lay + 1 + 5 * k;
+        const int *L = lay + 1 + MLP_LSTRIDE * k;
         const int ni = L[0], no = L[1];
         const double *W = params + L[3];
         // dE/dh_{k-1}[j] = sum_n W[n][j] g_pre[n]; then times act'_{k-1}
         if (lane < ni) {
             double g = 0.0;
+#pragma unroll 8
             for (int n = 0; n < no; n++) g += W[(size_t)n * ni + lane] * dact[k * MLP_MAXW + n];
             dact[(k - 1) * MLP_MAXW + lane] *= g;
         }
@@ -131,6 +137,7 @@ __global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int
         const double *W = params + L[3];
         for (int j = lane; j < ni; j += 64) {
             double g = 0.0;
+#pragma unroll 8
             for (int n = 0; n < no; n++) g += W[(size_t)n * ni + j] * dact[n];
             beta[(size_t)i * nin + j] = g * sc[j];
         }
@@ -148,12 +155,12 @@ extern "C" int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const in
     PXFF_REQUIRE(nin > 0 && nin <= MLP_MAXIN, "descriptor length must be in 1..1024");
     PXFF_REQUIRE(nelem > 0 && h_layout != nullptr, "need at least one element network");
     for (int e = 0; e < nelem; e++) {
-        const int *lay = h_layout + e * (1 + 5 * MLP_MAXL);
+        const int *lay = h_layout + e * MLP_STRIDE;
         const int nl = lay[0];
         PXFF_REQUIRE(nl >= 1 && nl <= MLP_MAXL, "1..8 layers per element network");
         int prev = nin;
         for (int k = 0; k < nl; k++) {
-            const int *L = lay + 1 + 5 * k;
+            const int *L = lay + 1 + MLP_LSTRIDE * k;
             PXFF_REQUIRE(L[0] == prev, "layer input width does not match the previous layer");
             PXFF_REQUIRE(L[1] >= 1 && L[1] <= MLP_MAXW, "layer widths must be in 1..64");
             PXFF_REQUIRE(L[2] >= 0 && L[2] <= 5, "unknown activation id");

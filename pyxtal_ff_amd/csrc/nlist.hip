@@ -309,7 +309,7 @@ __global__ void nl_row_rev_kernel(int64_t nrows, const int64_t *__restrict__ seq
 // fill pass ranks the list by (j, T) (rank = number of smaller keys) and
 // writes it in order.  Centres with more than NL_KMAX images are left to
 // the thread-per-centre kernel above (nl_fill_kernel with a count floor).
-constexpr int NL_KMAX = 512;
+constexpr int NL_KMAX = 128;  // 5 KB of LDS per centre: many resident wavefronts
 
 struct NlHit {
     int64_t j;

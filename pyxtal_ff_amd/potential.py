@@ -103,7 +103,7 @@ class NNPotential(torch.nn.Module):
         key = tuple((t.data_ptr(), t._version) for t in ts)
         if self._packed is not None and self._packed[0] == key:
             return self._packed[1]
-        stride = 1 + 5 * 8
+        stride = 1 + 6 * 8
         lay = np.zeros((len(self.elements), stride), dtype=np.int32)
         chunks, off = [], 0
         for ei, e in enumerate(self.elements):
@@ -114,10 +114,11 @@ class NNPotential(torch.nn.Module):
             for k, (lin, act) in enumerate(zip(m.layers, m.acts)):
                 w = lin.weight.detach().reshape(-1)
                 b = lin.bias.detach().reshape(-1)
-                lay[ei, 1 + 5 * k: 6 + 5 * k] = [lin.in_features, lin.out_features, ACTIVATIONS[act], off,
-                                                 off + w.numel()]
-                chunks += [w, b]
-                off += w.numel() + b.numel()
+                wt = lin.weight.detach().t().contiguous().reshape(-1)
+                lay[ei, 1 + 6 * k: 7 + 6 * k] = [lin.in_features, lin.out_features, ACTIVATIONS[act], off,
+                                                 off + w.numel(), off + w.numel() + b.numel()]
+                chunks += [w, b, wt]
+                off += 2 * w.numel() + b.numel()
         params = torch.cat(chunks).contiguous()
         dev = params.device
         dmin = torch.stack([getattr(self, "dmin_" + e) for e in self.elements]).to(dev).contiguous()
