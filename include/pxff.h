@@ -126,7 +126,8 @@ int pxff_scan_i64_capped(const int64_t *d_in, int64_t n, int64_t cap, int64_t *d
  * Z weights).                                                          *
  * Layout: per atom a "half" U/Y array of nhalf complex (levels l, rows *
  * mb <= l/2, columns ma), Z [nz] complex in the reference's idxz order, *
- * B [nb] real in the reference's idxb order.                            *
+ * B [nb] real in the reference's idxb order.  U_tot and Z are stored    *
+ * tile-major (64-atom tiles, atom index innermost).                     *
  * ------------------------------------------------------------------ */
 typedef struct {
     int twol;      /* 2J = round(2*lmax); even, 0..10 */
@@ -151,14 +152,21 @@ int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_s
                 const double *d_img_r, const double *d_atom_w, const double *d_site_w,
                 const int64_t *d_work_order, double *d_utot, void *stream);
 
-/* Z (and B) from the tile-major U_tot.  d_z [natoms*nz*2] out (atom-major);
- * d_b [natoms*nb] out (or NULL). */
+/* Z (and B) from the tile-major U_tot (compute_zi + compute_bi,
+ * SNAP.py:1050-1141).  d_z out, pxff_so4_zlen(2J, natoms) doubles: Z
+ * tile-major like U_tot (element e of atom i at [(i/64)*nz + e]*64 + i%64,
+ * complex; padding lanes of the last tile are zero), followed by the B row
+ * partials the Z kernel writes when d_b != NULL; d_b [natoms*nb] out (or
+ * NULL). */
 int pxff_so4_zb(const pxff_so4_params *p, int64_t natoms, const double *d_utot, double *d_z, double *d_b,
                 void *stream);
 
-/* B alone (compute_bi, SNAP.py:1101-1141) from the tile-major U_tot and Z:
- * d_b [natoms*nb] out.  pxff_so4_zb with d_b != NULL is pxff_so4_zb with
- * d_b == NULL followed by this call. */
+/* Length in doubles of the d_z buffer of pxff_so4_zb for natoms atoms. */
+int pxff_so4_zlen(int twol, int64_t natoms, int64_t *len);
+
+/* B alone (compute_bi, SNAP.py:1101-1141) from the tile-major U_tot and Z
+ * (the layouts pxff_so4_zb writes): d_b [natoms*nb] out.  Same values as
+ * pxff_so4_zb with d_b != NULL (which sums the rows in another order). */
 int pxff_so4_b(const pxff_so4_params *p, int64_t natoms, const double *d_utot, const double *d_z, double *d_b,
                void *stream);
 

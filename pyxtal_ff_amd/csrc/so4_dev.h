@@ -16,7 +16,6 @@ struct So4Dev {
     int *b_src;       // [nb*3] blocks targeting b (-1 = none)
     double *b_srcf;   // [nb*3] their factors
     int *zh;          // [nz] half-U element paired with each Z element
-    int *zblk;        // [nz] block (level-sorted numbering) of each Z element
     double *hscale;   // [nhalf] 2 hw(h) u_norm(l(h)): Y scale per half element
     double *rootpq, *lnorm;
     int *hoff, *h_level, *h_diag;
@@ -24,7 +23,17 @@ struct So4Dev {
     int *idxu_block;
     int nzgroups;
     int *zgroups;
+    int nbrow;        // B row partials per atom
+    int *b_brow0;     // [nb] first B row partial of b
+    int nytask;       // Y tasks: (level, run of half elements) per wavefront
+    int *ytask;       // [nytask * YTASK_STRIDE]
 };
+
+// A Y task is {l, loc0, count <= YTASK_ROWS, first block, end block, h0}: the
+// half elements h0 .. h0 + count - 1 (locations loc0.. of level l) summed
+// over the blocks [first, end) of level l.
+constexpr int YTASK_ROWS = 8;
+constexpr int YTASK_STRIDE = 6;
 
 int get_so4_tables(int twol, int norm, const So4Dev **out);
 int so4_check_params(const pxff_so4_params *p);

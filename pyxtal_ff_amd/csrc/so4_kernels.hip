@@ -1,8 +1,10 @@
 // SO(4)/SNAP bispectrum kernels for gfx950 (FP64, wave64).
 //
-// Device layout (HBM), per atom i:
-//   utot[i][nhalf]  complex  U_tot half rows (mb <= l/2), self term, u_norm
-//   z[i][nz]        complex  Z in the reference's idxz order
+// Device layout (HBM), per atom i (tile t = i/64, lane = i%64):
+//   utot[t][nhalf][lane] complex  U_tot half rows (mb <= l/2), self term,
+//                                 u_norm (tile-major; the Z, B and Y kernels
+//                                 run one atom per lane)
+//   z[t][nz][lane]  complex  Z in the reference's idxz order, tile-major
 //   b[i][nb]        real     B in the reference's idxb order
 //   y[i][nhalf]     complex  adjoint Y (dE/dB-weighted Z, half weights,
 //                            factor 2 and u_norm folded in)
@@ -19,6 +21,8 @@
 // (sum_l l recursion steps), so groups never diverge.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -84,6 +88,8 @@ int get_so4_tables(int twol, int norm, const So4Dev **out) {
     if ((rc = upload(t.h_diag, &d.h_diag))) return rc;
     if ((rc = upload(t.h_weight, &d.h_weight))) return rc;
     if ((rc = upload(t.idxu_block, &d.idxu_block))) return rc;
+    d.nbrow = t.nbrow;
+    if ((rc = upload(t.b_brow0, &d.b_brow0))) return rc;
     d.nzgroups = t.nzgroups;
     if ((rc = upload(t.zgroups, &d.zgroups))) return rc;
     // inverse of the block targets: the (block, factor) pairs of each b
@@ -106,21 +112,35 @@ int get_so4_tables(int twol, int norm, const So4Dev **out) {
     if ((rc = upload(bsrcf, &d.b_srcf))) return rc;
     // half-U element of each Z element: Z block (l1, l2, l) element loc pairs
     // with U^l element loc (same (mb, ma), mb <= l/2)
-    std::vector<int> zh((size_t)t.nz, 0), zblk((size_t)t.nz, 0);
+    std::vector<int> zh((size_t)t.nz, 0);
     for (int k = 0; k < t.nblk; k++) {
         int l = 0;
         while (l + 1 < t.ldim && k >= t.blk_level_start[l + 1]) l++;
         const int nloc = (l / 2 + 1) * (l + 1);
         for (int q = 0; q < nloc; q++) {
             zh[t.blk_zoff[k] + q] = t.hoff[l] + q;
-            zblk[t.blk_zoff[k] + q] = k;
         }
     }
     if ((rc = upload(zh, &d.zh))) return rc;
-    if ((rc = upload(zblk, &d.zblk))) return rc;
     std::vector<double> hscale((size_t)t.nhalf, 0.0);
     for (int h = 0; h < t.nhalf; h++) hscale[h] = 2.0 * t.h_weight[h] * t.lnorm[t.h_level[h]];
     if ((rc = upload(hscale, &d.hscale))) return rc;
+    // Y tasks, most blocks first (longest wavefronts start first)
+    std::vector<std::array<int, YTASK_STRIDE>> yt;
+    for (int l = 0; l < t.ldim; l++) {
+        const int nloc = (l / 2 + 1) * (l + 1);
+        for (int loc0 = 0; loc0 < nloc; loc0 += YTASK_ROWS)
+            yt.push_back({l, loc0, std::min(YTASK_ROWS, nloc - loc0), t.blk_level_start[l],
+                          t.blk_level_start[l + 1], t.hoff[l] + loc0});
+    }
+    std::stable_sort(yt.begin(), yt.end(), [](const std::array<int, YTASK_STRIDE> &a,
+                                              const std::array<int, YTASK_STRIDE> &b) {
+        return (a[4] - a[3]) * a[2] > (b[4] - b[3]) * b[2];
+    });
+    std::vector<int> ytf;
+    for (auto &q : yt) ytf.insert(ytf.end(), q.begin(), q.end());
+    d.nytask = (int)yt.size();
+    if ((rc = upload(ytf, &d.ytask))) return rc;
     g_tabs[key] = d;
     *out = &g_tabs[key];
     return 0;
@@ -417,7 +437,7 @@ __global__ __launch_bounds__(256) void so4_b_kernel(int64_t natoms, const double
     const int64_t i = tile * 64 + lane;
     if (i >= natoms) return;
     const cd *ut = reinterpret_cast<const cd *>(utile) + (size_t)tile * NH * 64 + lane;
-    const cd *zi = reinterpret_cast<const cd *>(z) + (size_t)i * nz;
+    const cd *zi = reinterpret_cast<const cd *>(z) + (size_t)tile * nz * 64 + lane;  // Z tile-major
     for (int b = w; b < nb; b += nw) {
         const int l = __builtin_amdgcn_readfirstlane(b_level[b]);
         const int z0 = __builtin_amdgcn_readfirstlane(b_zoff[b]);
@@ -427,12 +447,13 @@ __global__ __launch_bounds__(256) void so4_b_kernel(int64_t natoms, const double
         const int nfull = ((l + 1) / 2) * (l + 1);
         double s = 0.0;
 #pragma unroll 8
-        for (int q = 0; q < nfull; q++) s += cdotre(ut[(size_t)(h0 + q) * 64], zi[z0 + q]);
+        for (int q = 0; q < nfull; q++) s += cdotre(ut[(size_t)(h0 + q) * 64], zi[(size_t)(z0 + q) * 64]);
         if ((l & 1) == 0) {
             const int mb = l / 2;
 #pragma unroll 4
-            for (int q = nfull; q < nfull + mb; q++) s += cdotre(ut[(size_t)(h0 + q) * 64], zi[z0 + q]);
-            s += 0.5 * cdotre(ut[(size_t)(h0 + nfull + mb) * 64], zi[z0 + nfull + mb]);
+            for (int q = nfull; q < nfull + mb; q++)
+                s += cdotre(ut[(size_t)(h0 + q) * 64], zi[(size_t)(z0 + q) * 64]);
+            s += 0.5 * cdotre(ut[(size_t)(h0 + nfull + mb) * 64], zi[(size_t)(z0 + nfull + mb) * 64]);
         }
         bout[(size_t)i * nb + b] = 2.0 * s;
     }
@@ -440,56 +461,78 @@ __global__ __launch_bounds__(256) void so4_b_kernel(int64_t natoms, const double
 
 // ---------------------------------------------------------------------- Y
 // Y[h] = 2 hw(h) norm_l sum_{blocks k at level l} c_k Z[k][loc(h)],
-// c_k = sum_targets f beta[b].  One wavefront per atom (4 per workgroup):
-// the block coefficients c_k go to LDS, then the atom's Z row is streamed
-// once in order (coalesced 1 KB loads) and each element is added into its
-// half element's LDS accumulator (ds_add_f64; one wavefront issues all adds
-// of an atom, so their order is fixed), and the accumulators are scaled and
-// stored.
-__global__ __launch_bounds__(256) void so4_yi_kernel(int64_t natoms, int nhalf, int nz, int nb, int nblk,
-                                                      const double *__restrict__ z, const double *__restrict__ beta,
-                                                      const int *__restrict__ zblk, const int *__restrict__ zh,
-                                                      const double *__restrict__ hscale,
-                                                      const int *__restrict__ blk_tb,
-                                                      const double *__restrict__ blk_tf, double *__restrict__ y) {
-    extern __shared__ double sm[];  // per wavefront: c [nblk], Y accumulators [2 nhalf]
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t i = (int64_t)blockIdx.x * 4 + wv;
-    const bool valid = i < natoms;
-    double *c = sm + (size_t)wv * (nblk + 2 * nhalf);
-    double *ya = c + nblk;
-    if (valid) {
-        const double *bi = beta + (size_t)i * nb;
-        for (int k = lane; k < nblk; k += 64) {
-            double v = 0.0;
+// c_k = sum_targets f beta[b].  One atom per lane over a 64-atom tile, like
+// the Z kernel whose tile-major output it reads: a wavefront owns one task
+// (a run of at most YTASK_ROWS half elements of one level), loops over the
+// level's blocks in order (fixed summation order, no atomics) and adds
+// c_k Z[k][loc] into register accumulators; every Z load is a coalesced
+// 1 KB row and every Z element is read by exactly one wavefront.  The four
+// wavefronts of a workgroup take four tasks of the same tile and share its
+// dE/dB rows, staged in LDS.
+__global__ __launch_bounds__(256) void so4_yi_kernel(int64_t natoms, int ntasks, int nz, int nb, int nhalf,
+                                                      const int *__restrict__ ytask, const double *__restrict__ z,
+                                                      const double *__restrict__ beta,
+                                                      const int *__restrict__ blk_zoff, const int *__restrict__ blk_tb,
+                                                      const double *__restrict__ blk_tf,
+                                                      const double *__restrict__ hscale, double *__restrict__ y) {
+    extern __shared__ double sb[];  // [64][nb] dE/dB of the tile (zero beyond natoms)
+    const int tg = (ntasks + 3) / 4;
+    const int64_t tile = blockIdx.x / tg;
+    const int task = (int)(blockIdx.x % tg) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t a0 = tile * 64;
+    const int64_t nv = (natoms - a0 < 64 ? natoms - a0 : 64) * nb;
+    for (int64_t k = threadIdx.x; k < 64 * (int64_t)nb; k += blockDim.x) sb[k] = k < nv ? beta[a0 * nb + k] : 0.0;
+    __syncthreads();
+    if (task >= ntasks) return;
+    const int *T = ytask + YTASK_STRIDE * task;
+    const int loc0 = __builtin_amdgcn_readfirstlane(T[1]);
+    const int cnt = __builtin_amdgcn_readfirstlane(T[2]);
+    const int k0 = __builtin_amdgcn_readfirstlane(T[3]);
+    const int k1 = __builtin_amdgcn_readfirstlane(T[4]);
+    const int h0 = __builtin_amdgcn_readfirstlane(T[5]);
+    const double *bl = sb + lane * nb;
+    const cd *zt = reinterpret_cast<const cd *>(z) + (size_t)tile * nz * 64 + lane;
+    cd acc[YTASK_ROWS];
 #pragma unroll
-            for (int t = 0; t < 3; t++) {
-                const int b = blk_tb[3 * k + t];
-                if (b >= 0) v += blk_tf[3 * k + t] * bi[b];
+    for (int q = 0; q < YTASK_ROWS; q++) acc[q] = cmake(0.0, 0.0);
+    // YB blocks per step, so that their Z loads are in flight together
+    constexpr int YB = 4;
+    for (int kb = k0; kb < k1; kb += YB) {
+        double c[YB];
+        cd v[YB][YTASK_ROWS];
+#pragma unroll
+        for (int u = 0; u < YB; u++) {
+            const int k = kb + u;
+            c[u] = 0.0;
+            if (k < k1) {
+                const cd *zk = zt + (size_t)(__builtin_amdgcn_readfirstlane(blk_zoff[k]) + loc0) * 64;
+#pragma unroll
+                for (int q = 0; q < YTASK_ROWS; q++)
+                    if (q < cnt) v[u][q] = zk[q * 64];
+#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    const int b = __builtin_amdgcn_readfirstlane(blk_tb[3 * k + t]);
+                    if (b >= 0) c[u] += blk_tf[3 * k + t] * bl[b];
+                }
             }
-            c[k] = v;
         }
-        for (int k = lane; k < 2 * nhalf; k += 64) ya[k] = 0.0;
+#pragma unroll
+        for (int u = 0; u < YB; u++)
+            if (kb + u < k1) {
+#pragma unroll
+                for (int q = 0; q < YTASK_ROWS; q++)
+                    if (q < cnt) {
+                        acc[q].re += c[u] * v[u][q].re;
+                        acc[q].im += c[u] * v[u][q].im;
+                    }
+            }
     }
-    __syncthreads();
-    if (valid) {
-        const cd *zi = reinterpret_cast<const cd *>(z) + (size_t)i * nz;
-#pragma unroll 4
-        for (int e = lane; e < nz; e += 64) {
-            const cd v = zi[e];
-            const double ck = c[zblk[e]];
-            const int h = zh[e];
-            atomicAdd(&ya[2 * h], ck * v.re);
-            atomicAdd(&ya[2 * h + 1], ck * v.im);
-        }
-    }
-    __syncthreads();
-    if (valid) {
-        cd *yo = reinterpret_cast<cd *>(y) + (size_t)i * nhalf;
-        for (int h = lane; h < nhalf; h += 64) {
-            const double f = hscale[h];
-            yo[h] = cmake(f * ya[2 * h], f * ya[2 * h + 1]);
-        }
+    if (a0 + lane < natoms) {
+        cd *yo = reinterpret_cast<cd *>(y) + (size_t)(a0 + lane) * nhalf + h0;
+#pragma unroll
+        for (int q = 0; q < YTASK_ROWS; q++)
+            if (q < cnt) yo[q] = cscale(acc[q], hscale[h0 + q]);
     }
 }
 
@@ -644,15 +687,16 @@ __global__ __launch_bounds__(64) void so4_dbdr_kernel(
             const int64_t a = s_atom[gg];
             if (a < 0) continue;
             const int blk = b0 + bk;
-            const cd *zb = reinterpret_cast<const cd *>(z) + (size_t)a * nz + blk_zoff[blk];
+            // Z tile-major: element e of atom a at [(a/64)*nz + e]*64 + a%64
+            const cd *zb = reinterpret_cast<const cd *>(z) + ((size_t)(a >> 6) * nz + blk_zoff[blk]) * 64 + (a & 63);
             double p = 0.0;
             int loc = 0;
             for (int hm = 0; 2 * hm < l; hm++)
-                for (int ma = 0; ma <= l; ma++, loc++) p += cdotre(Dl[gg][k][loc], zb[loc]);
+                for (int ma = 0; ma <= l; ma++, loc++) p += cdotre(Dl[gg][k][loc], zb[(size_t)loc * 64]);
             if ((l & 1) == 0) {
                 const int hm = l / 2;
-                for (int ma = 0; ma < hm; ma++, loc++) p += cdotre(Dl[gg][k][loc], zb[loc]);
-                p += 0.5 * cdotre(Dl[gg][k][loc], zb[loc]);
+                for (int ma = 0; ma < hm; ma++, loc++) p += cdotre(Dl[gg][k][loc], zb[(size_t)loc * 64]);
+                p += 0.5 * cdotre(Dl[gg][k][loc], zb[(size_t)loc * 64]);
             }
             (void)nloc;
             p *= nrm2 * s_w[gg];
@@ -843,12 +887,13 @@ __global__ __launch_bounds__(256) void so4_tangent_b_kernel(int64_t natoms, int 
     const int tid = threadIdx.x;
     for (int64_t i = blockIdx.x; i < natoms; i += gridDim.x) {
         const cd *dui = reinterpret_cast<const cd *>(dutot) + (size_t)i * nhalf;
-        const cd *zi = reinterpret_cast<const cd *>(z) + (size_t)i * nz;
+        // Z tile-major: element e of atom i at [(i/64)*nz + e]*64 + i%64
+        const cd *zi = reinterpret_cast<const cd *>(z) + (size_t)(i >> 6) * nz * 64 + (i & 63);
         for (int h = tid; h < nhalf; h += blockDim.x) du[h] = dui[h];
         __syncthreads();
         for (int e = tid; e < nz; e += blockDim.x) {
             const int h = zh[e];
-            q[e] = 2.0 * h_weight[h] * cdotre(du[h], zi[e]);
+            q[e] = 2.0 * h_weight[h] * cdotre(du[h], zi[(size_t)e * 64]);
         }
         __syncthreads();
         for (int k = tid; k < nblk; k += blockDim.x) {
@@ -973,9 +1018,15 @@ extern "C" int pxff_so4_yi(const pxff_so4_params *p, int64_t natoms, const doubl
     const So4Dev *T = nullptr;
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    const size_t lds = sizeof(double) * 4 * ((size_t)T->nblk + 2 * (size_t)T->nhalf);
-    so4_yi_kernel<<<div_up(natoms, 4), 256, lds, s>>>(natoms, T->nhalf, T->nz, T->nb, T->nblk, d_z, d_beta, T->zblk,
-                                                      T->zh, T->hscale, T->blk_tb, T->blk_tf, d_y);
+    const size_t lds = sizeof(double) * 64 * (size_t)T->nb;
+    const int64_t ntiles = (natoms + 63) / 64;
+    const int64_t grid = ntiles * ((T->nytask + 3) / 4);
+    PXFF_REQUIRE(grid <= 0x7fffffff, "too many atoms for one launch");
+    if (lds > 65536)
+        PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)so4_yi_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds));
+    so4_yi_kernel<<<(unsigned)grid, 256, lds, s>>>(natoms, T->nytask, T->nz, T->nb, T->nhalf, T->ytask, d_z, d_beta,
+                                                   T->blk_zoff, T->blk_tb, T->blk_tf, T->hscale, d_y);
     PXFF_LAUNCH_CHECK();
     return 0;
 }

@@ -128,6 +128,12 @@ bool build_so4_tables(int twol, int normalize_u, So4HostTables &t) {
                     t.idxb.push_back(l);
                 }
     t.nb = (int)t.idxb.size() / 3;
+    t.b_brow0.assign(t.nb, 0);
+    t.nbrow = 0;
+    for (int b = 0; b < t.nb; b++) {
+        t.b_brow0[b] = t.nbrow;
+        t.nbrow += t.idxb[3 * b + 2] / 2 + 1;
+    }
 
     // idxz, SNAP.py:1358-1416, plus the block list
     struct Blk {
@@ -255,6 +261,10 @@ bool build_so4_tables(int twol, int normalize_u, So4HostTables &t) {
                     if (mb < 0 || 2 * mb > l) continue;
                     mask |= 1 << j;
                     g[ZG_ZROW + j] = t.idxz_block[I3(l1, l2, l)] + mb * (l + 1);
+                    g[ZG_UROW + j] = t.hoff[l] + mb * (l + 1);
+                    for (int b = 0; b < t.nb; b++)
+                        if (t.idxb[3 * b] == l1 && t.idxb[3 * b + 1] == l2 && t.idxb[3 * b + 2] == l)
+                            g[ZG_BROW + j] = t.b_brow0[b] + mb;
                     cost += (l + 1) * (hi - lo + 1) * (l1 + 1);
                 }
                 if (!mask) continue;

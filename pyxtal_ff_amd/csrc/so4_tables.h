@@ -37,10 +37,15 @@ struct So4HostTables {
     std::vector<int> h_level;            // [nhalf] level of each half element
     std::vector<double> h_weight;        // [nhalf] half weight (1, 0.5 or 0)
     std::vector<int> h_diag;             // [nhalf] 1 if mb == ma
+    // B row partials: row (b, mb), mb = 0..l_b/2, is b_brow0[b] + mb
+    std::vector<int> b_brow0;            // [nb]
+    int nbrow = 0;
     // Z row groups (l1, l2, K) for the atom-per-lane kernel, ZG_STRIDE ints
     // each: {l1, l2, K, mb1lo, nbb, hoff[l1], hoff[l2], active-l mask, cost,
     // Z offset of the row of each l (ZG_ZROW + j, -1 if inactive), CG block
-    // of each l (ZG_CGBLK + j)}, l = l1 - l2 + 2 j
+    // of each l (ZG_CGBLK + j), B row of each l (ZG_BROW + j, -1 if the row
+    // is inactive or (l1, l2, l) is not a B component), half-U offset of
+    // row (l, mb) (ZG_UROW + j)}, l = l1 - l2 + 2 j
     std::vector<int> zgroups;            // [nzgroups*ZG_STRIDE]
     int nzgroups = 0;
 };
@@ -48,7 +53,9 @@ struct So4HostTables {
 constexpr int ZG_MAXL = 6;  // at most six l per (l1, l2) for 2J <= 10
 constexpr int ZG_ZROW = 9;
 constexpr int ZG_CGBLK = ZG_ZROW + ZG_MAXL;
-constexpr int ZG_STRIDE = 24;
+constexpr int ZG_BROW = ZG_CGBLK + ZG_MAXL;
+constexpr int ZG_UROW = ZG_BROW + ZG_MAXL;
+constexpr int ZG_STRIDE = ZG_UROW + ZG_MAXL + 3;
 
 // Split the Z row groups into nchunks contiguous chunks of about equal cost.
 std::vector<int> chunk_zgroups(const So4HostTables &t, int nchunks);

@@ -58,18 +58,34 @@ def ui(nl, cfg, atom_w, site_w, work_order=None, img_r=None):
     return out
 
 
+@lru_cache(maxsize=None)
+def _zlen_per_atom(twol):
+    n = ctypes.c_int64(0)
+    nat.check(nat.lib().pxff_so4_zlen(int(twol), 64, ctypes.byref(n)), "pxff_so4_zlen")
+    return n.value // 64
+
+
+def zlen(cfg, natoms):
+    """Length (float64) of the Z buffer of pxff_so4_zb (pxff_so4_zlen)."""
+    return (int(natoms) + 63) // 64 * 64 * _zlen_per_atom(cfg.twol)
+
+
 def z(nl, cfg, utot):
-    """Z [N, nz, 2] (flat) from the tile-major U_tot (one kernel launch)."""
-    p = nat.ptr
-    zz = _f64(nl.natoms * cfg.nz * 2, nl.pos.device)
-    nat.check(nat.lib().pxff_so4_zb(cfg.params, nl.natoms, p(utot), p(zz), None, nat.stream()), "pxff_so4_zb")
-    return zz
+    """Z from the tile-major U_tot (one kernel launch), tile-major too:
+    [ceil(N/64), nz, 64] complex (flat float64 pairs), padding lanes zero;
+    the buffer has room for the B row partials after Z."""
+    return zb(nl, cfg, utot, want_b=False)[0]
 
 
 def zb(nl, cfg, utot, want_b=True):
-    """Z [N, nz, 2] and B [N, nb]."""
-    zz = z(nl, cfg, utot)
-    return zz, (bi(nl, cfg, utot, zz) if want_b else None)
+    """Z (tile-major, see z) and B [N, nb]: one Z launch that also writes
+    per-row B partials, and a short launch summing them."""
+    p = nat.ptr
+    zz = _f64(zlen(cfg, nl.natoms), nl.pos.device)
+    b = _f64(nl.natoms * cfg.nb, nl.pos.device) if want_b else None
+    nat.check(nat.lib().pxff_so4_zb(cfg.params, nl.natoms, p(utot), p(zz), p(b) if want_b else None, nat.stream()),
+              "pxff_so4_zb")
+    return zz, (b[: nl.natoms * cfg.nb].view(nl.natoms, cfg.nb) if want_b else None)
 
 
 def bi(nl, cfg, utot, z):
