@@ -189,13 +189,18 @@ int pxff_so4_gather(int64_t natoms, const int64_t *d_img_start, const int64_t *d
 
 /* Forward-mode (tangent) of B along per-image displacements dr_e
  * [nimg*3]: dB [natoms*nb] = sum_e dB/dr_e . dr_e.  Used by the double
- * backward of the adjoint (force-matched training).  d_dutot is workspace
- * [natoms*nhalf*2]. */
+ * backward of the adjoint (force-matched training).  d_z is the tile-major Z
+ * of pxff_so4_zb; d_dutot is workspace [ceil(natoms/64)*64*nhalf*2]. */
 int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_j,
                      const double *d_img_r, const double *d_atom_w, const double *d_dr, const double *d_z,
                      double *d_dutot, double *d_db, void *stream);
 
-/* Plain (materialised) path, step 1: dB/dr per image [nimg*nb*3]. */
+/* Atom-major copy of the tile-major Z of pxff_so4_zb: d_za [natoms*nz*2],
+ * element e of atom i at i*nz + e (complex). */
+int pxff_so4_z_atom_major(const pxff_so4_params *p, int64_t natoms, const double *d_z, double *d_za, void *stream);
+
+/* Plain (materialised) path, step 1: dB/dr per image [nimg*nb*3] from the
+ * atom-major Z (pxff_so4_z_atom_major). */
 int pxff_so4_dbdr(const pxff_so4_params *p, int64_t nimg, const int64_t *d_img_atom, const int64_t *d_img_j,
                   const double *d_img_r, const double *d_atom_w, const double *d_z, double *d_dbdr, void *stream);
 

@@ -49,6 +49,7 @@ _SIGS = {
     "pxff_so4_ui": ([P, I64, P, P, P, P, P, P, P, P], I32),
     "pxff_so4_zb": ([P, I64, P, P, P, P], I32),
     "pxff_so4_zlen": ([ctypes.c_int, I64, P], I32),
+    "pxff_so4_z_atom_major": ([P, I64, P, P, P], I32),
     "pxff_so4_b": ([P, I64, P, P, P, P], I32),
     "pxff_segment_sum": ([I64, ctypes.c_int, P, P, P, P], I32),
     "pxff_mlp_energy_grad": ([I64, ctypes.c_int, ctypes.c_int, P, P, P, P, P, P, P, P, P, P], I32),

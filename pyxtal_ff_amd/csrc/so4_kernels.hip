@@ -687,16 +687,16 @@ __global__ __launch_bounds__(64) void so4_dbdr_kernel(
             const int64_t a = s_atom[gg];
             if (a < 0) continue;
             const int blk = b0 + bk;
-            // Z tile-major: element e of atom a at [(a/64)*nz + e]*64 + a%64
-            const cd *zb = reinterpret_cast<const cd *>(z) + ((size_t)(a >> 6) * nz + blk_zoff[blk]) * 64 + (a & 63);
+            // atom-major Z: each lane walks one contiguous block row
+            const cd *zb = reinterpret_cast<const cd *>(z) + (size_t)a * nz + blk_zoff[blk];
             double p = 0.0;
             int loc = 0;
             for (int hm = 0; 2 * hm < l; hm++)
-                for (int ma = 0; ma <= l; ma++, loc++) p += cdotre(Dl[gg][k][loc], zb[(size_t)loc * 64]);
+                for (int ma = 0; ma <= l; ma++, loc++) p += cdotre(Dl[gg][k][loc], zb[loc]);
             if ((l & 1) == 0) {
                 const int hm = l / 2;
-                for (int ma = 0; ma < hm; ma++, loc++) p += cdotre(Dl[gg][k][loc], zb[(size_t)loc * 64]);
-                p += 0.5 * cdotre(Dl[gg][k][loc], zb[(size_t)loc * 64]);
+                for (int ma = 0; ma < hm; ma++, loc++) p += cdotre(Dl[gg][k][loc], zb[loc]);
+                p += 0.5 * cdotre(Dl[gg][k][loc], zb[loc]);
             }
             (void)nloc;
             p *= nrm2 * s_w[gg];
@@ -868,54 +868,68 @@ __global__ void rdxdr_reduce_kernel(int64_t natoms, int nb, const int64_t *__res
 // ----------------------------------------------------------- tangent of B
 // dB_b = sum over the (block k, factor f) pairs targeting b of f p_k with
 // p_k = 2 Re sum_loc hw(loc) conj(dU^l[loc]) Z_k[loc] and dU = dU_tot (u_norm
-// already applied by the accumulation kernel).  One workgroup per atom
-// (persistent): dU into LDS, then the per-element products over the whole
-// Z row of the atom (coalesced reads), per-block sums, per-b gathers.
+// already applied by the accumulation kernel).  One workgroup per 64-atom
+// tile, one atom per lane: dU and Z are both tile-major, so every load is a
+// coalesced 1 KB row; the wavefronts split the blocks (p_k into LDS), then
+// the components b.
 __global__ __launch_bounds__(256) void so4_tangent_b_kernel(int64_t natoms, int nhalf, int nz, int nb, int nblk,
-                                                             int ldim, const double *__restrict__ dutot,
-                                                             const double *__restrict__ z, const int *__restrict__ zh,
+                                                             int ldim, const double *__restrict__ dutile,
+                                                             const double *__restrict__ z,
                                                              const double *__restrict__ h_weight,
+                                                             const int *__restrict__ hoff,
                                                              const int *__restrict__ blk_level_start,
                                                              const int *__restrict__ blk_zoff,
                                                              const int *__restrict__ b_src,
                                                              const double *__restrict__ b_srcf,
                                                              double *__restrict__ db) {
-    extern __shared__ double sm[];
-    cd *du = reinterpret_cast<cd *>(sm);  // [nhalf]
-    double *q = sm + 2 * nhalf;           // [nz]
-    double *pk = q + nz;                  // [nblk]
-    const int tid = threadIdx.x;
-    for (int64_t i = blockIdx.x; i < natoms; i += gridDim.x) {
-        const cd *dui = reinterpret_cast<const cd *>(dutot) + (size_t)i * nhalf;
-        // Z tile-major: element e of atom i at [(i/64)*nz + e]*64 + i%64
-        const cd *zi = reinterpret_cast<const cd *>(z) + (size_t)(i >> 6) * nz * 64 + (i & 63);
-        for (int h = tid; h < nhalf; h += blockDim.x) du[h] = dui[h];
-        __syncthreads();
-        for (int e = tid; e < nz; e += blockDim.x) {
-            const int h = zh[e];
-            q[e] = 2.0 * h_weight[h] * cdotre(du[h], zi[(size_t)e * 64]);
-        }
-        __syncthreads();
-        for (int k = tid; k < nblk; k += blockDim.x) {
-            int l = 0;
-            while (l + 1 < ldim && k >= blk_level_start[l + 1]) l++;
-            const int nloc = (l / 2 + 1) * (l + 1);
-            const int z0 = blk_zoff[k];
-            double p = 0.0;
-            for (int t = 0; t < nloc; t++) p += q[z0 + t];
-            pk[k] = p;
-        }
-        __syncthreads();
-        for (int b = tid; b < nb; b += blockDim.x) {
-            double acc = 0.0;
+    extern __shared__ double pk[];  // [nblk][64]
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t tile = blockIdx.x;
+    const cd *dut = reinterpret_cast<const cd *>(dutile) + (size_t)tile * nhalf * 64 + lane;
+    const cd *zt = reinterpret_cast<const cd *>(z) + (size_t)tile * nz * 64 + lane;
+    for (int k = wv; k < nblk; k += 4) {
+        int l = 0;
+        while (l + 1 < ldim && k >= blk_level_start[l + 1]) l++;
+        const int nloc = (l / 2 + 1) * (l + 1);
+        const int z0 = blk_zoff[k], h0 = hoff[l];
+        double p = 0.0;
+#pragma unroll 8
+        for (int t = 0; t < nloc; t++)
+            p += h_weight[h0 + t] * cdotre(dut[(size_t)(h0 + t) * 64], zt[(size_t)(z0 + t) * 64]);
+        pk[k * 64 + lane] = 2.0 * p;
+    }
+    __syncthreads();
+    const int64_t i = tile * 64 + lane;
+    if (i >= natoms) return;
+    for (int b = wv; b < nb; b += 4) {
+        double acc = 0.0;
 #pragma unroll
-            for (int t = 0; t < 3; t++) {
-                const int k = b_src[3 * b + t];
-                if (k >= 0) acc += b_srcf[3 * b + t] * pk[k];
-            }
-            db[(size_t)i * nb + b] = acc;
+        for (int t = 0; t < 3; t++) {
+            const int k = b_src[3 * b + t];
+            if (k >= 0) acc += b_srcf[3 * b + t] * pk[k * 64 + lane];
         }
-        __syncthreads();
+        db[(size_t)i * nb + b] = acc;
+    }
+}
+
+// Atom-major copy of the tile-major Z: za[i][e] = Z[i/64][e][i%64].  A
+// workgroup moves 32 rows of one tile through LDS: coalesced 1 KB row loads,
+// 512 B contiguous stores per atom.
+__global__ __launch_bounds__(256) void so4_z_atom_major_kernel(int64_t natoms, int nz, const double *__restrict__ zt,
+                                                                double *__restrict__ za) {
+    __shared__ cd buf[32][65];
+    const int64_t tile = blockIdx.x;
+    const int e0 = blockIdx.y * 32;
+    const int lane = threadIdx.x & 63;
+    const cd *src = reinterpret_cast<const cd *>(zt) + (size_t)tile * nz * 64;
+    for (int r = threadIdx.x >> 6; r < 32; r += 4)
+        if (e0 + r < nz) buf[r][lane] = src[(size_t)(e0 + r) * 64 + lane];
+    __syncthreads();
+    cd *dst = reinterpret_cast<cd *>(za);
+    for (int idx = threadIdx.x; idx < 64 * 32; idx += blockDim.x) {
+        const int a = idx >> 5, r = idx & 31;
+        const int64_t i = tile * 64 + a;
+        if (i < natoms && e0 + r < nz) dst[(size_t)i * nz + e0 + r] = buf[r][a];
     }
 }
 
@@ -1072,7 +1086,7 @@ extern "C" int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const 
     hipStream_t s = (hipStream_t)stream;
     const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * g_tune[PXFF_TUNE_ACCUM_WAVES_PER_CU]);
 #define PXFF_ACCUM_TAN(MODE)                                                                      \
-    PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, false, MODE><<<grid, 64, 0, s>>>(        \
+    PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, true, MODE><<<grid, 64, 0, s>>>(         \
                                     natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr, d_dr, nullptr, \
                                     T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, 0.0, d_dutot)))
     switch (tune_value(PXFF_TUNE_ACCUM_LDSADD)) {
@@ -1082,14 +1096,26 @@ extern "C" int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const 
     }
 #undef PXFF_ACCUM_TAN
     PXFF_LAUNCH_CHECK();
-    const size_t lds = sizeof(double) * (2 * (size_t)T->nhalf + T->nz + T->nblk);
-    const int gridb = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * 8);
+    const size_t lds = sizeof(double) * 64 * (size_t)T->nblk;
     if (lds > 65536)
         PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)so4_tangent_b_kernel,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    so4_tangent_b_kernel<<<gridb, 256, lds, s>>>(natoms, T->nhalf, T->nz, T->nb, T->nblk, p->twol + 1, d_dutot, d_z,
-                                                 T->zh, T->h_weight, T->blk_level_start, T->blk_zoff, T->b_src,
-                                                 T->b_srcf, d_db);
+    so4_tangent_b_kernel<<<(unsigned)div_up(natoms, 64), 256, lds, s>>>(
+        natoms, T->nhalf, T->nz, T->nb, T->nblk, p->twol + 1, d_dutot, d_z, T->h_weight, T->hoff,
+        T->blk_level_start, T->blk_zoff, T->b_src, T->b_srcf, d_db);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_so4_z_atom_major(const pxff_so4_params *p, int64_t natoms, const double *d_z, double *d_za,
+                                     void *stream) {
+    int rc = so4_check_params(p);
+    if (rc) return rc;
+    if (natoms <= 0) return 0;
+    const So4Dev *T = nullptr;
+    if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
+    const dim3 grid((unsigned)div_up(natoms, 64), (unsigned)div_up(T->nz, 32));
+    so4_z_atom_major_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(natoms, T->nz, d_z, d_za);
     PXFF_LAUNCH_CHECK();
     return 0;
 }

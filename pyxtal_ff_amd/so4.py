@@ -139,7 +139,7 @@ def tangent(nl, cfg, atom_w, z, dr, img_r=None):
     """dB [N, nb] = sum_e dB/dr_e . dr_e (forward-mode derivative of B)."""
     p = nat.ptr
     dev = nl.pos.device
-    du = _f64(nl.natoms * cfg.nhalf * 2, dev)
+    du = _f64((nl.natoms + 63) // 64 * 64 * cfg.nhalf * 2, dev)  # tile-major dU_tot
     db = _f64(nl.natoms * cfg.nb, dev)
     r = nl.img_r if img_r is None else img_r.contiguous()
     d = dr.contiguous()
@@ -148,12 +148,22 @@ def tangent(nl, cfg, atom_w, z, dr, img_r=None):
     return db[: nl.natoms * cfg.nb].view(nl.natoms, cfg.nb)
 
 
+def z_atom_major(nl, cfg, z):
+    """Atom-major copy [N, nz, 2] of the tile-major Z."""
+    za = _f64(nl.natoms * cfg.nz * 2, nl.pos.device)
+    nat.check(nat.lib().pxff_so4_z_atom_major(cfg.params, nl.natoms, nat.ptr(z), nat.ptr(za), nat.stream()),
+              "pxff_so4_z_atom_major")
+    return za
+
+
 def dbdr(nl, cfg, atom_w, z, img_r=None):
-    """Materialised dB/dr per image [nimg, nb, 3]."""
+    """Materialised dB/dr per image [nimg, nb, 3] (z: the tile-major Z; the
+    kernel reads an atom-major copy)."""
     p = nat.ptr
     out = _f64(nl.nimg * cfg.nb * 3, nl.pos.device)
     r = nl.img_r if img_r is None else img_r.contiguous()
-    nat.check(nat.lib().pxff_so4_dbdr(cfg.params, nl.nimg, p(nl.img_atom), p(nl.img_j), p(r), p(atom_w), p(z),
+    za = z_atom_major(nl, cfg, z)
+    nat.check(nat.lib().pxff_so4_dbdr(cfg.params, nl.nimg, p(nl.img_atom), p(nl.img_j), p(r), p(atom_w), p(za),
                                       p(out), nat.stream()), "pxff_so4_dbdr")
     return out
 
