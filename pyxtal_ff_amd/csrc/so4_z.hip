@@ -272,25 +272,24 @@ __global__ __launch_bounds__(256) void so4_z_tile_kernel(int64_t natoms, int nti
 }
 
 // B[i][b] = 2 sum_{mb <= l_b/2} P[b_brow0[b] + mb] from the row partials P
-// of the Z kernel ([tile][nbrow][64], one atom per lane).  The components b
-// are spread over the wavefronts of a tile (grid.y).
+// of the Z kernel ([tile][nbrow][64], one atom per lane): one thread per
+// (atom, b), the four wavefronts of a workgroup on four components of one
+// tile, so each thread's few loads are independent and coalesced.
 __global__ __launch_bounds__(256) void so4_brow_sum_kernel(int64_t natoms, int nb, int nbrow,
                                                             const int *__restrict__ b_brow0,
                                                             const int *__restrict__ b_level,
                                                             const double *__restrict__ bpart, double *__restrict__ bout) {
     const int lane = threadIdx.x & 63;
-    const int w = blockIdx.y * 4 + (threadIdx.x >> 6), nw = gridDim.y * 4;
+    const int b = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int64_t tile = blockIdx.x;
     const int64_t i = tile * 64 + lane;
-    if (i >= natoms) return;
+    if (i >= natoms || b >= nb) return;
     const double *pt = bpart + (size_t)tile * nbrow * 64 + lane;
-    for (int b = w; b < nb; b += nw) {
-        const int r0 = __builtin_amdgcn_readfirstlane(b_brow0[b]);
-        const int nr = __builtin_amdgcn_readfirstlane(b_level[b]) / 2 + 1;
-        double s = 0.0;
-        for (int r = 0; r < nr; r++) s += pt[(size_t)(r0 + r) * 64];
-        bout[(size_t)i * nb + b] = 2.0 * s;
-    }
+    const int r0 = b_brow0[b];
+    const int nr = b_level[b] / 2 + 1;
+    double s = 0.0;
+    for (int r = 0; r < nr; r++) s += pt[(size_t)(r0 + r) * 64];
+    bout[(size_t)i * nb + b] = 2.0 * s;
 }
 
 }  // namespace pxff
@@ -323,7 +322,7 @@ extern "C" int pxff_so4_zb(const pxff_so4_params *p, int64_t natoms, const doubl
     });
     PXFF_LAUNCH_CHECK();
     if (d_b) {
-        const dim3 gb((unsigned)ntiles, (unsigned)div_up(T->nb, 16));
+        const dim3 gb((unsigned)ntiles, (unsigned)div_up(T->nb, 4));
         so4_brow_sum_kernel<<<gb, 256, 0, s>>>(natoms, T->nb, T->nbrow, T->b_brow0, T->b_level, bpart, d_b);
         PXFF_LAUNCH_CHECK();
     }
