@@ -181,6 +181,15 @@ int pxff_so4_deidrj(const pxff_so4_params *p, int64_t nimg, const int64_t *d_img
                     const double *d_img_r, const double *d_atom_w, const double *d_y, double *d_dedr,
                     void *stream);
 
+/* pxff_so4_deidrj over an image capacity: images e < min(cap, *d_count)
+ * are processed, the count being read on the device (d_count = the capped
+ * image offsets' last entry, d_img_start + natoms), so the launch needs no
+ * host copy of the image count (HIP graph capture of a whole E+F step).
+ * d_dedr [cap*3]; entries at and beyond the count are not written. */
+int pxff_so4_deidrj_counted(const pxff_so4_params *p, int64_t cap, const int64_t *d_count,
+                            const int64_t *d_img_atom, const int64_t *d_img_j, const double *d_img_r,
+                            const double *d_atom_w, const double *d_y, double *d_dedr, void *stream);
+
 /* Forces and virial from dE/dr per image (deterministic gather via the
  * reverse-image index).  d_grad [natoms*3] = dE/dR_i (= -F_i);
  * d_virial [natoms*9] (or NULL) = per-centre sum_e r_e (x) dE/dr_e. */

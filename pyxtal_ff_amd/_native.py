@@ -55,6 +55,7 @@ _SIGS = {
     "pxff_mlp_energy_grad": ([I64, ctypes.c_int, ctypes.c_int, P, P, P, P, P, P, P, P, P, P], I32),
     "pxff_so4_yi": ([P, I64, P, P, P, P], I32),
     "pxff_so4_deidrj": ([P, I64, P, P, P, P, P, P, P], I32),
+    "pxff_so4_deidrj_counted": ([P, I64, P, P, P, P, P, P, P, P], I32),
     "pxff_so4_gather": ([I64, P, P, P, P, P, P, P], I32),
     "pxff_so4_tangent": ([P, I64, P, P, P, P, P, P, P, P, P], I32),
     "pxff_so4_dbdr": ([P, I64, P, P, P, P, P, P, P], I32),

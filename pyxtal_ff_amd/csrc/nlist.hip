@@ -483,6 +483,14 @@ __global__ __launch_bounds__(64) void nl_fill_wave_kernel(
     if (lane == 0) row_count[i] = firsts + (any_self ? 0 : 1);
 }
 
+// Zeroes n ints.  Used instead of a memset so that a build can be captured
+// into a HIP graph (memset nodes have given wrong results on replays after
+// the first).
+__global__ void nl_zero_i32_kernel(int32_t *__restrict__ p, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
+}
+
 // ------------------------------------------------------------ device scan
 // Exclusive scan by one workgroup of 1024 threads (chunked).  out has n+1
 // entries; out[n] = total.  With cap >= 0 every output is clamped to cap
@@ -548,13 +556,15 @@ extern "C" int pxff_nlist_count(const double *d_pos, int64_t natoms, const int64
     PXFF_REQUIRE(nbins_total > 0, "nbins_total must be > 0");
     if (natoms <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
-    PXFF_HIP_CHECK(hipMemsetAsync(d_bin_count, 0, sizeof(int32_t) * nbins_total, s));
+    nl_zero_i32_kernel<<<div_up(nbins_total, 256), 256, 0, s>>>(d_bin_count, nbins_total);
+    PXFF_LAUNCH_CHECK();
     nl_bin_kernel<<<div_up(natoms, 256), 256, 0, s>>>(natoms, d_pos, d_atom_struct, d_inv_cells, d_pbc, d_geom,
                                                       d_atom_bin, d_wrap, d_bin_count);
     PXFF_LAUNCH_CHECK();
     scan_kernel<int32_t><<<1, 1024, 0, s>>>(nbins_total, d_bin_count, d_bin_start);
     PXFF_LAUNCH_CHECK();
-    PXFF_HIP_CHECK(hipMemsetAsync(d_bin_count, 0, sizeof(int32_t) * nbins_total, s));
+    nl_zero_i32_kernel<<<div_up(nbins_total, 256), 256, 0, s>>>(d_bin_count, nbins_total);
+    PXFF_LAUNCH_CHECK();
     nl_bin_fill_kernel<<<div_up(natoms, 256), 256, 0, s>>>(natoms, d_atom_bin, d_bin_start, d_bin_count, d_bin_atoms);
     PXFF_LAUNCH_CHECK();
     nl_count_wave_kernel<<<(unsigned)natoms, 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom, rc * rc,
@@ -590,7 +600,8 @@ extern "C" int pxff_nlist_rev(int64_t natoms, int64_t nimg, const int64_t *d_img
                               void *stream) {
     if (natoms <= 0 || nimg <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
-    PXFF_HIP_CHECK(hipMemsetAsync(d_flag, 0, sizeof(int32_t), s));
+    nl_zero_i32_kernel<<<1, 64, 0, s>>>(d_flag, 1);
+    PXFF_LAUNCH_CHECK();
     nl_rev_kernel<<<div_up(nimg, 256), 256, 0, s>>>(natoms, nimg, d_img_start, d_img_atom, d_img_j, d_img_T, d_img_rev,
                                                     d_flag);
     PXFF_LAUNCH_CHECK();
@@ -603,7 +614,8 @@ extern "C" int pxff_nlist_rows(int64_t natoms, int64_t nimg, int64_t nrows, cons
                                void *stream) {
     if (natoms <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
-    PXFF_HIP_CHECK(hipMemsetAsync(d_flag, 0, sizeof(int32_t), s));
+    nl_zero_i32_kernel<<<1, 64, 0, s>>>(d_flag, 1);
+    PXFF_LAUNCH_CHECK();
     nl_rows_kernel<<<div_up(natoms, 64), 64, 0, s>>>(natoms, nimg, d_img_start, d_img_j, d_row_start, d_seq, d_row_img,
                                                      d_img_row, d_nsite);
     PXFF_LAUNCH_CHECK();

@@ -546,12 +546,19 @@ __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int
                                                            const double *__restrict__ atom_w,
                                                            const double *__restrict__ y, const double *rootpq_g,
                                                            double rcut, double rfac0, int cutoff_id,
-                                                           double *__restrict__ dedr) {
+                                                           double *__restrict__ dedr,
+                                                           const int64_t *__restrict__ d_count) {
     constexpr int G = TWOJ / 2 + 1;
     constexpr int NG = 64 / G;
     constexpr int NH = nhalf_of(TWOJ);
     __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
     __shared__ double red[4][64][4];
+    // with d_count the grid covers an image capacity and the images in use
+    // are counted on the device
+    if (d_count) {
+        const int64_t c = *d_count;
+        nimg = c < nimg ? c : nimg;
+    }
     const int tid = threadIdx.x;
     const int wv = tid >> 6, lane = tid & 63;
     const int wpb = blockDim.x >> 6;
@@ -1045,9 +1052,9 @@ extern "C" int pxff_so4_yi(const pxff_so4_params *p, int64_t natoms, const doubl
     return 0;
 }
 
-extern "C" int pxff_so4_deidrj(const pxff_so4_params *p, int64_t nimg, const int64_t *d_img_atom,
-                               const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
-                               const double *d_y, double *d_dedr, void *stream) {
+static int so4_deidrj_launch(const pxff_so4_params *p, int64_t nimg, const int64_t *d_count,
+                             const int64_t *d_img_atom, const int64_t *d_img_j, const double *d_img_r,
+                             const double *d_atom_w, const double *d_y, double *d_dedr, void *stream) {
     int rc = so4_check_params(p);
     if (rc) return rc;
     if (nimg <= 0) return 0;
@@ -1059,9 +1066,22 @@ extern "C" int pxff_so4_deidrj(const pxff_so4_params *p, int64_t nimg, const int
     const int grid = div_up(nimg, wpb * ng);
     PXFF_DISPATCH_TWOJ(p->twol, (so4_deidrj_kernel<TJ><<<grid, 64 * wpb, 0, s>>>(nimg, d_img_atom, d_img_j, d_img_r, d_atom_w,
                                                                            d_y, T->rootpq, p->rcut, p->rfac0,
-                                                                           p->cutoff_id, d_dedr)));
+                                                                           p->cutoff_id, d_dedr, d_count)));
     PXFF_LAUNCH_CHECK();
     return 0;
+}
+
+extern "C" int pxff_so4_deidrj(const pxff_so4_params *p, int64_t nimg, const int64_t *d_img_atom,
+                               const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
+                               const double *d_y, double *d_dedr, void *stream) {
+    return so4_deidrj_launch(p, nimg, nullptr, d_img_atom, d_img_j, d_img_r, d_atom_w, d_y, d_dedr, stream);
+}
+
+extern "C" int pxff_so4_deidrj_counted(const pxff_so4_params *p, int64_t cap, const int64_t *d_count,
+                                       const int64_t *d_img_atom, const int64_t *d_img_j, const double *d_img_r,
+                                       const double *d_atom_w, const double *d_y, double *d_dedr, void *stream) {
+    PXFF_REQUIRE(d_count != nullptr, "null image count");
+    return so4_deidrj_launch(p, cap, d_count, d_img_atom, d_img_j, d_img_r, d_atom_w, d_y, d_dedr, stream);
 }
 
 extern "C" int pxff_so4_gather(int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_rev,

@@ -92,9 +92,12 @@ class NeighborListBuilder:
         # it grows when a build overflows it
         self.img_capacity = max(1024, 48 * n)
 
-    def build(self, pos):
-        """A DeviceNeighborList of float64 device positions pos [natoms, 3]."""
-        return DeviceNeighborList(None, None, None, self.rc, pos_device=pos, builder=self)
+    def build(self, pos, host_count=True):
+        """A DeviceNeighborList of float64 device positions pos [natoms, 3].
+        host_count=False: no copy of the image count to the host (the list
+        can be built inside a HIP graph capture; ``nimg`` is then not
+        available and consumers size their launches by ``img_capacity``)."""
+        return DeviceNeighborList(None, None, None, self.rc, pos_device=pos, builder=self, host_count=host_count)
 
 
 class DeviceNeighborList:
@@ -104,7 +107,7 @@ class DeviceNeighborList:
     pbcs: [nstruct, 3].  With ``pos_device`` (a float64 [N, 3] device tensor)
     positions are taken from HBM; ``builder`` supplies cached metadata."""
 
-    def __init__(self, positions, cells, pbcs, rc, device="cuda", pos_device=None, builder=None):
+    def __init__(self, positions, cells, pbcs, rc, device="cuda", pos_device=None, builder=None, host_count=True):
         if builder is None:
             if pos_device is not None:
                 if positions is not None:
@@ -131,6 +134,7 @@ class DeviceNeighborList:
         else:
             pos_all = np.concatenate(positions, axis=0) if b.nstruct else np.zeros((0, 3))
             self.pos = torch.as_tensor(pos_all, **f64).contiguous()
+        self._host_count = bool(host_count)
         self._build(b.nbins, f64, i64, i32)
 
     def _build(self, nbins, f64, i64, i32):
@@ -171,10 +175,12 @@ class DeviceNeighborList:
         total = torch.empty(1, **i64)
         nat.check(L.pxff_scan_i64_capped(p(self._img_count), n, self._cap, p(img_start), p(total), s),
                   "pxff_scan_i64_capped")
-        self._total_host = torch.empty(1, dtype=torch.int64, pin_memory=True)
-        self._total_host.copy_(total, non_blocking=True)
-        self._total_ev = torch.cuda.Event()
-        self._total_ev.record()
+        self._total = total
+        if self._host_count:
+            self._total_host = torch.empty(1, dtype=torch.int64, pin_memory=True)
+            self._total_host.copy_(total, non_blocking=True)
+            self._total_ev = torch.cuda.Event()
+            self._total_ev.record()
         self._nimg = None
         self.img_start = img_start
         self.img_atom = torch.empty(m, **i64)
@@ -201,6 +207,8 @@ class DeviceNeighborList:
         ``generation`` increases: work launched on this list before reading
         nimg used a cut list and must be redone."""
         if self._nimg is None:
+            if not self._host_count:
+                raise nat.PxffError("this list was built without the host image count (host_count=False)")
             self._total_ev.synchronize()
             total = int(self._total_host[0])
             if total > self._cap:
@@ -216,6 +224,12 @@ class DeviceNeighborList:
     @property
     def img_capacity(self):
         return self._cap
+
+    @property
+    def total_device(self):
+        """Unclamped image count [1] on the device (a capacity above it means
+        the list is complete)."""
+        return self._total
 
     def verify(self):
         """Raise if the reverse-image / reverse-row search failed (reads a

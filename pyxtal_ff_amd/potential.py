@@ -220,3 +220,93 @@ def energy_forces_plain(nl, cfg, atom_w, site_w, model, species, stress=False):
         rv = dev_so4.rdxdr_reduce(nl, cfg.nb, rd)  # [N, nb, 6]
         out["stress_voigt"] = -torch.einsum("ik,ikl->l", beta, rv)
     return out
+
+
+def _segment_total(v, seg):
+    """Column sums of v [rows, ncol] over the one segment seg = [0, rows]
+    (pxff_segment_sum: fixed reduction order, no atomics, no memset)."""
+    from . import _native as nat
+    vv = v.contiguous()
+    ncol = vv.shape[1] if vv.dim() > 1 else 1
+    out = torch.empty(ncol, dtype=torch.float64, device=v.device)
+    nat.check(nat.lib().pxff_segment_sum(1, ncol, nat.ptr(seg), nat.ptr(vv), nat.ptr(out), nat.stream()),
+              "pxff_segment_sum")
+    return out
+
+
+class FusedEFGraph:
+    """Energy, forces (and the virial) of a fixed set of atoms as one HIP
+    graph replay: the neighbour-list rebuild from the positions in HBM and
+    the fused adjoint kernels of ``energy_forces_fused``, captured once and
+    replayed per step, so no host work sits between the kernels and the step
+    never waits for the host.
+
+    The captured list has a fixed image capacity (the builder's).  Each
+    replay's unclamped image count is copied back asynchronously; with
+    ``check`` (default) the call waits for it, and if it exceeded the
+    capacity the capacity is grown, the step re-captured and replayed (the
+    recovery ``DeviceNeighborList.nimg`` does for eager builds).  The
+    returned tensors belong to the graph: the next replay overwrites them.
+    Per-image work is launched over the capacity and bounded by the image
+    count on the device (``pxff_so4_deidrj_counted``)."""
+
+    def __init__(self, builder, cfg, atom_w, site_w, model, species_atom, pos, virial=False, warmup=2):
+        self.builder, self.cfg, self.atom_w, self.site_w = builder, cfg, atom_w, site_w
+        self.model, self.species_atom, self.virial = model, species_atom, virial
+        self.pos = pos.detach().clone().contiguous()
+        self._seg = torch.tensor([0, builder.natoms], dtype=torch.int64, device=pos.device)
+        self._count_host = torch.empty(1, dtype=torch.int64, pin_memory=True)
+        self._ev = torch.cuda.Event()
+        self.captures = 0
+        self._capture(warmup)
+
+    def _body(self):
+        n, cfg = self.builder.natoms, self.cfg
+        nl = self.builder.build(self.pos, host_count=False)
+        b, z, _ = dev_so4.forward(nl, cfg, self.atom_w, self.site_w)
+        e_atom, beta = self.model.energy_grad(b, self.species_atom)
+        y = dev_so4.yi(nl, cfg, z, beta)
+        dedr = dev_so4.deidrj(nl, cfg, self.atom_w, y, counted=True)
+        out = {"atom_energies": e_atom, "energy": _segment_total(e_atom, self._seg)[0]}
+        if self.virial:
+            g, vir = dev_so4.gather(nl, dedr, virial=True)
+            out["virial"] = _segment_total(vir.reshape(n, 9), self._seg).view(3, 3)
+        else:
+            g = dev_so4.gather(nl, dedr)
+        out["forces"] = -g
+        self._nl = nl
+        return out
+
+    def _capture(self, warmup):
+        for _ in range(warmup):  # tables, tuned launch parameters, allocator pools
+            self._body()
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = self._body()
+        self.captures += 1
+
+    def __call__(self, pos=None, check=True):
+        if pos is not None:
+            self.pos.copy_(pos)
+        self.graph.replay()
+        self._count_host.copy_(self._nl.total_device, non_blocking=True)
+        self._ev.record()
+        if check:
+            self.check()
+        return self.out
+
+    def check(self):
+        """Wait for the last replay's image count; redo the step with a
+        larger capacity if the list was cut.  Returns the image count."""
+        while True:
+            self._ev.synchronize()
+            total = int(self._count_host[0])
+            if total <= self._nl.img_capacity:
+                return total
+            b = self.builder
+            b.img_capacity = max(b.img_capacity, total + total // 4 + 64)
+            self._capture(1)
+            self.graph.replay()
+            self._count_host.copy_(self._nl.total_device, non_blocking=True)
+            self._ev.record()

@@ -111,11 +111,20 @@ def yi(nl, cfg, z, beta):
     return y
 
 
-def deidrj(nl, cfg, atom_w, y, img_r=None):
-    """dE/dr per image [nimg, 3]."""
+def deidrj(nl, cfg, atom_w, y, img_r=None, counted=False):
+    """dE/dr per image [nimg, 3].  counted: over the list's image capacity
+    with the image count read on the device (no host wait; returns
+    [capacity, 3], rows at and beyond the count unwritten)."""
     p = nat.ptr
-    out = _f64(nl.nimg * 3, nl.pos.device)
     r = nl.img_r if img_r is None else img_r.contiguous()
+    if counted:
+        cap = nl.img_capacity
+        out = _f64(cap * 3, nl.pos.device)
+        count = nl.img_start[nl.natoms:]
+        nat.check(nat.lib().pxff_so4_deidrj_counted(cfg.params, cap, p(count), p(nl.img_atom), p(nl.img_j), p(r),
+                                                    p(atom_w), p(y), p(out), nat.stream()), "pxff_so4_deidrj_counted")
+        return out[: cap * 3].view(cap, 3)
+    out = _f64(nl.nimg * 3, nl.pos.device)
     nat.check(nat.lib().pxff_so4_deidrj(cfg.params, nl.nimg, p(nl.img_atom), p(nl.img_j), p(r), p(atom_w), p(y),
                                         p(out), nat.stream()), "pxff_so4_deidrj")
     return out[: nl.nimg * 3].view(nl.nimg, 3)
