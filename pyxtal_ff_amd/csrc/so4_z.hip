@@ -47,7 +47,7 @@ static int get_zchunks(int twol, int norm, int nch, const int **out) {
 // compute_zi (SNAP.py:1050-1098) with one atom per lane: a wavefront owns a
 // tile of 64 atoms and a chunk of Z row groups, so the loop structure and
 // the Clebsch-Gordan coefficients are the same for all lanes (uniform
-// branches, broadcast LDS reads).  Row (l1, l2, l, mb) is
+// branches, coefficients as scalar operands).  Row (l1, l2, l, mb) is
 //   Z[ma] = sum_{mb1+mb2=K} C[mb1][mb2] sum_{ma1+ma2=ma+s} C[ma1][ma2] U1[mb1][ma1] U2[mb2][ma2],
 // s = (l1 + l2 - l)/2, K = mb + s.  The rows of one (l1, l2) with the same
 // K (one per l) share every (mb1, mb2) pair and hence every product
@@ -158,6 +158,9 @@ __device__ __forceinline__ void zgroup(const cd *__restrict__ ut, int lane, cons
         asm volatile("" ::: "memory");
         static_for<L1 + 1>([&](auto A) {
             constexpr int a = decltype(A)::value;
+            // one ma1 at a time: its coefficient loads are not hoisted over
+            // the whole pair (they would not fit in the scalar registers)
+            asm volatile("" ::: "memory");
             const cd u1a = urow_get<a>(r1);
             static_for<L2 + 1>([&](auto C) {
                 constexpr int c = decltype(C)::value;
@@ -247,9 +250,11 @@ __global__ __launch_bounds__(256) void so4_z_tile_kernel(int64_t natoms, int nti
                                                           double *__restrict__ zout, int nz,
                                                           double *__restrict__ bpart, int nbrow) {
     constexpr int NH = nhalf_of(TWOJ);
-    extern __shared__ double cg[];  // the whole CG table; uniform-address reads broadcast
-    for (int k = threadIdx.x; k < ncg; k += blockDim.x) cg[k] = cg_g[k];
-    __syncthreads();
+    // CG coefficients by wave-uniform addresses: scalar loads into SGPR
+    // operands (an LDS copy would cost 512 B of LDS bandwidth per coefficient
+    // read, all lanes receiving the same value)
+    const double *__restrict__ cg = cg_g;
+    (void)ncg;
     // workgroup = 4 wavefronts on 4 consecutive chunks of one tile
     const int b = blockIdx.x;
     const int nq = (nch + 3) / 4;
@@ -310,16 +315,11 @@ extern "C" int pxff_so4_zb(const pxff_so4_params *p, int64_t natoms, const doubl
     const int *zch = nullptr;
     if ((rc = get_zchunks(p->twol, p->normalize_u, nch, &zch))) return rc;
     const int grid = 8 * ((ntiles + 7) / 8) * ((nch + 3) / 4);
-    const size_t lds = sizeof(double) * (size_t)T->ncg;
     // B row partials after Z in the same buffer
     double *bpart = d_b ? d_z + 2 * (size_t)ntiles * 64 * T->nz : nullptr;
-    PXFF_DISPATCH_TWOJ(p->twol, {
-        auto kern = so4_z_tile_kernel<TJ>;
-        if (lds > 65536)
-            PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        kern<<<grid, 256, lds, s>>>(natoms, ntiles, nch, zch, T->zgroups, T->cg, T->ncg, d_utot, d_z, T->nz, bpart,
-                                    T->nbrow);
-    });
+    PXFF_DISPATCH_TWOJ(p->twol, (so4_z_tile_kernel<TJ><<<grid, 256, 0, s>>>(natoms, ntiles, nch, zch, T->zgroups,
+                                                                           T->cg, T->ncg, d_utot, d_z, T->nz, bpart,
+                                                                           T->nbrow)));
     PXFF_LAUNCH_CHECK();
     if (d_b) {
         const dim3 gb((unsigned)ntiles, (unsigned)div_up(T->nb, 4));
