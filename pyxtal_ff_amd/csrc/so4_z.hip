@@ -84,20 +84,22 @@ __device__ __forceinline__ double xsign(double x, uint32_t m) {
 // pattern applied by XOR: element ma gets sign masks (sr, si) = (m0, m0^SB)
 // for even ma and (m0^SB, m0) for odd ma, m0 = 0 for stored rows.
 struct URow {
-    const cd *p;  // element 0, lane offset included
+    const cd *p;  // element 0 of the row in lane 0 (wave-uniform)
     int step;     // +-64 (elements)
     uint32_t se, so;
+    int lane;
 };
 template <int LV>
 __device__ __forceinline__ URow urow(const cd *__restrict__ ut, int mb, int hoffl, int lane) {
     URow r;
+    r.lane = lane;
     if (2 * mb > LV) {
-        r.p = ut + (hoffl + (LV - mb) * (LV + 1) + LV) * 64 + lane;
+        r.p = ut + (hoffl + (LV - mb) * (LV + 1) + LV) * 64;
         r.step = -64;
         r.se = (uint32_t)(mb & 1) << 31;
         r.so = r.se ^ 0x80000000u;
     } else {
-        r.p = ut + (hoffl + mb * (LV + 1)) * 64 + lane;
+        r.p = ut + (hoffl + mb * (LV + 1)) * 64;
         r.step = 64;
         r.se = 0;
         r.so = 0;
@@ -106,7 +108,10 @@ __device__ __forceinline__ URow urow(const cd *__restrict__ ut, int mb, int hoff
 }
 template <int MA>
 __device__ __forceinline__ cd urow_get(const URow &r) {
-    const cd v = r.p[MA * r.step];
+    // uniform row element address, lane as the only per-lane offset: the
+    // address arithmetic stays scalar
+    const cd *e = r.p + MA * r.step;
+    const cd v = e[r.lane];
     const uint32_t sr = (MA & 1) ? r.so : r.se;
     const uint32_t si = (MA & 1) ? r.se : r.so;
     // stored rows: se = so = 0 and the XORs are no-ops; mirrored rows:
@@ -152,7 +157,13 @@ __device__ __forceinline__ void zgroup(const cd *__restrict__ ut, int lane, cons
         static_for<L2 + 1>([&](auto C) { u2[decltype(C)::value] = urow_get<decltype(C)::value>(r2); });
         double cgb[NL];
 #pragma unroll
-        for (int j = 0; j < NL; j++) cgb[j] = ((mask >> j) & 1) ? cg[cgblk[j] + mb1 * L2P + mb2] : 0.0;
+        for (int j = 0; j < NL; j++) {
+            cgb[j] = ((mask >> j) & 1) ? cg[cgblk[j] + mb1 * L2P + mb2] : 0.0;
+            // in a VGPR: the product with a scalar coefficient per term then
+            // takes one scalar operand (a VALU instruction reads at most
+            // one), no copy of the coefficient into a VGPR per term
+            asm volatile("" : "+v"(cgb[j]));
+        }
         // keep the CG reads of the product loop inside the pair loop:
         // hoisted, they would not fit in registers beside acc and U
         asm volatile("" ::: "memory");
