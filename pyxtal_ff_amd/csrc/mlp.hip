@@ -9,7 +9,10 @@
 // from L2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pxff_common.h"
+#include "so4_dev.h"
 
 namespace pxff {
 
@@ -67,28 +70,36 @@ __device__ inline void act_fdf(int id, double z, double &a, double &da) {
 //   row-major [nout][nin] as torch.nn.Linear stores it (read by the
 //   backward: lanes over inputs, consecutive addresses) and its transpose
 //   [nin][nout] (read by the forward: lanes over outputs), offsets into
-//   d_params.
-__global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int nin, int nelem,
-                                                              const int64_t *__restrict__ species,
-                                                               const int *__restrict__ layout,
-                                                               const double *__restrict__ params,
-                                                               const double *__restrict__ dmin,
-                                                               const double *__restrict__ scale,
-                                                               const double *__restrict__ x, double *__restrict__ e_out,
-                                                               double *__restrict__ beta) {
-    extern __shared__ double sm[];
-    const int lane = threadIdx.x;
-    // input [nin], then per layer the activation derivative [MLP_MAXW] (the
-    // pre-activation gradient after the backward) and the output [MLP_MAXW]
-    double *h0 = sm;
-    double *dact = h0 + nin;                    // [MLP_MAXL][MLP_MAXW]
-    double *hout = dact + MLP_MAXL * MLP_MAXW;  // [MLP_MAXL][MLP_MAXW]
-    const int64_t i = blockIdx.x;
+//   the parameter array.
+
+// Workgroup-wide barrier (one wavefront per workgroup).
+struct WgSync {
+    __device__ void operator()() const { __syncthreads(); }
+};
+// The wavefront's own LDS writes are visible to its later reads (LDS
+// operations of one wavefront execute in order; the fences keep the
+// compiler from moving them).
+struct WaveSync {
+    __device__ void operator()() const {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
+// One atom on one wavefront: lane n holds neuron n of a layer; h0 [nin],
+// dact / hout [MLP_MAXL][MLP_MAXW] are the wavefront's LDS scratch.
+template <typename Sync>
+__device__ inline void mlp_atom(int64_t i, int lane, int nin, int nelem, const int64_t *__restrict__ species,
+                                const int *__restrict__ layout, const double *params, const double *__restrict__ dmin,
+                                const double *__restrict__ scale, const double *__restrict__ x,
+                                double *__restrict__ e_out, double *__restrict__ beta, double *h0, double *dact,
+                                double *hout, Sync sync) {
     const int64_t el64 = species[i];
     if (el64 < 0 || el64 >= nelem) {  // no network for this atom: NaN results
         if (lane == 0) e_out[i] = __builtin_nan("");
         for (int j = lane; j < nin; j += 64) beta[(size_t)i * nin + j] = __builtin_nan("");
-        return;  // whole workgroup (one wavefront)
+        return;  // whole wavefront
     }
     const int el = (int)el64;
     const int *lay = layout + el * MLP_STRIDE;
@@ -96,7 +107,7 @@ __global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int
     const double *xi = x + (size_t)i * nin;
     const double *dm = dmin + (size_t)el * nin, *sc = scale + (size_t)el * nin;
     for (int j = lane; j < nin; j += 64) h0[j] = (xi[j] - dm[j]) * sc[j];
-    __syncthreads();
+    sync();
     // forward
     const double *hin = h0;
     for (int k = 0; k < nl; k++) {
@@ -112,7 +123,7 @@ __global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int
             hout[k * MLP_MAXW + lane] = a;
             dact[k * MLP_MAXW + lane] = da;
         }
-        __syncthreads();
+        sync();
         hin = hout + k * MLP_MAXW;
     }
     if (lane == 0) e_out[i] = hin[0];
@@ -129,7 +140,7 @@ __global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int
             for (int n = 0; n < no; n++) g += W[(size_t)n * ni + lane] * dact[k * MLP_MAXW + n];
             dact[(k - 1) * MLP_MAXW + lane] *= g;
         }
-        __syncthreads();
+        sync();
     }
     {
         const int *L = lay + 1;
@@ -142,6 +153,50 @@ __global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int
             beta[(size_t)i * nin + j] = g * sc[j];
         }
     }
+    sync();  // the next atom reuses the scratch
+}
+
+// One wavefront (= one workgroup) per atom, weights read from global memory
+// (L2).
+__global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int nin, int nelem,
+                                                              const int64_t *__restrict__ species,
+                                                              const int *__restrict__ layout,
+                                                              const double *__restrict__ params,
+                                                              const double *__restrict__ dmin,
+                                                              const double *__restrict__ scale,
+                                                              const double *__restrict__ x, double *__restrict__ e_out,
+                                                              double *__restrict__ beta) {
+    extern __shared__ double sm[];
+    double *h0 = sm;
+    double *dact = h0 + nin;
+    double *hout = dact + MLP_MAXL * MLP_MAXW;
+    mlp_atom(blockIdx.x, threadIdx.x, nin, nelem, species, layout, params, dmin, scale, x, e_out, beta, h0, dact, hout,
+             WgSync());
+}
+
+// Persistent workgroups of MLP_LW wavefronts: the parameters of every
+// element network are copied into LDS once per workgroup, then each
+// wavefront runs atoms (grid stride) with its own scratch, reading the
+// weights from LDS instead of L2.
+constexpr int MLP_LW = 4;
+__global__ __launch_bounds__(64 * MLP_LW) void mlp_lds_kernel(int64_t natoms, int nin, int nelem,
+                                                               const int64_t *__restrict__ species,
+                                                               const int *__restrict__ layout,
+                                                               const double *__restrict__ params, int nparams,
+                                                               const double *__restrict__ dmin,
+                                                               const double *__restrict__ scale,
+                                                               const double *__restrict__ x,
+                                                               double *__restrict__ e_out, double *__restrict__ beta) {
+    extern __shared__ double sm[];
+    double *P = sm;
+    for (int k = threadIdx.x; k < nparams; k += blockDim.x) P[k] = params[k];
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *h0 = sm + nparams + (size_t)wv * (nin + 2 * MLP_MAXL * MLP_MAXW);
+    double *dact = h0 + nin;
+    double *hout = dact + MLP_MAXL * MLP_MAXW;
+    for (int64_t i = (int64_t)blockIdx.x * MLP_LW + wv; i < natoms; i += (int64_t)gridDim.x * MLP_LW)
+        mlp_atom(i, lane, nin, nelem, species, layout, P, dmin, scale, x, e_out, beta, h0, dact, hout, WaveSync());
 }
 
 }  // namespace pxff
@@ -170,7 +225,32 @@ extern "C" int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const in
     }
     if (natoms <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
-    const size_t lds = sizeof(double) * ((size_t)nin + 2 * MLP_MAXL * MLP_MAXW);
+    // parameter array length: the furthest end of any W, b or W^T
+    int64_t nparams = 0;
+    for (int e = 0; e < nelem; e++) {
+        const int *lay = h_layout + e * MLP_STRIDE;
+        for (int k = 0; k < lay[0]; k++) {
+            const int *L = lay + 1 + MLP_LSTRIDE * k;
+            const int64_t nw = (int64_t)L[0] * L[1];
+            nparams = std::max({nparams, L[3] + nw, (int64_t)L[4] + L[1], L[5] + nw});
+        }
+    }
+    const size_t scratch = sizeof(double) * ((size_t)nin + 2 * MLP_MAXL * MLP_MAXW);
+    const size_t ldsp = sizeof(double) * (size_t)nparams + MLP_LW * scratch;
+    if (ldsp <= 160 * 1024) {  // the weights fit in LDS next to the scratch
+        static int attr_set = 0;
+        if (ldsp > 65536 && !attr_set) {
+            PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)mlp_lds_kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr_set = 1;
+        }
+        const int64_t nwg = std::min<int64_t>((natoms + MLP_LW - 1) / MLP_LW, (int64_t)num_cus() * 2);
+        mlp_lds_kernel<<<(unsigned)nwg, 64 * MLP_LW, ldsp, s>>>(natoms, nin, nelem, d_species, d_layout, d_params,
+                                                               (int)nparams, d_dmin, d_scale, d_x, d_e, d_beta);
+        PXFF_LAUNCH_CHECK();
+        return 0;
+    }
+    const size_t lds = scratch;
     if (lds > 65536)
         PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)mlp_energy_grad_kernel,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
