@@ -59,12 +59,12 @@ __device__ inline double dist2(const double r[3]) {
 }
 
 // ---------------------------------------------------------------- binning
-__global__ void nl_bin_kernel(int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
-                              const double *__restrict__ inv_cells, const int32_t *__restrict__ pbc,
-                              const int32_t *__restrict__ geom, int32_t *__restrict__ atom_bin,
-                              int32_t *__restrict__ wrap, int32_t *__restrict__ bin_count) {
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= natoms) return;
+// Bin of atom a (global bin index) and its lattice wrap, stored to
+// atom_bin[a] / wrap[3a..3a+2].
+__device__ inline int nl_bin_of(int64_t a, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
+                                const double *__restrict__ inv_cells, const int32_t *__restrict__ pbc,
+                                const int32_t *__restrict__ geom, int32_t *__restrict__ atom_bin,
+                                int32_t *__restrict__ wrap) {
     const int64_t s = atom_struct[a];
     const double *inv = inv_cells + 9 * s;
     const int32_t *gm = geom + 8 * s;
@@ -86,7 +86,65 @@ __global__ void nl_bin_kernel(int64_t natoms, const double *__restrict__ pos, co
     }
     const int gb = gm[6] + (b[0] * gm[1] + b[1]) * gm[2] + b[2];
     atom_bin[a] = gb;
-    atomicAdd(&bin_count[gb], 1);
+    return gb;
+}
+
+__global__ void nl_bin_kernel(int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
+                              const double *__restrict__ inv_cells, const int32_t *__restrict__ pbc,
+                              const int32_t *__restrict__ geom, int32_t *__restrict__ atom_bin,
+                              int32_t *__restrict__ wrap, int32_t *__restrict__ bin_count) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms) return;
+    atomicAdd(&bin_count[nl_bin_of(a, pos, atom_struct, inv_cells, pbc, geom, atom_bin, wrap)], 1);
+}
+
+// The whole binning of a small list in one workgroup of 1024 threads, with
+// the bin counters in LDS: atom bins and wraps, bin sizes, their exclusive
+// scan (bin_start, nbins + 1 entries) and the atoms of every bin -- one
+// launch instead of five.  The order of the atoms inside a bin is not
+// fixed (LDS atomics), as with the multi-kernel path; the image lists do
+// not depend on it (their fill ranks images by (j, T)).
+constexpr int NL_SMALL_BINS = 8192;
+constexpr int NL_SMALL_ATOMS = 1 << 16;
+__global__ __launch_bounds__(1024) void nl_bin_small_kernel(int64_t natoms, int nbins, const double *__restrict__ pos,
+                                                             const int64_t *__restrict__ atom_struct,
+                                                             const double *__restrict__ inv_cells,
+                                                             const int32_t *__restrict__ pbc,
+                                                             const int32_t *__restrict__ geom,
+                                                             int32_t *__restrict__ atom_bin, int32_t *__restrict__ wrap,
+                                                             int32_t *__restrict__ bin_start,
+                                                             int32_t *__restrict__ bin_atoms) {
+    __shared__ int cnt[NL_SMALL_BINS];
+    __shared__ int part[1024];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < nbins; k += 1024) cnt[k] = 0;
+    __syncthreads();
+    for (int64_t a = tid; a < natoms; a += 1024)
+        atomicAdd(&cnt[nl_bin_of(a, pos, atom_struct, inv_cells, pbc, geom, atom_bin, wrap)], 1);
+    __syncthreads();
+    // exclusive scan: a contiguous chunk of bins per thread
+    const int chunk = (nbins + 1023) / 1024;
+    const int b0 = tid * chunk, b1 = b0 + chunk < nbins ? b0 + chunk : nbins;
+    int sum = 0;
+    for (int k = b0; k < b1; k++) sum += cnt[k];
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = tid > 0 ? part[tid - 1] : 0;
+    for (int k = b0; k < b1; k++) {
+        const int c = cnt[k];
+        bin_start[k] = run;
+        cnt[k] = run;  // from here on the insertion cursor of bin k
+        run += c;
+    }
+    if (tid == 1023) bin_start[nbins] = part[1023];
+    __syncthreads();
+    for (int64_t a = tid; a < natoms; a += 1024) bin_atoms[atomicAdd(&cnt[atom_bin[a]], 1)] = (int32_t)a;
 }
 
 __global__ void nl_bin_fill_kernel(int64_t natoms, const int32_t *__restrict__ atom_bin,
@@ -556,6 +614,16 @@ extern "C" int pxff_nlist_count(const double *d_pos, int64_t natoms, const int64
     PXFF_REQUIRE(nbins_total > 0, "nbins_total must be > 0");
     if (natoms <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
+    if (natoms <= NL_SMALL_ATOMS && nbins_total <= NL_SMALL_BINS) {
+        nl_bin_small_kernel<<<1, 1024, 0, s>>>(natoms, (int)nbins_total, d_pos, d_atom_struct, d_inv_cells, d_pbc,
+                                               d_geom, d_atom_bin, d_wrap, d_bin_start, d_bin_atoms);
+        PXFF_LAUNCH_CHECK();
+        nl_count_wave_kernel<<<(unsigned)natoms, 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom,
+                                                               rc * rc, d_bin_start, d_atom_bin, d_wrap, d_bin_atoms,
+                                                               d_img_count);
+        PXFF_LAUNCH_CHECK();
+        return 0;
+    }
     nl_zero_i32_kernel<<<div_up(nbins_total, 256), 256, 0, s>>>(d_bin_count, nbins_total);
     PXFF_LAUNCH_CHECK();
     nl_bin_kernel<<<div_up(natoms, 256), 256, 0, s>>>(natoms, d_pos, d_atom_struct, d_inv_cells, d_pbc, d_geom,

@@ -88,6 +88,29 @@ def test_nlist_batched_equals_single(oracle_mod):
     assert pos == nl.nimg
 
 
+def test_nlist_multi_kernel_binning_equals_single_workgroup():
+    """A batch with more bins than the single-workgroup binning takes (400
+    sparse 20 A boxes, > 8192 bins) is binned by the multi-kernel path; its
+    images equal those of the structures built one at a time (each binned
+    by one workgroup)."""
+    rng = np.random.default_rng(11)
+    cell = np.eye(3) * 20.0
+    pos = [rng.random((12, 3)) * 20.0 for _ in range(400)]
+    nl = DeviceNeighborList(pos, [cell] * 400, [[True] * 3] * 400, 4.9)
+    assert nl.builder.nbins > 8192
+    gi, gj, gT, gr = [t.cpu().numpy() for t in nl.images()]
+    off = at = 0
+    for p in pos:
+        one = DeviceNeighborList([p], [cell], [[True] * 3], 4.9)
+        ii, jj, TT, rr = [t.cpu().numpy() for t in one.images()]
+        n = len(ii)
+        assert np.array_equal(gi[at:at + n] - off, ii) and np.array_equal(gj[at:at + n] - off, jj)
+        assert np.array_equal(gT[at:at + n], TT) and np.array_equal(gr[at:at + n], rr)
+        at += n
+        off += len(p)
+    assert at == nl.nimg
+
+
 @pytest.mark.parametrize("lmax,rc", [(3, 4.9), (4, 4.9)])
 def test_snap_diamond64_parity(oracle_mod, lmax, rc):
     s = workloads.diamond_si(reps=2, sigma=0.05, seed=0)
