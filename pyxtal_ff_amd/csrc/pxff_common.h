@@ -47,6 +47,10 @@ __host__ __device__ inline cd cadd(cd a, cd b) { return cd{a.re + b.re, a.im + b
 __host__ __device__ inline cd csub(cd a, cd b) { return cd{a.re - b.re, a.im - b.im}; }
 __host__ __device__ inline cd cmul(cd a, cd b) { return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
 __host__ __device__ inline cd cscale(cd a, double s) { return cd{a.re * s, a.im * s}; }
+// a * b + c (complex multiply-add; four fused multiply-adds)
+__host__ __device__ inline cd cmadd(cd a, cd b, cd c) {
+    return cd{fma(a.re, b.re, fma(-a.im, b.im, c.re)), fma(a.re, b.im, fma(a.im, b.re, c.im))};
+}
 __host__ __device__ inline cd cconj(cd a) { return cd{a.re, -a.im}; }
 // Re(conj(a) * b)
 __host__ __device__ inline double cdotre(cd a, cd b) { return a.re * b.re + a.im * b.im; }

@@ -190,24 +190,26 @@ __device__ inline void row_step(int l, int q, const double *__restrict__ rpq, co
         if (m > l) continue;
         cd un = cmake(0.0, 0.0);
         cd dn[3] = {cmake(0, 0), cmake(0, 0), cmake(0, 0)};
+        // the coefficient is folded into one factor of each product (c1 ca,
+        // c1 u), so every derivative term is a complex multiply-add
         if (m < l) {
             double c1 = rpq[(l - m) * (TWOJ + 1) + q];
             cd c1a = cscale(ck.ca, c1);
             un = cmul(c1a, u[m]);
             if (GRAD) {
+                const cd c1u = cscale(u[m], c1);
 #pragma unroll
-                for (int k = 0; k < 3; k++)
-                    dn[k] = cscale(cadd(cmul(ck.cda[k], u[m]), cmul(ck.ca, du[k][m])), c1);
+                for (int k = 0; k < 3; k++) dn[k] = cmadd(c1a, du[k][m], cmul(ck.cda[k], c1u));
             }
         }
         if (m >= 1) {
             double c2 = rpq[m * (TWOJ + 1) + q];
             cd c2b = cscale(ck.cb, -c2);
-            un = cadd(un, cmul(c2b, u[m - 1]));
+            un = cmadd(c2b, u[m - 1], un);
             if (GRAD) {
+                const cd c2u = cscale(u[m - 1], -c2);
 #pragma unroll
-                for (int k = 0; k < 3; k++)
-                    dn[k] = cadd(dn[k], cscale(cadd(cmul(ck.cdb[k], u[m - 1]), cmul(ck.cb, du[k][m - 1])), -c2));
+                for (int k = 0; k < 3; k++) dn[k] = cmadd(c2b, du[k][m - 1], cmadd(ck.cdb[k], c2u, dn[k]));
             }
         }
         u[m] = un;
