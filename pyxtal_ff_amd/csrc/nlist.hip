@@ -119,8 +119,18 @@ __global__ __launch_bounds__(1024) void nl_bin_small_kernel(int64_t natoms, int 
     const int tid = threadIdx.x;
     for (int k = tid; k < nbins; k += 1024) cnt[k] = 0;
     __syncthreads();
-    for (int64_t a = tid; a < natoms; a += 1024)
-        atomicAdd(&cnt[nl_bin_of(a, pos, atom_struct, inv_cells, pbc, geom, atom_bin, wrap)], 1);
+    // four atoms per thread per step, their loads in flight together
+    for (int64_t a0 = tid; a0 < natoms; a0 += 4 * 1024) {
+        int gb[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t a = a0 + u * 1024;
+            gb[u] = a < natoms ? nl_bin_of(a, pos, atom_struct, inv_cells, pbc, geom, atom_bin, wrap) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (gb[u] >= 0) atomicAdd(&cnt[gb[u]], 1);
+    }
     __syncthreads();
     // exclusive scan: a contiguous chunk of bins per thread
     const int chunk = (nbins + 1023) / 1024;
@@ -144,7 +154,14 @@ __global__ __launch_bounds__(1024) void nl_bin_small_kernel(int64_t natoms, int 
     }
     if (tid == 1023) bin_start[nbins] = part[1023];
     __syncthreads();
-    for (int64_t a = tid; a < natoms; a += 1024) bin_atoms[atomicAdd(&cnt[atom_bin[a]], 1)] = (int32_t)a;
+    for (int64_t a0 = tid; a0 < natoms; a0 += 4 * 1024) {
+        int gb[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) gb[u] = a0 + u * 1024 < natoms ? atom_bin[a0 + u * 1024] : -1;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (gb[u] >= 0) bin_atoms[atomicAdd(&cnt[gb[u]], 1)] = (int32_t)(a0 + u * 1024);
+    }
 }
 
 __global__ void nl_bin_fill_kernel(int64_t natoms, const int32_t *__restrict__ atom_bin,
