@@ -304,6 +304,7 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
     __shared__ cd part[NP][NH];
     __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
     __shared__ double lnorm[TWOJ + 1];
+    __shared__ double stg[TANGENT ? 1 : 6][64];  // staged ca, cb, fc w, valid of 64 images
     const int lane = threadIdx.x;
     const int g = lane / G, mb = lane % G;
     for (int k = lane; k < (TWOJ + 1) * (TWOJ + 1); k += 64) rpq[k] = rootpq_g[k];
@@ -321,55 +322,100 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
         for (int k = lane; k < NP * NH; k += 64) (&part[0][0])[k] = cmake(0.0, 0.0);
         __syncthreads();
         const int64_t s = img_start[i], e = img_start[i + 1];
-        for (int64_t base = s; base < e; base += NG) {
-            const int64_t img = base + g;
-            bool valid = (g < NG) && (img < e);
-            double x = 0, y = 0, z = 0, r = 1.0, w = 0.0;
-            if (valid) {
-                x = img_r[3 * img];
-                y = img_r[3 * img + 1];
-                z = img_r[3 * img + 2];
-                r = sqrt(x * x + y * y + z * z);
-                w = atom_w[img_j[img]];
-                if (r < 1e-8) valid = false;  // SNAP.py:1487 (padding rows)
-            }
-            if (!valid) {
-                x = 0;
-                y = 0;
-                z = 1.0;
-                r = 1.0;
-            }
-            double fc, dfc;
-            cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
-            CK ck;
-            cayley_klein(x, y, z, r, rfac0 * M_PI * r / rcut, TANGENT, ck);
-            if (!TANGENT) {
-                const double sc = fc * w;
-                const int pidx = g < NG ? g % NP : 0;  // index the __shared__ array itself: LDS atomics
-                sweep_rows<TWOJ, false>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
-                    const bool act = valid && half;
-                    const int h0 = hoff_of(l) + mb * (l + 1);
+        if constexpr (!TANGENT) {
+            // up to 64 images at a time: their Cayley-Klein parameters and
+            // weights, one lane each, staged in LDS for the row sweeps (the G
+            // lanes of an image would otherwise each compute them)
+            for (int64_t b64 = s; b64 < e; b64 += 64) {
+                const int64_t e64 = b64 + 64 < e ? b64 + 64 : e;
+                {
+                    const int64_t img = b64 + lane;
+                    bool ok = img < e64;
+                    double x = 0, y = 0, z = 1.0, r = 1.0, w = 0.0;
+                    if (ok) {
+                        x = img_r[3 * img];
+                        y = img_r[3 * img + 1];
+                        z = img_r[3 * img + 2];
+                        r = sqrt(x * x + y * y + z * z);
+                        w = atom_w[img_j[img]];
+                        if (r < 1e-8) ok = false;  // SNAP.py:1487 (padding rows)
+                    }
+                    if (!ok) {
+                        x = 0;
+                        y = 0;
+                        z = 1.0;
+                        r = 1.0;
+                    }
+                    double fc, dfc;
+                    cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
+                    CK ck;
+                    cayley_klein(x, y, z, r, rfac0 * M_PI * r / rcut, false, ck);
+                    stg[0][lane] = ck.ca.re;
+                    stg[1][lane] = ck.ca.im;
+                    stg[2][lane] = ck.cb.re;
+                    stg[3][lane] = ck.cb.im;
+                    stg[4][lane] = ok ? fc * w : 0.0;
+                    stg[5][lane] = ok ? 1.0 : 0.0;
+                }
+                __syncthreads();
+                for (int64_t base = b64; base < e64; base += NG) {
+                    const int64_t img = base + g;
+                    const int si = (g < NG && img < e64) ? (int)(img - b64) : 0;
+                    const bool valid = (g < NG) && (img < e64) && stg[5][si] != 0.0;
+                    CK ck;
+                    ck.ca = cmake(stg[0][si], stg[1][si]);
+                    ck.cb = cmake(stg[2][si], stg[3][si]);
+                    const double sc = stg[4][si];
+                    const int pidx = g < NG ? g % NP : 0;  // index the __shared__ array itself: LDS atomics
+                    sweep_rows<TWOJ, false>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
+                        const bool act = valid && half;
+                        const int h0 = hoff_of(l) + mb * (l + 1);
 #pragma unroll
-                    for (int pass = 0; pass < (MODE == 2 ? 2 : 1); pass++) {
-                        // MODE 2: the two groups sharing an array add in
-                        // separate passes, ordered by workgroup barriers
-                        if (act && (MODE != 2 || (g >= NP) == (pass == 1))) {
+                        for (int pass = 0; pass < (MODE == 2 ? 2 : 1); pass++) {
+                            // MODE 2: the two groups sharing an array add in
+                            // separate passes, ordered by workgroup barriers
+                            if (act && (MODE != 2 || (g >= NP) == (pass == 1))) {
 #pragma unroll
-                            for (int m = 0; m <= TWOJ; m++) {
-                                if (m > l) continue;
-                                if (LDSADD) {
-                                    atomicAdd(&part[pidx][h0 + m].re, sc * u[m].re);
-                                    atomicAdd(&part[pidx][h0 + m].im, sc * u[m].im);
-                                } else {
-                                    cd v = part[pidx][h0 + m];
-                                    part[pidx][h0 + m] = cmake(v.re + sc * u[m].re, v.im + sc * u[m].im);
+                                for (int m = 0; m <= TWOJ; m++) {
+                                    if (m > l) continue;
+                                    if (LDSADD) {
+                                        atomicAdd(&part[pidx][h0 + m].re, sc * u[m].re);
+                                        atomicAdd(&part[pidx][h0 + m].im, sc * u[m].im);
+                                    } else {
+                                        cd v = part[pidx][h0 + m];
+                                        part[pidx][h0 + m] = cmake(v.re + sc * u[m].re, v.im + sc * u[m].im);
+                                    }
                                 }
                             }
+                            if (MODE == 2) __syncthreads();
                         }
-                        if (MODE == 2) __syncthreads();
-                    }
-                });
-            } else {
+                    });
+                }
+                __syncthreads();  // the next 64 images reuse the stage
+            }
+        } else {
+            for (int64_t base = s; base < e; base += NG) {
+                const int64_t img = base + g;
+                bool valid = (g < NG) && (img < e);
+                double x = 0, y = 0, z = 0, r = 1.0, w = 0.0;
+                if (valid) {
+                    x = img_r[3 * img];
+                    y = img_r[3 * img + 1];
+                    z = img_r[3 * img + 2];
+                    r = sqrt(x * x + y * y + z * z);
+                    w = atom_w[img_j[img]];
+                    if (r < 1e-8) valid = false;  // SNAP.py:1487 (padding rows)
+                }
+                if (!valid) {
+                    x = 0;
+                    y = 0;
+                    z = 1.0;
+                    r = 1.0;
+                }
+                double fc, dfc;
+                cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
+                CK ck;
+                cayley_klein(x, y, z, r, rfac0 * M_PI * r / rcut, TANGENT, ck);
                 double d0 = 0, d1 = 0, d2 = 0;
                 if (valid) {
                     d0 = dr[3 * img];
@@ -408,11 +454,16 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
         for (int h = lane; h < NH; h += 64) {
             cd acc = cmake(0.0, 0.0);
             for (int q = 0; q < NP; q++) acc = cadd(acc, part[q][h]);
-            // level / row / column of h
-            int l = 0;
-            while (l < TWOJ && h >= hoff_of(l + 1)) l++;
-            const int loc = h - hoff_of(l);
-            const int hm = loc / (l + 1), hma = loc % (l + 1);
+            // level / row / column of h (compile-time level offsets and
+            // divisors: compares and selects instead of loops and divisions)
+            int l = 0, hm = 0, hma = 0;
+#pragma unroll
+            for (int L = 0; L <= TWOJ; L++)
+                if (h >= hoff_of(L) && h < hoff_of(L + 1)) {
+                    l = L;
+                    hm = (h - hoff_of(L)) / (L + 1);
+                    hma = (h - hoff_of(L)) % (L + 1);
+                }
             if (!TANGENT && hm == hma) acc.re += fcut0 * site_w[i];  // addself_uarraytot
             acc = cscale(acc, lnorm[l]);                               // ulisttot *= u_norm
             if (TILED)  // [atom / 64][h][atom % 64], the layout of the Z kernel's loads
