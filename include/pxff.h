@@ -93,7 +93,9 @@ int pxff_nlist_fill(const double *d_pos, int64_t natoms, const int64_t *d_atom_s
  * the deterministic force gather relies on.  nimg may be an upper bound
  * (the image capacity): images at or beyond d_img_start[natoms] are
  * skipped.  A missing reverse (cannot happen for this construction) sets
- * d_flag[0] (int32, zeroed by the call). */
+ * d_flag[0] (int32, zeroed by the call); with d_flag NULL nothing is
+ * checked (one launch fewer; the call is deterministic, so a later call
+ * with a flag checks the same result). */
 int pxff_nlist_rev(int64_t natoms, int64_t nimg, const int64_t *d_img_start, const int64_t *d_img_atom,
                    const int64_t *d_img_j, const int32_t *d_img_T, int64_t *d_img_rev, int32_t *d_flag,
                    void *stream);
@@ -195,6 +197,12 @@ int pxff_so4_deidrj_counted(const pxff_so4_params *p, int64_t cap, const int64_t
  * d_virial [natoms*9] (or NULL) = per-centre sum_e r_e (x) dE/dr_e. */
 int pxff_so4_gather(int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_rev, const double *d_img_r,
                     const double *d_dedr, double *d_grad, double *d_virial, void *stream);
+
+/* pxff_so4_gather with the sign of the forces: d_forces [natoms*3] = F_i =
+ * -dE/dR_i (bit-identical to negating d_grad); d_virial as above.  The
+ * reference's forces are -dE/dR (neuralnetwork.py:491). */
+int pxff_so4_forces(int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_rev, const double *d_img_r,
+                    const double *d_dedr, double *d_forces, double *d_virial, void *stream);
 
 /* Forward-mode (tangent) of B along per-image displacements dr_e
  * [nimg*3]: dB [natoms*nb] = sum_e dB/dr_e . dr_e.  Used by the double

@@ -32,26 +32,34 @@ __device__ inline int floordiv_i(int a, int b) {
 //   r = (pos_j + ((T0 c0 + T1 c1) + T2 c2)) - pos_i       if (i,j,T) is "forward"
 //   r = -((pos_i + ((-T0 c0 - T1 c1) - T2 c2)) - pos_j)   otherwise,
 // forward meaning j > i, or j == i and T > 0 lexicographically.  All
-// operations are individually rounded (no contraction).
-__device__ inline void canon_sep(const double *__restrict__ pos, const double *__restrict__ cell, int64_t i, int64_t j,
-                                 int T0, int T1, int T2, double r[3]) {
+// operations are individually rounded (no contraction).  pi, pj: the two
+// positions, cl: the cell (rows = lattice vectors).
+__device__ inline void canon_sep_v(const double *pi, const double *pj, const double *cl, int64_t i, int64_t j, int T0,
+                                   int T1, int T2, double r[3]) {
     bool fwd;
     if (j != i)
         fwd = j > i;
     else
         fwd = (T0 > 0) || (T0 == 0 && T1 > 0) || (T0 == 0 && T1 == 0 && T2 > 0);
     const int s = fwd ? 1 : -1;
-    const int64_t a = fwd ? j : i, b = fwd ? i : j;
     const double t0 = (double)(s * T0), t1 = (double)(s * T1), t2 = (double)(s * T2);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        double t = __dmul_rn(t0, cell[k]);
-        t = __dadd_rn(t, __dmul_rn(t1, cell[3 + k]));
-        t = __dadd_rn(t, __dmul_rn(t2, cell[6 + k]));
-        double p = __dadd_rn(pos[3 * a + k], t);
-        double v = __dsub_rn(p, pos[3 * b + k]);
+        double t = __dmul_rn(t0, cl[k]);
+        t = __dadd_rn(t, __dmul_rn(t1, cl[3 + k]));
+        t = __dadd_rn(t, __dmul_rn(t2, cl[6 + k]));
+        const double pa = fwd ? pj[k] : pi[k], pb = fwd ? pi[k] : pj[k];
+        double p = __dadd_rn(pa, t);
+        double v = __dsub_rn(p, pb);
         r[k] = fwd ? v : -v;
     }
+}
+
+__device__ inline void canon_sep(const double *__restrict__ pos, const double *__restrict__ cell, int64_t i, int64_t j,
+                                 int T0, int T1, int T2, double r[3]) {
+    const double pi[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    const double pj[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
+    canon_sep_v(pi, pj, cell, i, j, T0, T1, T2, r);
 }
 
 __device__ inline double dist2(const double r[3]) {
@@ -236,17 +244,18 @@ __device__ inline bool key_less(int64_t ja, const int32_t *Ta, int64_t jb, const
     return Ta[2] < Tb[2];
 }
 
-__global__ void nl_fill_kernel(int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
-                               const double *__restrict__ cells, const int32_t *__restrict__ pbc,
-                               const int32_t *__restrict__ geom, double rc2, const int32_t *__restrict__ bin_start,
-                               const int32_t *__restrict__ atom_bin, const int32_t *__restrict__ wrap,
-                               const int32_t *__restrict__ bin_atoms, const int64_t *__restrict__ img_start,
-                               int64_t *__restrict__ img_atom, int64_t *__restrict__ img_j, int32_t *__restrict__ img_T,
-                               double *__restrict__ img_r, int64_t *__restrict__ row_count, int64_t min_count) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natoms) return;
+// Images of centre i by one thread, sorted in place in the output arrays
+// (insertion sort by (j, T)): the path of centres with more images than
+// the wavefront fill's LDS list holds.
+__device__ inline void nl_fill_serial(int64_t i, const double *__restrict__ pos,
+                                      const int64_t *__restrict__ atom_struct, const double *__restrict__ cells,
+                                      const int32_t *__restrict__ pbc, const int32_t *__restrict__ geom, double rc2,
+                                      const int32_t *__restrict__ bin_start, const int32_t *__restrict__ atom_bin,
+                                      const int32_t *__restrict__ wrap, const int32_t *__restrict__ bin_atoms,
+                                      const int64_t *__restrict__ img_start, int64_t *__restrict__ img_atom,
+                                      int64_t *__restrict__ img_j, int32_t *__restrict__ img_T,
+                                      double *__restrict__ img_r, int64_t *__restrict__ row_count) {
     const int64_t s0 = img_start[i], send = img_start[i + 1];
-    if (send - s0 < min_count) return;
     int64_t c = s0;
     nl_enumerate(i, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap, bin_atoms,
                  [&](int64_t j, int T0, int T1, int T2, const double *r) {
@@ -350,7 +359,7 @@ __global__ void nl_rev_kernel(int64_t natoms, int64_t nimg, const int64_t *__res
         img_rev[e] = lo;
     } else {
         img_rev[e] = e;  // cannot happen for a symmetric list; flagged
-        atomicOr(flag, 1);
+        if (flag) atomicOr(flag, 1);
     }
 }
 
@@ -379,11 +388,11 @@ __global__ void nl_row_rev_kernel(int64_t nrows, const int64_t *__restrict__ seq
 // One workgroup = one wavefront = one centre.  The stencil bins are taken 64
 // at a time: each lane owns one stencil bin, an exclusive wave prefix sum of
 // the bin sizes flattens the candidates (bin, slot), and every lane tests
-// one candidate per step (the owner of candidate c is the last bin whose
+// NL_U candidates per step (the owner of candidate c is the last bin whose
 // prefix is <= c).  Hits are compacted with a ballot into an LDS list; the
 // fill pass ranks the list by (j, T) (rank = number of smaller keys) and
-// writes it in order.  Centres with more than NL_KMAX images are left to
-// the thread-per-centre kernel above (nl_fill_kernel with a count floor).
+// writes it in order.  Centres with more than NL_KMAX images are filled by
+// one lane of their wavefront (nl_fill_serial).
 constexpr int NL_KMAX = 128;  // 5 KB of LDS per centre: many resident wavefronts
 
 struct NlHit {
@@ -391,6 +400,8 @@ struct NlHit {
     int32_t T[4];
     double r[3];
 };
+
+constexpr int NL_U = 4;  // candidate rounds per step, their loads in flight together
 
 template <bool FILL>
 __device__ inline int nl_wave_enumerate(int64_t i, int lane, const double *__restrict__ pos,
@@ -409,12 +420,16 @@ __device__ inline int nl_wave_enumerate(int64_t i, int lane, const double *__res
     const int bi0 = lb / (nb1 * nb2), bi1 = (lb / nb2) % nb1, bi2 = lb % nb2;
     const int ki0 = wrap[3 * i], ki1 = wrap[3 * i + 1], ki2 = wrap[3 * i + 2];
     const bool p0 = pbc[3 * s], p1 = pbc[3 * s + 1], p2 = pbc[3 * s + 2];
+    const double pi[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    double cl[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) cl[k] = cell[k];
     const int w0 = 2 * S0 + 1, w1 = 2 * S1 + 1, w2 = 2 * S2 + 1;
     const int nsten = w0 * w1 * w2;
     int count = 0;
     for (int sb = 0; sb < nsten; sb += 64) {
         const int st = sb + lane;
-        int gb = -1, t0 = 0, t1 = 0, t2 = 0, sz = 0;
+        int q0 = 0, t0 = 0, t1 = 0, t2 = 0, sz = 0;
         if (st < nsten) {
             const int d0 = st / (w1 * w2) - S0, d1 = (st / w2) % w1 - S1, d2 = st % w2 - S2;
             int tb0 = bi0 + d0, tb1 = bi1 + d1, tb2 = bi2 + d2;
@@ -423,8 +438,9 @@ __device__ inline int nl_wave_enumerate(int64_t i, int lane, const double *__res
             if (p1) { t1 = floordiv_i(tb1, nb1); tb1 -= t1 * nb1; } else { ok = ok && tb1 >= 0 && tb1 < nb1; }
             if (p2) { t2 = floordiv_i(tb2, nb2); tb2 -= t2 * nb2; } else { ok = ok && tb2 >= 0 && tb2 < nb2; }
             if (ok) {
-                gb = off + (tb0 * nb1 + tb1) * nb2 + tb2;
-                sz = bin_start[gb + 1] - bin_start[gb];
+                const int gb = off + (tb0 * nb1 + tb1) * nb2 + tb2;
+                q0 = bin_start[gb];
+                sz = bin_start[gb + 1] - q0;
             }
         }
         int incl = sz;
@@ -435,51 +451,67 @@ __device__ inline int nl_wave_enumerate(int64_t i, int lane, const double *__res
         const int total = __shfl(incl, 63);
         __syncthreads();  // previous chunk's readers are done with s_pref / s_info
         s_pref[lane] = incl - sz;
-        s_info[4 * lane] = gb;
+        s_info[4 * lane] = q0;
         s_info[4 * lane + 1] = t0;
         s_info[4 * lane + 2] = t1;
         s_info[4 * lane + 3] = t2;
         __syncthreads();
-        for (int cb = 0; cb < total; cb += 64) {
-            const int c = cb + lane;
-            bool hit = false;
-            int64_t j = 0;
-            int T0 = 0, T1 = 0, T2 = 0;
-            double r[3] = {0, 0, 0};
-            if (c < total) {
-                int lo = 0, hi = 63;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_pref[mid] <= c)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
+        for (int cb = 0; cb < total; cb += 64 * NL_U) {
+            // candidate c belongs to the last stencil bin whose prefix is
+            // <= c (s_pref is non-decreasing, s_pref[0] = 0); a fixed
+            // six-step search, the NL_U candidates of a lane interleaved
+            int own[NL_U], q[NL_U];
+            bool in[NL_U];
+#pragma unroll
+            for (int u = 0; u < NL_U; u++) {
+                const int c = cb + 64 * u + lane;
+                in[u] = c < total;
+                int lo = 0;
+#pragma unroll
+                for (int w = 32; w > 0; w >>= 1)
+                    if (s_pref[lo + w] <= c) lo += w;
+                own[u] = lo;
+                q[u] = s_info[4 * lo] + (c - s_pref[lo]);
+            }
+            int64_t j[NL_U];
+#pragma unroll
+            for (int u = 0; u < NL_U; u++) j[u] = in[u] ? (int64_t)bin_atoms[q[u]] : i;
+            int wj[NL_U][3];
+            double pj[NL_U][3];
+#pragma unroll
+            for (int u = 0; u < NL_U; u++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    wj[u][k] = wrap[3 * j[u] + k];
+                    pj[u][k] = pos[3 * j[u] + k];
                 }
-                const int g = s_info[4 * lo];
-                j = bin_atoms[bin_start[g] + (c - s_pref[lo])];
-                T0 = s_info[4 * lo + 1] - wrap[3 * j] + ki0;
-                T1 = s_info[4 * lo + 2] - wrap[3 * j + 1] + ki1;
-                T2 = s_info[4 * lo + 3] - wrap[3 * j + 2] + ki2;
-                if (!(j == i && T0 == 0 && T1 == 0 && T2 == 0)) {
-                    canon_sep(pos, cell, i, j, T0, T1, T2, r);
+#pragma unroll
+            for (int u = 0; u < NL_U; u++) {
+                const int T0 = s_info[4 * own[u] + 1] - wj[u][0] + ki0;
+                const int T1 = s_info[4 * own[u] + 2] - wj[u][1] + ki1;
+                const int T2 = s_info[4 * own[u] + 3] - wj[u][2] + ki2;
+                double r[3] = {0, 0, 0};
+                bool hit = false;
+                if (in[u] && !(j[u] == i && T0 == 0 && T1 == 0 && T2 == 0)) {
+                    canon_sep_v(pi, pj[u], cl, i, j[u], T0, T1, T2, r);
                     hit = dist2(r) < rc2;
                 }
-            }
-            const uint64_t mask = __ballot(hit);
-            if (FILL && hit) {
-                const int rank = count + (int)__popcll(mask & ((1ull << lane) - 1ull));
-                if (rank < NL_KMAX) {
-                    NlHit &h = hits[rank];
-                    h.j = j;
-                    h.T[0] = T0;
-                    h.T[1] = T1;
-                    h.T[2] = T2;
-                    h.r[0] = r[0];
-                    h.r[1] = r[1];
-                    h.r[2] = r[2];
+                const uint64_t mask = __ballot(hit);
+                if (FILL && hit) {
+                    const int rank = count + (int)__popcll(mask & ((1ull << lane) - 1ull));
+                    if (rank < NL_KMAX) {
+                        NlHit &h = hits[rank];
+                        h.j = j[u];
+                        h.T[0] = T0;
+                        h.T[1] = T1;
+                        h.T[2] = T2;
+                        h.r[0] = r[0];
+                        h.r[1] = r[1];
+                        h.r[2] = r[2];
+                    }
                 }
+                count += (int)__popcll(mask);
             }
-            count += (int)__popcll(mask);
         }
     }
     return count;
@@ -522,7 +554,13 @@ __global__ __launch_bounds__(64) void nl_fill_wave_kernel(
     if (i >= natoms) return;
     const int64_t s0 = img_start[i];
     const int k = (int)(img_start[i + 1] - s0);
-    if (k > NL_KMAX) return;  // handled by nl_fill_kernel (overflow pass)
+    if (k > NL_KMAX) {
+        // more images than the LDS list holds: one lane fills the centre
+        if (lane == 0)
+            nl_fill_serial(i, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap, bin_atoms, img_start,
+                           img_atom, img_j, img_T, img_r, row_count);
+        return;
+    }
     const int c = nl_wave_enumerate<true>(i, lane, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap,
                                           bin_atoms, s_pref, s_info, hits);
     (void)c;
@@ -671,12 +709,6 @@ extern "C" int pxff_nlist_fill(const double *d_pos, int64_t natoms, const int64_
                                                           d_bin_start, d_atom_bin, d_wrap, d_bin_atoms, d_img_start,
                                                           d_img_atom, d_img_j, d_img_T, d_img_r, d_row_count);
     PXFF_LAUNCH_CHECK();
-    // centres with more images than the LDS list holds
-    nl_fill_kernel<<<div_up(natoms, 64), 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom, rc * rc,
-                                                     d_bin_start, d_atom_bin, d_wrap, d_bin_atoms, d_img_start,
-                                                     d_img_atom, d_img_j, d_img_T, d_img_r, d_row_count,
-                                                     (int64_t)NL_KMAX + 1);
-    PXFF_LAUNCH_CHECK();
     return 0;
 }
 
@@ -685,8 +717,10 @@ extern "C" int pxff_nlist_rev(int64_t natoms, int64_t nimg, const int64_t *d_img
                               void *stream) {
     if (natoms <= 0 || nimg <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
-    nl_zero_i32_kernel<<<1, 64, 0, s>>>(d_flag, 1);
-    PXFF_LAUNCH_CHECK();
+    if (d_flag) {
+        nl_zero_i32_kernel<<<1, 64, 0, s>>>(d_flag, 1);
+        PXFF_LAUNCH_CHECK();
+    }
     nl_rev_kernel<<<div_up(nimg, 256), 256, 0, s>>>(natoms, nimg, d_img_start, d_img_atom, d_img_j, d_img_T, d_img_rev,
                                                     d_flag);
     PXFF_LAUNCH_CHECK();

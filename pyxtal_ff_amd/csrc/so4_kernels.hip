@@ -618,6 +618,9 @@ __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int
     const int g = lane / G, mb = lane % G;
     for (int k = tid; k < (TWOJ + 1) * (TWOJ + 1); k += blockDim.x) rpq[k] = rootpq_g[k];
     __syncthreads();
+    // a wavefront past the images in use (a launch over an image capacity)
+    // has nothing to do; the only later synchronisation is wavefront-local
+    if (((int64_t)blockIdx.x * wpb + wv) * NG >= nimg) return;
     const int64_t img = ((int64_t)blockIdx.x * wpb + wv) * NG + g;
     bool valid = (g < NG) && (img < nimg);
     double x = 0, yy = 0, zz = 1.0, r = 1.0, w = 0.0;
@@ -660,7 +663,10 @@ __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int
     red[wv][lane][1] = a0;
     red[wv][lane][2] = a1;
     red[wv][lane][3] = a2;
-    __syncthreads();
+    // the sums below read lanes of the same wavefront only
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (valid && mb == 0) {
         double s[4] = {0, 0, 0, 0};
         for (int q = 0; q < G; q++)
@@ -874,6 +880,9 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+// NEG: the forces -dE/dR_i, summed as (dedr_e - dedr_rev(e)) -- the exact
+// negation of the dE/dR sum, term by term in the same order.
+template <bool NEG>
 __global__ __launch_bounds__(256) void so4_gather_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
                                                           const int64_t *__restrict__ img_rev,
                                                           const double *__restrict__ img_r,
@@ -888,7 +897,7 @@ __global__ __launch_bounds__(256) void so4_gather_kernel(int64_t natoms, const i
         const int64_t q = img_rev[e];
         const double d[3] = {dedr[3 * e], dedr[3 * e + 1], dedr[3 * e + 2]};
 #pragma unroll
-        for (int k = 0; k < 3; k++) g[k] += dedr[3 * q + k] - d[k];
+        for (int k = 0; k < 3; k++) g[k] += NEG ? d[k] - dedr[3 * q + k] : dedr[3 * q + k] - d[k];
         if (virial) {
 #pragma unroll
             for (int a = 0; a < 3; a++) {
@@ -1142,8 +1151,19 @@ extern "C" int pxff_so4_gather(int64_t natoms, const int64_t *d_img_start, const
                                void *stream) {
     if (natoms <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
-    so4_gather_kernel<<<div_up(natoms, 4), 256, 0, s>>>(natoms, d_img_start, d_img_rev, d_img_r, d_dedr, d_grad,
-                                                        d_virial);
+    so4_gather_kernel<false><<<div_up(natoms, 4), 256, 0, s>>>(natoms, d_img_start, d_img_rev, d_img_r, d_dedr,
+                                                               d_grad, d_virial);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_so4_forces(int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_rev,
+                               const double *d_img_r, const double *d_dedr, double *d_forces, double *d_virial,
+                               void *stream) {
+    if (natoms <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    so4_gather_kernel<true><<<div_up(natoms, 4), 256, 0, s>>>(natoms, d_img_start, d_img_rev, d_img_r, d_dedr,
+                                                              d_forces, d_virial);
     PXFF_LAUNCH_CHECK();
     return 0;
 }

@@ -193,9 +193,12 @@ class DeviceNeighborList:
                                     p(bin_atoms), p(img_start), p(self.img_atom), p(self.img_j), p(self.img_T),
                                     p(self.img_r), p(self._row_count), s), "pxff_nlist_fill")
         self.img_rev = torch.empty(m, **i64)
-        self._flag = torch.empty(2, **i32)
+        # no symmetry flag here (one launch fewer per build); verify() redoes
+        # this deterministic phase with the flag
         nat.check(L.pxff_nlist_rev(n, self._cap, p(img_start), p(self.img_atom), p(self.img_j), p(self.img_T),
-                                   p(self.img_rev), p(self._flag), s), "pxff_nlist_rev")
+                                   p(self.img_rev), None, s), "pxff_nlist_rev")
+        self._flag = None
+        self._rev_checked = False
         self._rows_built = False
         self._shift = None
 
@@ -234,9 +237,20 @@ class DeviceNeighborList:
     def verify(self):
         """Raise if the reverse-image / reverse-row search failed (reads a
         device flag, i.e. synchronises)."""
+        self._ensure_flag()
+        if not self._rev_checked:
+            p = nat.ptr
+            nat.check(nat.lib().pxff_nlist_rev(self.natoms, self._cap, p(self.img_start), p(self.img_atom),
+                                               p(self.img_j), p(self.img_T), p(self.img_rev), p(self._flag),
+                                               nat.stream()), "pxff_nlist_rev")
+            self._rev_checked = True
         fl = int(self._flag.max().item())
         if fl != 0:
             raise nat.PxffError("neighbour list is not symmetric (flag %d)" % fl)
+
+    def _ensure_flag(self):
+        if self._flag is None:
+            self._flag = torch.zeros(2, dtype=torch.int32, device=self.pos.device)
 
     def _build_rows(self):
         if self._rows_built:
@@ -259,6 +273,7 @@ class DeviceNeighborList:
         self._img_row = torch.empty(m, **i64)
         self._nsite = torch.empty(max(n, 1), **i64)
         self._row_rev = torch.empty(r, **i64)
+        self._ensure_flag()
         flag = self._flag[1:2]
         nat.check(L.pxff_nlist_rows(n, self.nimg, nrows, p(self.img_start), p(self.img_j), p(row_start),
                                     p(self._seq), p(self._row_img), p(self._img_row), p(self._nsite),

@@ -197,11 +197,11 @@ def energy_forces_fused(nl, state, model, species_atom, virial=False):
     dedr = dev_so4.deidrj(nl, cfg, state.atom_w, y)
     out = {"energy": e_atom.sum(), "atom_energies": e_atom}
     if virial:
-        g, vir = dev_so4.gather(nl, dedr, virial=True)
+        f, vir = dev_so4.gather(nl, dedr, virial=True, forces=True)
         out["virial"] = vir.sum(0)
     else:
-        g = dev_so4.gather(nl, dedr)
-    out["forces"] = -g
+        f = dev_so4.gather(nl, dedr, forces=True)
+    out["forces"] = f
     return out
 
 
@@ -269,11 +269,11 @@ class FusedEFGraph:
         dedr = dev_so4.deidrj(nl, cfg, self.atom_w, y, counted=True)
         out = {"atom_energies": e_atom, "energy": _segment_total(e_atom, self._seg)[0]}
         if self.virial:
-            g, vir = dev_so4.gather(nl, dedr, virial=True)
+            f, vir = dev_so4.gather(nl, dedr, virial=True, forces=True)
             out["virial"] = _segment_total(vir.reshape(n, 9), self._seg).view(3, 3)
         else:
-            g = dev_so4.gather(nl, dedr)
-        out["forces"] = -g
+            f = dev_so4.gather(nl, dedr, forces=True)
+        out["forces"] = f
         self._nl = nl
         return out
 

@@ -130,16 +130,19 @@ def deidrj(nl, cfg, atom_w, y, img_r=None, counted=False):
     return out[: nl.nimg * 3].view(nl.nimg, 3)
 
 
-def gather(nl, dedr, virial=False, img_r=None):
-    """dE/dR per atom [N, 3] and (optionally) per-centre virial [N, 3, 3]."""
+def gather(nl, dedr, virial=False, img_r=None, forces=False):
+    """dE/dR per atom [N, 3] (with ``forces``: the forces -dE/dR, summed
+    with the opposite sign in the kernel) and (optionally) the per-centre
+    virial [N, 3, 3]."""
     p = nat.ptr
     dev = nl.pos.device
     grad = _f64(nl.natoms * 3, dev)
     vir = _f64(nl.natoms * 9, dev) if virial else None
     r = nl.img_r if img_r is None else img_r.contiguous()
     d = dedr.contiguous()
-    nat.check(nat.lib().pxff_so4_gather(nl.natoms, p(nl.img_start), p(nl.img_rev), p(r), p(d), p(grad), p(vir),
-                                        nat.stream()), "pxff_so4_gather")
+    fn = "pxff_so4_forces" if forces else "pxff_so4_gather"
+    nat.check(getattr(nat.lib(), fn)(nl.natoms, p(nl.img_start), p(nl.img_rev), p(r), p(d), p(grad), p(vir),
+                                     nat.stream()), fn)
     g = grad[: nl.natoms * 3].view(nl.natoms, 3)
     return (g, vir[: nl.natoms * 9].view(nl.natoms, 3, 3)) if virial else g
 

@@ -88,6 +88,32 @@ def test_nlist_batched_equals_single(oracle_mod):
     assert pos == nl.nimg
 
 
+def test_nlist_dense_centres_serial_path(oracle_mod):
+    """Centres with more images than the wavefront fill's LDS list (168 >
+    128: two atoms in a 2.5 A cube, rc 7) are filled by one lane; batched
+    with ordinary centres (diamond, skewed non-periodic dense cell), every
+    list is still bit-identical to the oracle's."""
+    dense = (np.array([[0.1, 0.2, 0.05], [1.3, 1.2, 1.31]]), np.eye(3) * 2.5, np.array([True] * 3))
+    skew = (dense[0], np.array([[2.5, 0, 0], [0.6, 2.4, 0], [0.3, -0.2, 2.6]]), np.array([True, True, False]))
+    d = workloads.diamond_si(reps=2, sigma=0.05, seed=1)
+    cases = [dense, (d.positions, d.cell, d.pbc), skew]
+    nl = DeviceNeighborList([c[0] for c in cases], [c[1] for c in cases], [c[2] for c in cases], 7.0)
+    gi, gj, gT, gr = [t.cpu().numpy() for t in nl.images()]
+    at = off = 0
+    for pos, cell, pbc in cases:
+        ii, jj, TT, rr = oracle_mod.nlist(pos, cell, pbc, 7.0)
+        n = len(ii)
+        _print("dense nlist case", len(pos), "images", n, "max per centre", np.bincount(ii).max())
+        assert np.array_equal(gi[at:at + n] - off, ii) and np.array_equal(gj[at:at + n] - off, jj)
+        assert np.array_equal(gT[at:at + n], TT) and np.array_equal(gr[at:at + n], rr)
+        at += n
+        off += len(pos)
+    assert at == nl.nimg
+    assert np.bincount(gi).max() > 128
+    rev = nl.img_rev[: nl.nimg].cpu().numpy()
+    assert np.array_equal(gr[rev], -gr) and np.array_equal(gj[rev], gi)
+
+
 def test_nlist_multi_kernel_binning_equals_single_workgroup():
     """A batch with more bins than the single-workgroup binning takes (400
     sparse 20 A boxes, > 8192 bins) is binned by the multi-kernel path; its
@@ -211,6 +237,19 @@ def test_adjoint_forces_match_plain_dxdr():
         W = a["virial"].cpu().numpy()
         sv = -np.array([W[0, 0], W[1, 1], W[2, 2], W[0, 1], W[0, 2], W[1, 2]]) / V
         assert rel_err(sv, p["stress_voigt"].cpu().numpy()) < 1e-9
+
+
+def test_forces_entry_is_exact_negation_of_gather():
+    """pxff_so4_forces (F = -dE/dR summed with the opposite sign) equals the
+    negated pxff_so4_gather bit for bit; the virial is the same."""
+    s = workloads.diamond_si(reps=2, sigma=0.08, seed=3)
+    nl = DeviceNeighborList([s.positions], [s.cell], [s.pbc], 4.9)
+    g = torch.Generator(device="cpu").manual_seed(4)
+    dedr = torch.randn(nl.nimg, 3, generator=g, dtype=torch.float64).cuda()
+    grad, vir = dso4.gather(nl, dedr, virial=True)
+    f, vir2 = dso4.gather(nl, dedr, virial=True, forces=True)
+    assert torch.equal(f, -grad) and torch.equal(vir, vir2)
+    nl.verify()  # the reverse images the gather used are complete
 
 
 def test_adjoint_energy_central_fd():
