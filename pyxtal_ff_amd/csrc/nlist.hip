@@ -106,20 +106,27 @@ __global__ void nl_bin_kernel(int64_t natoms, const double *__restrict__ pos, co
     atomicAdd(&bin_count[nl_bin_of(a, pos, atom_struct, inv_cells, pbc, geom, atom_bin, wrap)], 1);
 }
 
-// The whole binning of a small list in one workgroup of 1024 threads, with
-// the bin counters in LDS: atom bins and wraps, bin sizes, their exclusive
-// scan (bin_start, nbins + 1 entries) and the atoms of every bin -- one
-// launch instead of five.  The order of the atoms inside a bin is not
-// fixed (LDS atomics), as with the multi-kernel path; the image lists do
-// not depend on it (their fill ranks images by (j, T)).
+// Bin and lattice wrap of every atom (no counting): the parallel half of
+// the small-list binning.
+__global__ void nl_bin_place_kernel(int64_t natoms, const double *__restrict__ pos,
+                                    const int64_t *__restrict__ atom_struct, const double *__restrict__ inv_cells,
+                                    const int32_t *__restrict__ pbc, const int32_t *__restrict__ geom,
+                                    int32_t *__restrict__ atom_bin, int32_t *__restrict__ wrap) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms) return;
+    nl_bin_of(a, pos, atom_struct, inv_cells, pbc, geom, atom_bin, wrap);
+}
+
+// The counting half of a small list's binning in one workgroup of 1024
+// threads, with the bin counters in LDS: bin sizes from the atoms' bins,
+// their exclusive scan (bin_start, nbins + 1 entries) and the atoms of
+// every bin -- two launches instead of five.  The order of the atoms
+// inside a bin is not fixed (LDS atomics), as with the multi-kernel path;
+// the image lists do not depend on it (their fill ranks images by (j, T)).
 constexpr int NL_SMALL_BINS = 8192;
 constexpr int NL_SMALL_ATOMS = 1 << 16;
-__global__ __launch_bounds__(1024) void nl_bin_small_kernel(int64_t natoms, int nbins, const double *__restrict__ pos,
-                                                             const int64_t *__restrict__ atom_struct,
-                                                             const double *__restrict__ inv_cells,
-                                                             const int32_t *__restrict__ pbc,
-                                                             const int32_t *__restrict__ geom,
-                                                             int32_t *__restrict__ atom_bin, int32_t *__restrict__ wrap,
+__global__ __launch_bounds__(1024) void nl_bin_small_kernel(int64_t natoms, int nbins,
+                                                             const int32_t *__restrict__ atom_bin,
                                                              int32_t *__restrict__ bin_start,
                                                              int32_t *__restrict__ bin_atoms) {
     __shared__ int cnt[NL_SMALL_BINS];
@@ -131,10 +138,7 @@ __global__ __launch_bounds__(1024) void nl_bin_small_kernel(int64_t natoms, int 
     for (int64_t a0 = tid; a0 < natoms; a0 += 4 * 1024) {
         int gb[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int64_t a = a0 + u * 1024;
-            gb[u] = a < natoms ? nl_bin_of(a, pos, atom_struct, inv_cells, pbc, geom, atom_bin, wrap) : -1;
-        }
+        for (int u = 0; u < 4; u++) gb[u] = a0 + u * 1024 < natoms ? atom_bin[a0 + u * 1024] : -1;
 #pragma unroll
         for (int u = 0; u < 4; u++)
             if (gb[u] >= 0) atomicAdd(&cnt[gb[u]], 1);
@@ -393,7 +397,7 @@ __global__ void nl_row_rev_kernel(int64_t nrows, const int64_t *__restrict__ seq
 // fill pass ranks the list by (j, T) (rank = number of smaller keys) and
 // writes it in order.  Centres with more than NL_KMAX images are filled by
 // one lane of their wavefront (nl_fill_serial).
-constexpr int NL_KMAX = 128;  // 5 KB of LDS per centre: many resident wavefronts
+constexpr int NL_KMAX = 128;  // 7 KB of LDS per centre: many resident wavefronts
 
 struct NlHit {
     int64_t j;
@@ -549,6 +553,7 @@ __global__ __launch_bounds__(64) void nl_fill_wave_kernel(
     __shared__ int s_pref[64];
     __shared__ int s_info[256];
     __shared__ NlHit hits[NL_KMAX];
+    __shared__ uint64_t keys[NL_KMAX];
     const int lane = threadIdx.x;
     const int64_t i = blockIdx.x;
     if (i >= natoms) return;
@@ -565,17 +570,41 @@ __global__ __launch_bounds__(64) void nl_fill_wave_kernel(
                                           bin_atoms, s_pref, s_info, hits);
     (void)c;
     __syncthreads();
+    // sort keys (j, T0, T1, T2) packed in 64 bits, T_d + 128 in 8 bits each,
+    // when every |T_d| < 128 (the lexicographic order of the packed key is
+    // that of hit_less); otherwise the comparison of the fields
+    bool wide = false;
+    for (int e = lane; e < k; e += 64) {
+        const NlHit &h = hits[e];
+        wide = wide || h.T[0] < -128 || h.T[0] > 127 || h.T[1] < -128 || h.T[1] > 127 || h.T[2] < -128 ||
+               h.T[2] > 127;
+        keys[e] = ((uint64_t)h.j << 24) | ((uint64_t)(h.T[0] + 128) << 16) | ((uint64_t)(h.T[1] + 128) << 8) |
+                  (uint64_t)(h.T[2] + 128);
+    }
+    const bool packed = __ballot(wide) == 0;
+    __syncthreads();
     int firsts = 0;
     bool self = false;
     for (int e = lane; e < k; e += 64) {
         const NlHit h = hits[e];
         int rank = 0;
         bool first = true;
-        for (int q = 0; q < k; q++) {
-            const NlHit &o = hits[q];
-            if (hit_less(o, h)) {
-                rank++;
-                if (o.j == h.j) first = false;
+        if (packed) {
+            const uint64_t mk = keys[e];
+#pragma unroll 4
+            for (int q = 0; q < k; q++) {
+                const uint64_t o = keys[q];
+                const bool lt = o < mk;
+                rank += lt ? 1 : 0;
+                if (lt && (o >> 24) == (mk >> 24)) first = false;
+            }
+        } else {
+            for (int q = 0; q < k; q++) {
+                const NlHit &o = hits[q];
+                if (hit_less(o, h)) {
+                    rank++;
+                    if (o.j == h.j) first = false;
+                }
             }
         }
         if (rank >= k) continue;  // the range was clamped to the image capacity
@@ -670,8 +699,10 @@ extern "C" int pxff_nlist_count(const double *d_pos, int64_t natoms, const int64
     if (natoms <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
     if (natoms <= NL_SMALL_ATOMS && nbins_total <= NL_SMALL_BINS) {
-        nl_bin_small_kernel<<<1, 1024, 0, s>>>(natoms, (int)nbins_total, d_pos, d_atom_struct, d_inv_cells, d_pbc,
-                                               d_geom, d_atom_bin, d_wrap, d_bin_start, d_bin_atoms);
+        nl_bin_place_kernel<<<div_up(natoms, 256), 256, 0, s>>>(natoms, d_pos, d_atom_struct, d_inv_cells, d_pbc,
+                                                                d_geom, d_atom_bin, d_wrap);
+        PXFF_LAUNCH_CHECK();
+        nl_bin_small_kernel<<<1, 1024, 0, s>>>(natoms, (int)nbins_total, d_atom_bin, d_bin_start, d_bin_atoms);
         PXFF_LAUNCH_CHECK();
         nl_count_wave_kernel<<<(unsigned)natoms, 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom,
                                                                rc * rc, d_bin_start, d_atom_bin, d_wrap, d_bin_atoms,

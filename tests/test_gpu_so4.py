@@ -114,6 +114,22 @@ def test_nlist_dense_centres_serial_path(oracle_mod):
     assert np.array_equal(gr[rev], -gr) and np.array_equal(gj[rev], gi)
 
 
+def test_nlist_unwrapped_positions_large_translations(oracle_mod):
+    """Every fifth atom carried 200 c - 150 a away (unwrapped MD coordinates):
+    image translations reach |T| = 201, beyond the packed 8-bit sort key, so
+    the fill compares the key fields; the list is still bit-identical."""
+    s = workloads.diamond_si(reps=2, sigma=0.05, seed=5)
+    pos = s.positions.copy()
+    pos[::5] += 200 * s.cell[2] - 150 * s.cell[0]
+    nl = DeviceNeighborList([pos], [s.cell], [s.pbc], 4.9)
+    ii, jj, TT, rr = oracle_mod.nlist(pos, s.cell, s.pbc, 4.9)
+    gi, gj, gT, gr = [t.cpu().numpy() for t in nl.images()]
+    assert np.abs(TT).max() > 128
+    assert np.array_equal(gi, ii) and np.array_equal(gj, jj) and np.array_equal(gT, TT)
+    assert np.array_equal(gr, rr)
+    nl.verify()
+
+
 def test_nlist_multi_kernel_binning_equals_single_workgroup():
     """A batch with more bins than the single-workgroup binning takes (400
     sparse 20 A boxes, > 8192 bins) is binned by the multi-kernel path; its
