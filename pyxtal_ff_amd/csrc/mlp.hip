@@ -88,13 +88,13 @@ struct WaveSync {
 };
 
 // One atom on one wavefront: lane n holds neuron n of a layer; h0 [nin],
-// dact / hout [MLP_MAXL][MLP_MAXW] are the wavefront's LDS scratch.
+// dact / hout [layers][hw] (hw = the widest layer) are the wavefront's LDS scratch.
 template <typename Sync>
 __device__ inline void mlp_atom(int64_t i, int lane, int nin, int nelem, const int64_t *__restrict__ species,
                                 const int *__restrict__ layout, const double *params, const double *__restrict__ dmin,
                                 const double *__restrict__ scale, const double *__restrict__ x,
                                 double *__restrict__ e_out, double *__restrict__ beta, double *h0, double *dact,
-                                double *hout, Sync sync) {
+                                double *hout, int hw, Sync sync) {
     const int64_t el64 = species[i];
     if (el64 < 0 || el64 >= nelem) {  // no network for this atom: NaN results
         if (lane == 0) e_out[i] = __builtin_nan("");
@@ -115,16 +115,24 @@ __device__ inline void mlp_atom(int64_t i, int lane, int nin, int nelem, const i
         const int ni = L[0], no = L[1], act = L[2];
         const double *WT = params + L[5], *b = params + L[4];
         if (lane < no) {
-            double z = b[lane];
-#pragma unroll 8
-            for (int j = 0; j < ni; j++) z += WT[(size_t)j * no + lane] * hin[j];
+            // four partial sums: four independent FMA chains
+            double z0 = b[lane], z1 = 0.0, z2 = 0.0, z3 = 0.0;
+            int j = 0;
+            for (; j + 4 <= ni; j += 4) {
+                z0 += WT[(size_t)j * no + lane] * hin[j];
+                z1 += WT[(size_t)(j + 1) * no + lane] * hin[j + 1];
+                z2 += WT[(size_t)(j + 2) * no + lane] * hin[j + 2];
+                z3 += WT[(size_t)(j + 3) * no + lane] * hin[j + 3];
+            }
+            for (; j < ni; j++) z0 += WT[(size_t)j * no + lane] * hin[j];
+            const double z = (z0 + z1) + (z2 + z3);
             double a, da;
             act_fdf(act, z, a, da);
-            hout[k * MLP_MAXW + lane] = a;
-            dact[k * MLP_MAXW + lane] = da;
+            hout[k * hw + lane] = a;
+            dact[k * hw + lane] = da;
         }
         sync();
-        hin = hout + k * MLP_MAXW;
+        hin = hout + k * hw;
     }
     if (lane == 0) e_out[i] = hin[0];
     // backward: dact[k] becomes the pre-activation gradient of layer k; for
@@ -135,10 +143,17 @@ __device__ inline void mlp_atom(int64_t i, int lane, int nin, int nelem, const i
         const double *W = params + L[3];
         // dE/dh_{k-1}[j] = sum_n W[n][j] g_pre[n]; then times act'_{k-1}
         if (lane < ni) {
-            double g = 0.0;
-#pragma unroll 8
-            for (int n = 0; n < no; n++) g += W[(size_t)n * ni + lane] * dact[k * MLP_MAXW + n];
-            dact[(k - 1) * MLP_MAXW + lane] *= g;
+            const double *gk = dact + k * hw;
+            double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+            int n = 0;
+            for (; n + 4 <= no; n += 4) {
+                g0 += W[(size_t)n * ni + lane] * gk[n];
+                g1 += W[(size_t)(n + 1) * ni + lane] * gk[n + 1];
+                g2 += W[(size_t)(n + 2) * ni + lane] * gk[n + 2];
+                g3 += W[(size_t)(n + 3) * ni + lane] * gk[n + 3];
+            }
+            for (; n < no; n++) g0 += W[(size_t)n * ni + lane] * gk[n];
+            dact[(k - 1) * hw + lane] *= (g0 + g1) + (g2 + g3);
         }
         sync();
     }
@@ -147,10 +162,16 @@ __device__ inline void mlp_atom(int64_t i, int lane, int nin, int nelem, const i
         const int ni = L[0], no = L[1];
         const double *W = params + L[3];
         for (int j = lane; j < ni; j += 64) {
-            double g = 0.0;
-#pragma unroll 8
-            for (int n = 0; n < no; n++) g += W[(size_t)n * ni + j] * dact[n];
-            beta[(size_t)i * nin + j] = g * sc[j];
+            double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+            int n = 0;
+            for (; n + 4 <= no; n += 4) {
+                g0 += W[(size_t)n * ni + j] * dact[n];
+                g1 += W[(size_t)(n + 1) * ni + j] * dact[n + 1];
+                g2 += W[(size_t)(n + 2) * ni + j] * dact[n + 2];
+                g3 += W[(size_t)(n + 3) * ni + j] * dact[n + 3];
+            }
+            for (; n < no; n++) g0 += W[(size_t)n * ni + j] * dact[n];
+            beta[(size_t)i * nin + j] = ((g0 + g1) + (g2 + g3)) * sc[j];
         }
     }
     sync();  // the next atom reuses the scratch
@@ -165,13 +186,13 @@ __global__ __launch_bounds__(64) void mlp_energy_grad_kernel(int64_t natoms, int
                                                               const double *__restrict__ dmin,
                                                               const double *__restrict__ scale,
                                                               const double *__restrict__ x, double *__restrict__ e_out,
-                                                              double *__restrict__ beta) {
+                                                              double *__restrict__ beta, int nlm, int hw) {
     extern __shared__ double sm[];
     double *h0 = sm;
     double *dact = h0 + nin;
-    double *hout = dact + MLP_MAXL * MLP_MAXW;
+    double *hout = dact + nlm * hw;
     mlp_atom(blockIdx.x, threadIdx.x, nin, nelem, species, layout, params, dmin, scale, x, e_out, beta, h0, dact, hout,
-             WgSync());
+             hw, WgSync());
 }
 
 // Persistent workgroups of MLP_LW wavefronts: the parameters of every
@@ -186,17 +207,19 @@ __global__ __launch_bounds__(64 * MLP_LW) void mlp_lds_kernel(int64_t natoms, in
                                                                const double *__restrict__ dmin,
                                                                const double *__restrict__ scale,
                                                                const double *__restrict__ x,
-                                                               double *__restrict__ e_out, double *__restrict__ beta) {
+                                                               double *__restrict__ e_out, double *__restrict__ beta,
+                                                               int nlm, int hw) {
     extern __shared__ double sm[];
     double *P = sm;
     for (int k = threadIdx.x; k < nparams; k += blockDim.x) P[k] = params[k];
     __syncthreads();
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double *h0 = sm + nparams + (size_t)wv * (nin + 2 * MLP_MAXL * MLP_MAXW);
+    double *h0 = sm + nparams + (size_t)wv * (nin + 2 * nlm * hw);
     double *dact = h0 + nin;
-    double *hout = dact + MLP_MAXL * MLP_MAXW;
+    double *hout = dact + nlm * hw;
     for (int64_t i = (int64_t)blockIdx.x * MLP_LW + wv; i < natoms; i += (int64_t)gridDim.x * MLP_LW)
-        mlp_atom(i, lane, nin, nelem, species, layout, P, dmin, scale, x, e_out, beta, h0, dact, hout, WaveSync());
+        mlp_atom(i, lane, nin, nelem, species, layout, P, dmin, scale, x, e_out, beta, h0, dact, hout, hw,
+                 WaveSync());
 }
 
 }  // namespace pxff
@@ -235,7 +258,15 @@ extern "C" int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const in
             nparams = std::max({nparams, L[3] + nw, (int64_t)L[4] + L[1], L[5] + nw});
         }
     }
-    const size_t scratch = sizeof(double) * ((size_t)nin + 2 * MLP_MAXL * MLP_MAXW);
+    // scratch per wavefront: the scaled input, then activations and their
+    // derivatives of every layer at a stride of the widest layer
+    int nlm = 1, hw = 1;
+    for (int e = 0; e < nelem; e++) {
+        const int *lay = h_layout + e * MLP_STRIDE;
+        nlm = std::max(nlm, lay[0]);
+        for (int k = 0; k < lay[0]; k++) hw = std::max(hw, lay[1 + MLP_LSTRIDE * k + 1]);
+    }
+    const size_t scratch = sizeof(double) * ((size_t)nin + 2 * (size_t)nlm * hw);
     const size_t ldsp = sizeof(double) * (size_t)nparams + MLP_LW * scratch;
     if (ldsp <= 160 * 1024) {  // the weights fit in LDS next to the scratch
         static int attr_set = 0;
@@ -244,9 +275,12 @@ extern "C" int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const in
                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             attr_set = 1;
         }
-        const int64_t nwg = std::min<int64_t>((natoms + MLP_LW - 1) / MLP_LW, (int64_t)num_cus() * 2);
+        // as many resident workgroups as the LDS holds (at most 4 per CU)
+        const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)(160 * 1024) / (int64_t)ldsp));
+        const int64_t nwg = std::min<int64_t>((natoms + MLP_LW - 1) / MLP_LW, (int64_t)num_cus() * per_cu);
         mlp_lds_kernel<<<(unsigned)nwg, 64 * MLP_LW, ldsp, s>>>(natoms, nin, nelem, d_species, d_layout, d_params,
-                                                               (int)nparams, d_dmin, d_scale, d_x, d_e, d_beta);
+                                                               (int)nparams, d_dmin, d_scale, d_x, d_e, d_beta, nlm,
+                                                               hw);
         PXFF_LAUNCH_CHECK();
         return 0;
     }
@@ -256,7 +290,7 @@ extern "C" int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const in
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     PXFF_REQUIRE(natoms <= 0x7fffffff, "too many atoms for one launch");
     mlp_energy_grad_kernel<<<(unsigned)natoms, 64, lds, s>>>(natoms, nin, nelem, d_species, d_layout, d_params, d_dmin,
-                                                               d_scale, d_x, d_e, d_beta);
+                                                               d_scale, d_x, d_e, d_beta, nlm, hw);
     PXFF_LAUNCH_CHECK();
     return 0;
 }
