@@ -39,7 +39,8 @@ extern "C" {
  *   PXFF_TUNE_IMAGE_WAVES_PER_BLOCK wavefronts per workgroup of the per-image adjoint kernel (1, 2, 4)
  *   PXFF_TUNE_ACCUM_LDSADD          per-centre U_tot sums: 0 = per-group LDS partials, 1 = one LDS array
  *                                   with LDS atomic adds (issued by one wavefront in a fixed order),
- *                                   2 = two groups per partial array, added in two ordered passes */
+ *                                   2 (3) = two (three) groups per partial array, added in two
+ *                                   (three) ordered passes */
 #define PXFF_TUNE_Z_CHUNKS 0
 #define PXFF_TUNE_ACCUM_WAVES_PER_CU 1
 #define PXFF_TUNE_IMAGE_WAVES_PER_BLOCK 2

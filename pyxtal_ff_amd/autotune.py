@@ -21,7 +21,7 @@ from . import so4 as dso4
 
 Z_CHUNKS, ACCUM_WAVES_PER_CU, IMAGE_WAVES_PER_BLOCK, ACCUM_LDSADD = 0, 1, 2, 3
 CANDIDATES = {Z_CHUNKS: (4, 8, 16, 24, 32, 48, 64), ACCUM_WAVES_PER_CU: (4, 8, 16, 32, 64),
-              IMAGE_WAVES_PER_BLOCK: (1, 2, 4), ACCUM_LDSADD: (0, 1, 2)}
+              IMAGE_WAVES_PER_BLOCK: (1, 2, 4), ACCUM_LDSADD: (0, 1, 2, 3)}
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "pyxtal_ff_amd", "tune.json")
 
 

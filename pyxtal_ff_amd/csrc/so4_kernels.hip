@@ -268,6 +268,10 @@ __device__ inline void sweep_rows(int mb, const double *__restrict__ rpq, const 
     f(0, u, du, mb == 0);
 #pragma unroll
     for (int l = 1; l <= TWOJ; l++) {
+        // without derivative rows: keep the coefficient loads of a level
+        // inside it (hoisted loads of later levels would hold registers
+        // across the whole sweep)
+        if (!GRAD) asm volatile("" ::: "memory");
         const int mbe = (l <= mb) ? 0 : mb;
         row_step<TWOJ, GRAD>(l, l - mbe, rpq, ck, u, du);
         if (l == mb) row_flip<TWOJ, GRAD>(l, u, du);
@@ -280,8 +284,10 @@ __device__ inline void sweep_rows(int mb, const double *__restrict__ rpq, const 
 // image per round; per-group partial sums live in LDS and are reduced in a
 // fixed order at the end (deterministic).  With TANGENT the kernel instead
 // accumulates dU_tot . dr_e (forward-mode derivative of U_tot).
+// The U_tot sums hold no derivative rows: registers are kept within three
+// wavefronts per SIMD (the tangent variant needs more than that).
 template <int TWOJ, bool TANGENT, bool TILED, int MODE>
-__global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TANGENT ? 1 : 3))) void so4_accum_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
                                                         const int64_t *__restrict__ img_j,
                                                         const double *__restrict__ img_r,
                                                         const double *__restrict__ atom_w,
@@ -296,15 +302,16 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
     // MODE 0: each group keeps its own partial array, summed in group order
     // at the end.  MODE 1: all groups add into one LDS array (ds_add_f64;
     // one wavefront per centre issues every add, so the order is fixed).
-    // MODE 2: groups g and g + NP share partial array g; the first half of
-    // the groups adds, then the second (two masked passes of one wavefront,
-    // in program order): half the LDS, so twice the resident wavefronts.
+    // MODE 2 / 3: MODE groups share each partial array (groups g, g + NP,
+    // g + 2 NP, ...); they add in MODE masked passes of one wavefront, in
+    // program order: 1/MODE of the LDS, so more resident wavefronts.
     constexpr bool LDSADD = MODE == 1;
-    constexpr int NP = MODE == 1 ? 1 : (MODE == 2 ? (NG + 1) / 2 : NG);
+    constexpr int NPASS = MODE >= 2 ? MODE : 1;
+    constexpr int NP = MODE == 1 ? 1 : (NG + NPASS - 1) / NPASS;
     __shared__ cd part[NP][NH];
     __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
     __shared__ double lnorm[TWOJ + 1];
-    __shared__ double stg[TANGENT ? 1 : 6][64];  // staged ca, cb, fc w, valid of 64 images
+    __shared__ double stg[TANGENT ? 1 : 5][64];  // staged ca, cb, fc w (0: no image) of 64 images
     const int lane = threadIdx.x;
     const int g = lane / G, mb = lane % G;
     for (int k = lane; k < (TWOJ + 1) * (TWOJ + 1); k += 64) rpq[k] = rootpq_g[k];
@@ -354,14 +361,13 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
                     stg[1][lane] = ck.ca.im;
                     stg[2][lane] = ck.cb.re;
                     stg[3][lane] = ck.cb.im;
-                    stg[4][lane] = ok ? fc * w : 0.0;
-                    stg[5][lane] = ok ? 1.0 : 0.0;
+                    stg[4][lane] = ok ? fc * w : 0.0;  // a zero weight adds nothing either way
                 }
                 __syncthreads();
                 for (int64_t base = b64; base < e64; base += NG) {
                     const int64_t img = base + g;
                     const int si = (g < NG && img < e64) ? (int)(img - b64) : 0;
-                    const bool valid = (g < NG) && (img < e64) && stg[5][si] != 0.0;
+                    const bool valid = (g < NG) && (img < e64) && stg[4][si] != 0.0;
                     CK ck;
                     ck.ca = cmake(stg[0][si], stg[1][si]);
                     ck.cb = cmake(stg[2][si], stg[3][si]);
@@ -371,10 +377,10 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
                         const bool act = valid && half;
                         const int h0 = hoff_of(l) + mb * (l + 1);
 #pragma unroll
-                        for (int pass = 0; pass < (MODE == 2 ? 2 : 1); pass++) {
-                            // MODE 2: the two groups sharing an array add in
+                        for (int pass = 0; pass < NPASS; pass++) {
+                            // MODE >= 2: the groups sharing an array add in
                             // separate passes, ordered by workgroup barriers
-                            if (act && (MODE != 2 || (g >= NP) == (pass == 1))) {
+                            if (act && (NPASS == 1 || g / NP == pass)) {
 #pragma unroll
                                 for (int m = 0; m <= TWOJ; m++) {
                                     if (m > l) continue;
@@ -387,7 +393,7 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
                                     }
                                 }
                             }
-                            if (MODE == 2) __syncthreads();
+                            if (NPASS > 1) __syncthreads();
                         }
                     });
                 }
@@ -429,8 +435,8 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
                     const bool act = valid && half;
                     const int h0 = hoff_of(l) + mb * (l + 1);
 #pragma unroll
-                    for (int pass = 0; pass < (MODE == 2 ? 2 : 1); pass++) {
-                        if (act && (MODE != 2 || (g >= NP) == (pass == 1))) {
+                    for (int pass = 0; pass < NPASS; pass++) {
+                        if (act && (NPASS == 1 || g / NP == pass)) {
 #pragma unroll
                             for (int m = 0; m <= TWOJ; m++) {
                                 if (m > l) continue;
@@ -445,7 +451,7 @@ __global__ __launch_bounds__(64) void so4_accum_kernel(int64_t natoms, const int
                                 }
                             }
                         }
-                        if (MODE == 2) __syncthreads();
+                        if (NPASS > 1) __syncthreads();
                     }
                 });
             }
@@ -1011,7 +1017,7 @@ namespace pxff {
 // Process-wide launch parameters set by the autotuner (pxff_tune_set).
 static int g_tune[PXFF_TUNE_COUNT] = {16, 16, 4, 0};
 static const int g_tune_lo[PXFF_TUNE_COUNT] = {1, 1, 1, 0};
-static const int g_tune_hi[PXFF_TUNE_COUNT] = {256, 256, 4, 2};
+static const int g_tune_hi[PXFF_TUNE_COUNT] = {256, 256, 4, 3};
 int tune_value(int knob) { return g_tune[knob]; }
 
 int so4_check_params(const pxff_so4_params *p) {
@@ -1070,6 +1076,7 @@ extern "C" int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64
     switch (tune_value(PXFF_TUNE_ACCUM_LDSADD)) {
     case 1: PXFF_ACCUM_UI(1); break;
     case 2: PXFF_ACCUM_UI(2); break;
+    case 3: PXFF_ACCUM_UI(3); break;
     default: PXFF_ACCUM_UI(0); break;
     }
 #undef PXFF_ACCUM_UI
@@ -1185,6 +1192,7 @@ extern "C" int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const 
     switch (tune_value(PXFF_TUNE_ACCUM_LDSADD)) {
     case 1: PXFF_ACCUM_TAN(1); break;
     case 2: PXFF_ACCUM_TAN(2); break;
+    case 3: PXFF_ACCUM_TAN(3); break;
     default: PXFF_ACCUM_TAN(0); break;
     }
 #undef PXFF_ACCUM_TAN

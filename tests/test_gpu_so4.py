@@ -355,15 +355,15 @@ def test_diamond4096_full_parity(oracle_mod):
 
 def test_accumulation_modes_agree_and_reproduce():
     """The U_tot summation modes (per-group LDS partials, LDS atomic adds
-    issued by one wavefront, partials shared by two groups) agree; each mode
-    is bitwise reproducible run to run (forces too)."""
+    issued by one wavefront, partials shared by two or three groups) agree;
+    each mode is bitwise reproducible run to run (forces too)."""
     from pyxtal_ff_amd import autotune
     s = workloads.diamond_si(reps=3, sigma=0.08, seed=4)
     nl, cfg, aw, sw, model, species, state = _engine(s, weights={"Si": 1.0})
     old = autotune.get_knob(autotune.ACCUM_LDSADD)
     try:
         res = {}
-        for mode in (0, 1, 2):
+        for mode in (0, 1, 2, 3):
             autotune.set_knob(autotune.ACCUM_LDSADD, mode)
             runs = []
             for _ in range(2):
@@ -373,7 +373,7 @@ def test_accumulation_modes_agree_and_reproduce():
             _print("mode", mode, "rerun max |dB|", np.abs(runs[0][0] - runs[1][0]).max())
             assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1]), mode
             res[mode] = runs[0]
-        for mode in (1, 2):
+        for mode in (1, 2, 3):
             assert rel_err(res[mode][0], res[0][0]) < 1e-13
             assert rel_err(res[mode][1], res[0][1]) < 1e-12
     finally:
