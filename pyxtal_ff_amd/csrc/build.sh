@@ -19,8 +19,12 @@ for src in pxff_api.cpp so4_tables.cpp nlist.hip so4_z.hip so4_kernels.hip so3_k
     for h in "$HERE"/*.h "$HERE/../../include/pxff.h"; do [ "$h" -nt "$obj" ] && stale=1; done
     if [ $stale = 0 ]; then OBJS+=("$obj"); continue; fi
   fi
+  EXTRA=()
+  # the network kernel's FP64 MFMA accumulators stay in VGPRs (no copies
+  # through the accumulation registers around every GEMM step)
+  [ "$src" = mlp.hip ] && EXTRA=(-mllvm -amdgpu-mfma-vgpr-form)
   if [[ "$src" == *.hip ]]; then
-    "$HIPCC" "${FLAGS[@]}" -c "$HERE/$src" -o "$obj" &
+    "$HIPCC" "${FLAGS[@]}" "${EXTRA[@]}" -c "$HERE/$src" -o "$obj" &
   else
     "$HIPCC" "${FLAGS[@]}" -x hip -c "$HERE/$src" -o "$obj" &
   fi

@@ -211,7 +211,18 @@ __global__ __launch_bounds__(64 * MLP_LW) void mlp_lds_kernel(int64_t natoms, in
                                                                int nlm, int hw) {
     extern __shared__ double sm[];
     double *P = sm;
-    for (int k = threadIdx.x; k < nparams; k += blockDim.x) P[k] = params[k];
+    // eight loads in flight per thread before their LDS stores
+    for (int k0 = threadIdx.x; k0 < nparams; k0 += 8 * blockDim.x) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int k = k0 + u * blockDim.x;
+            v[u] = k < nparams ? params[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (k0 + u * (int)blockDim.x < nparams) P[k0 + u * blockDim.x] = v[u];
+    }
     __syncthreads();
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double *h0 = sm + nparams + (size_t)wv * (nin + 2 * nlm * hw);
@@ -220,6 +231,196 @@ __global__ __launch_bounds__(64 * MLP_LW) void mlp_lds_kernel(int64_t natoms, in
     for (int64_t i = (int64_t)blockIdx.x * MLP_LW + wv; i < natoms; i += (int64_t)gridDim.x * MLP_LW)
         mlp_atom(i, lane, nin, nelem, species, layout, P, dmin, scale, x, e_out, beta, h0, dact, hout, hw,
                  WaveSync());
+}
+
+// ---------------------------------------------------------- matrix cores
+// 16 atoms per workgroup of four wavefronts; every layer is a GEMM on the
+// FP64 matrix cores (v_mfma_f64_16x16x4_f64: A[i][k] in lane i + 16 k,
+// B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i % 4), element i / 4),
+// one 16-wide column tile per wavefront, activations and their derivatives
+// through LDS.  Forward, per 16-neuron tile: Z^T = W_k H_{k-1}^T (A = W_k,
+// B = H^T), H_k = act(Z + b_k); backward: G_{k-1} = (G_k W_k) * act'_{k-1}
+// (A = G_k, B = W_k), dE/dx = (G_0 W_0) * scale.
+// The element's weights are staged in LDS first, zero-padded, and every
+// activation row is zero past its width, so the GEMM steps need no bounds
+// tests (the K loops run over multiples of 4).  LDS rows are 2 doubles past
+// a multiple of 32: the operand reads whose lanes walk down 16 rows fall in
+// distinct banks.  The 16 atoms may belong to different elements: the tile
+// runs the network of every element present and keeps each atom's own
+// results.
+typedef double mlp_d4 __attribute__((ext_vector_type(4)));
+constexpr int MLP_TILE = 16;
+
+__host__ __device__ inline int mlp_r16(int n) { return (n + 15) & ~15; }
+__host__ __device__ inline int mlp_r4(int n) { return (n + 3) & ~3; }
+__host__ __device__ inline int mlp_row(int n) { return ((n + 31) & ~31) + 2; }  // LDS row stride
+
+// acc = sum_k A[lane & 15][k] B[k][col], k < K (a multiple of 4):
+// a = &A[lane & 15][lane >> 4] (row-major A, k steps of 4 elements),
+// b = &B[lane >> 4][col], bstep = 4 x the row stride of B.  Four
+// accumulators take the k steps in turn (four independent MFMA chains).
+__device__ inline mlp_d4 mlp_gemm_tile(int K, const double *a, const double *b, int bstep) {
+    mlp_d4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = c0, c2 = c0, c3 = c0;
+    const int S = K >> 2;
+    int t = 0;
+    for (; t + 4 <= S; t += 4) {
+        const double a0 = a[4 * t], a1 = a[4 * t + 4], a2 = a[4 * t + 8], a3 = a[4 * t + 12];
+        const double b0 = b[t * bstep], b1 = b[(t + 1) * bstep], b2 = b[(t + 2) * bstep], b3 = b[(t + 3) * bstep];
+        c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c1, 0, 0, 0);
+        c2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b2, c2, 0, 0, 0);
+        c3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, b3, c3, 0, 0, 0);
+    }
+    for (; t < S; t++) c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[4 * t], b[t * bstep], c0, 0, 0, 0);
+    return (c0 + c1) + (c2 + c3);
+}
+
+__global__ __launch_bounds__(256) void mlp_mfma_kernel(int64_t natoms, int nin, int nelem,
+                                                        const int64_t *__restrict__ species,
+                                                        const int *__restrict__ layout,
+                                                        const double *__restrict__ params,
+                                                        const double *__restrict__ dmin,
+                                                        const double *__restrict__ scale,
+                                                        const double *__restrict__ x, double *__restrict__ e_out,
+                                                        double *__restrict__ beta, int nlm, int hw) {
+    extern __shared__ double sm[];
+    __shared__ int s_sp[MLP_TILE];  // element of each atom; -1 no network, -2 past the end
+    __shared__ int s_wo[MLP_MAXL];  // offset of layer k's padded W, then its b, in wl
+    const int P0 = mlp_row(nin), HP = mlp_row(hw);
+    double *h0 = sm;                                 // [16][P0] scaled inputs, zero-padded
+    double *hact = h0 + MLP_TILE * P0;               // [nlm][16][HP] activations
+    double *hder = hact + (size_t)nlm * MLP_TILE * HP;  // [nlm][16][HP] act', then gradients
+    double *wl = hder + (size_t)nlm * MLP_TILE * HP;    // W_k [r16(nout)][row(nin)], b_k [r16(nout)]
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int ri = lane & 15, kq = lane >> 4;
+    const int64_t a0 = (int64_t)blockIdx.x * MLP_TILE;
+    if (tid < MLP_TILE) {
+        const int64_t a = a0 + tid;
+        int e = -2;
+        if (a < natoms) {
+            const int64_t sp = species[a];
+            e = (sp >= 0 && sp < nelem) ? (int)sp : -1;
+        }
+        s_sp[tid] = e;
+    }
+    __syncthreads();
+    for (int el = 0; el < nelem; el++) {
+        bool any = false;
+        for (int t = 0; t < MLP_TILE; t++) any = any || s_sp[t] == el;
+        if (!any) continue;  // uniform over the workgroup
+        const int *lay = layout + el * MLP_STRIDE;
+        const int nl = lay[0];
+        const double *dm = dmin + (size_t)el * nin, *sc = scale + (size_t)el * nin;
+        // staging: eight loads in flight per thread before their LDS stores
+        for (int q0 = tid; q0 < MLP_TILE * P0; q0 += 256 * 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int q = q0 + 256 * u;
+                const int t = q / P0, j = q - t * P0;
+                const int64_t a = a0 + t;
+                v[u] = (q < MLP_TILE * P0 && a < natoms && j < nin) ? (x[(size_t)a * nin + j] - dm[j]) * sc[j]
+                                                                     : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (q0 + 256 * u < MLP_TILE * P0) h0[q0 + 256 * u] = v[u];
+        }
+        {
+            int wo = 0;
+            for (int k = 0; k < nl; k++) {
+                const int *L = lay + 1 + MLP_LSTRIDE * k;
+                const int ni = L[0], no = L[1], R = mlp_r16(no), C = mlp_row(ni);
+                const double *W = params + L[3], *bias = params + L[4];
+                for (int q0 = tid; q0 < R * C + R; q0 += 256 * 8) {
+                    double v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int q = q0 + 256 * u;
+                        v[u] = 0.0;
+                        if (q < R * C) {
+                            const int r = q / C, c = q - r * C;
+                            if (r < no && c < ni) v[u] = W[r * ni + c];
+                        } else if (q - R * C < no) {
+                            v[u] = bias[q - R * C];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                        if (q0 + 256 * u < R * C + R) wl[wo + q0 + 256 * u] = v[u];
+                }
+                if (tid == 0) s_wo[k] = wo;
+                wo += R * C + R;
+            }
+        }
+        __syncthreads();
+        // forward
+        const double *hin = h0;
+        int ldi = P0;
+        for (int k = 0; k < nl; k++) {
+            const int *L = lay + 1 + MLP_LSTRIDE * k;
+            const int ni = L[0], no = L[1], act = L[2], C = mlp_row(ni);
+            const double *W = wl + s_wo[k], *bias = W + mlp_r16(no) * C;
+            double *ho = hact + (size_t)k * MLP_TILE * HP, *hd = hder + (size_t)k * MLP_TILE * HP;
+            for (int ct = wv; ct * 16 < no; ct += 4) {
+                // D[j][i] = sum_k W[j][k] H[i][k]: A rows = neurons of the
+                // tile, B columns = atoms
+                const mlp_d4 acc = mlp_gemm_tile(mlp_r4(ni), W + (ct * 16 + ri) * C + kq, hin + ri * ldi + kq, 4);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int j = ct * 16 + kq + 4 * r;  // neuron; the lane's atom is ri
+                    double av = 0.0, da = 0.0;          // zero past the layer width
+                    if (j < no) act_fdf(act, acc[r] + bias[j], av, da);
+                    ho[ri * HP + j] = av;
+                    hd[ri * HP + j] = da;
+                }
+            }
+            __syncthreads();
+            hin = ho;
+            ldi = HP;
+        }
+        if (tid < MLP_TILE && s_sp[tid] == el) e_out[a0 + tid] = hin[tid * HP];
+        // backward; hder of the output layer (one neuron) already is dE/dz
+        for (int k = nl - 1; k >= 1; k--) {
+            const int *L = lay + 1 + MLP_LSTRIDE * k;
+            const int ni = L[0], no = L[1], C = mlp_row(ni);
+            const double *W = wl + s_wo[k];
+            const double *g = hder + (size_t)k * MLP_TILE * HP;
+            double *gp = hder + (size_t)(k - 1) * MLP_TILE * HP;
+            for (int ct = wv; ct * 16 < ni; ct += 4) {
+                const int j = ct * 16 + ri;
+                // B[n][j] = W[n][j]
+                const mlp_d4 acc = mlp_gemm_tile(mlp_r4(no), g + ri * HP + kq, W + kq * C + j, 4 * C);
+#pragma unroll
+                for (int r = 0; r < 4; r++) gp[(kq + 4 * r) * HP + j] *= acc[r];
+            }
+            __syncthreads();
+        }
+        {
+            const int *L = lay + 1;
+            const int no = L[1], C = mlp_row(nin);
+            const double *W = wl + s_wo[0];
+            for (int ct = wv; ct * 16 < nin; ct += 4) {
+                const int j = ct * 16 + ri;
+                const mlp_d4 acc = mlp_gemm_tile(mlp_r4(no), hder + ri * HP + kq, W + kq * C + j, 4 * C);
+                if (j < nin) {
+                    const double sj = sc[j];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int row = kq + 4 * r;
+                        if (s_sp[row] == el) beta[(size_t)(a0 + row) * nin + j] = acc[r] * sj;
+                    }
+                }
+            }
+        }
+        __syncthreads();  // the next element reuses the scratch
+    }
+    // atoms without a network: NaN results
+    for (int q = tid; q < MLP_TILE * nin; q += 256) {
+        const int t = q / nin;
+        if (s_sp[t] == -1) beta[(size_t)(a0 + t) * nin + (q - t * nin)] = __builtin_nan("");
+    }
+    if (tid < MLP_TILE && s_sp[tid] == -1) e_out[a0 + tid] = __builtin_nan("");
 }
 
 }  // namespace pxff
@@ -267,6 +468,26 @@ extern "C" int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const in
         for (int k = 0; k < lay[0]; k++) hw = std::max(hw, lay[1 + MLP_LSTRIDE * k + 1]);
     }
     const size_t scratch = sizeof(double) * ((size_t)nin + 2 * (size_t)nlm * hw);
+    // matrix cores: 16 atoms per workgroup, their scratch and one element's
+    // zero-padded weights and biases within 64 KB
+    size_t wmax = 0;
+    for (int e = 0; e < nelem; e++) {
+        const int *lay = h_layout + e * MLP_STRIDE;
+        size_t w = 0;
+        for (int k = 0; k < lay[0]; k++) {
+            const int *L = lay + 1 + MLP_LSTRIDE * k;
+            w += (size_t)mlp_r16(L[1]) * mlp_row(L[0]) + mlp_r16(L[1]);
+        }
+        wmax = std::max(wmax, w);
+    }
+    const size_t ldsm =
+        sizeof(double) * (MLP_TILE * ((size_t)mlp_row(nin) + 2 * (size_t)nlm * mlp_row(hw)) + wmax);
+    if (ldsm <= 65536) {
+        mlp_mfma_kernel<<<(unsigned)div_up(natoms, MLP_TILE), 256, ldsm, s>>>(
+            natoms, nin, nelem, d_species, d_layout, d_params, d_dmin, d_scale, d_x, d_e, d_beta, nlm, hw);
+        PXFF_LAUNCH_CHECK();
+        return 0;
+    }
     const size_t ldsp = sizeof(double) * (size_t)nparams + MLP_LW * scratch;
     if (ldsp <= 160 * 1024) {  // the weights fit in LDS next to the scratch
         static int attr_set = 0;
