@@ -596,8 +596,13 @@ __global__ __launch_bounds__(256) void so4_yi_kernel(int64_t natoms, int ntasks,
 }
 
 // -------------------------------------------------------- adjoint dE/dr
-// Flat over images: each wavefront serves NG images (any centres).  Per
-// image: dE/dr_k = w Re sum_h conj(fc dU_k[h] + dfc rhat_k U[h]) Y_i[h].
+constexpr int DEIDRJ_NST = 22;  // staged doubles per image of the dE/dr kernel
+// Flat over images: a wavefront serves NB = NR x NG images (any centres).
+// It first stages, one lane per image, the Cayley-Klein parameters with
+// their gradients, the cutoff and its derivative, the weight and r-hat of
+// its NB images in LDS (the G lanes of an image would otherwise each
+// compute them), then sweeps them in NR rounds of NG images.  Per image:
+// dE/dr_k = w Re sum_h conj(fc dU_k[h] + dfc rhat_k U[h]) Y_i[h].
 template <int TWOJ>
 __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int64_t *__restrict__ img_atom,
                                                            const int64_t *__restrict__ img_j,
@@ -609,9 +614,14 @@ __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int
                                                            const int64_t *__restrict__ d_count) {
     constexpr int G = TWOJ / 2 + 1;
     constexpr int NG = 64 / G;
+    constexpr int NR = 64 / NG;
+    constexpr int NB = NG * NR;
+    constexpr int NST = DEIDRJ_NST;
     constexpr int NH = nhalf_of(TWOJ);
     __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
-    __shared__ double red[4][64][4];
+    // per wavefront (dynamic, sized by the launch's wavefronts per block):
+    // NST x 64 staged doubles, 64 x 4 partial sums, 64 centre indices
+    extern __shared__ double dsm[];
     // with d_count the grid covers an image capacity and the images in use
     // are counted on the device
     if (d_count) {
@@ -624,61 +634,107 @@ __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int
     const int g = lane / G, mb = lane % G;
     for (int k = tid; k < (TWOJ + 1) * (TWOJ + 1); k += blockDim.x) rpq[k] = rootpq_g[k];
     __syncthreads();
+    const int64_t base = ((int64_t)blockIdx.x * wpb + wv) * NB;
     // a wavefront past the images in use (a launch over an image capacity)
-    // has nothing to do; the only later synchronisation is wavefront-local
-    if (((int64_t)blockIdx.x * wpb + wv) * NG >= nimg) return;
-    const int64_t img = ((int64_t)blockIdx.x * wpb + wv) * NG + g;
-    bool valid = (g < NG) && (img < nimg);
-    double x = 0, yy = 0, zz = 1.0, r = 1.0, w = 0.0;
-    int64_t i = 0;
-    if (valid) {
-        x = img_r[3 * img];
-        yy = img_r[3 * img + 1];
-        zz = img_r[3 * img + 2];
-        r = sqrt(x * x + yy * yy + zz * zz);
-        w = atom_w[img_j[img]];
-        i = img_atom[img];
-        if (r < 1e-8) {
-            valid = false;
+    // has nothing to do; the later synchronisation is wavefront-local
+    if (base >= nimg) return;
+    double(*S)[64] = reinterpret_cast<double(*)[64]>(dsm + (size_t)wv * (NST + 5) * 64);
+    double(*red)[4] = reinterpret_cast<double(*)[4]>(dsm + (size_t)wv * (NST + 5) * 64 + NST * 64);
+    int64_t *s_at = reinterpret_cast<int64_t *>(dsm + (size_t)wv * (NST + 5) * 64 + (NST + 4) * 64);
+    {
+        const int64_t img = base + lane;
+        bool ok = lane < NB && img < nimg;
+        double x = 0, yy = 0, zz = 1.0, r = 1.0, w = 0.0;
+        int64_t i = -1;
+        if (ok) {
+            x = img_r[3 * img];
+            yy = img_r[3 * img + 1];
+            zz = img_r[3 * img + 2];
+            r = sqrt(x * x + yy * yy + zz * zz);
+            w = atom_w[img_j[img]];
+            i = img_atom[img];
+            if (r < 1e-8) ok = false;  // SNAP.py:1487 (padding rows)
+        }
+        if (!ok) {
             x = 0;
             yy = 0;
             zz = 1.0;
             r = 1.0;
         }
-    }
-    double fc, dfc;
-    cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
-    CK ck;
-    cayley_klein(x, yy, zz, r, rfac0 * M_PI * r / rcut, true, ck);
-    const cd *yi = reinterpret_cast<const cd *>(y) + (size_t)i * NH;
-    double au = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    sweep_rows<TWOJ, true>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
-        if (!valid || !half) return;
-        const int h0 = hoff_of(l) + mb * (l + 1);
+        double fc, dfc;
+        cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
+        CK ck;
+        cayley_klein(x, yy, zz, r, rfac0 * M_PI * r / rcut, true, ck);
+        S[0][lane] = ck.ca.re;
+        S[1][lane] = ck.ca.im;
+        S[2][lane] = ck.cb.re;
+        S[3][lane] = ck.cb.im;
 #pragma unroll
-        for (int m = 0; m <= TWOJ; m++) {
-            if (m > l) continue;
-            const cd yv = yi[h0 + m];
-            au += cdotre(u[m], yv);
-            a0 += cdotre(du[0][m], yv);
-            a1 += cdotre(du[1][m], yv);
-            a2 += cdotre(du[2][m], yv);
+        for (int k = 0; k < 3; k++) {
+            S[4 + 2 * k][lane] = ck.cda[k].re;
+            S[5 + 2 * k][lane] = ck.cda[k].im;
+            S[10 + 2 * k][lane] = ck.cdb[k].re;
+            S[11 + 2 * k][lane] = ck.cdb[k].im;
         }
-    });
-    red[wv][lane][0] = au;
-    red[wv][lane][1] = a0;
-    red[wv][lane][2] = a1;
-    red[wv][lane][3] = a2;
-    // the sums below read lanes of the same wavefront only
+        S[16][lane] = fc;
+        S[17][lane] = dfc;
+        S[18][lane] = w;
+        S[19][lane] = x / r;
+        S[20][lane] = yy / r;
+        S[21][lane] = zz / r;
+        s_at[lane] = ok ? i : -1;
+    }
+    // the rounds read lanes of the same wavefront only
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (valid && mb == 0) {
-        double s[4] = {0, 0, 0, 0};
-        for (int q = 0; q < G; q++)
-            for (int k = 0; k < 4; k++) s[k] += red[wv][lane + q][k];
-        const double rh[3] = {x / r, yy / r, zz / r};
-        for (int k = 0; k < 3; k++) dedr[3 * img + k] = w * (fc * s[1 + k] + dfc * rh[k] * s[0]);
+    for (int rd = 0; rd < NR; rd++) {
+        if (base + rd * NG >= nimg) break;  // uniform
+        const int slot = rd * NG + (g < NG ? g : 0);
+        const int64_t i = s_at[slot];
+        const bool valid = g < NG && i >= 0;
+        CK ck;
+        ck.ca = cmake(S[0][slot], S[1][slot]);
+        ck.cb = cmake(S[2][slot], S[3][slot]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            ck.cda[k] = cmake(S[4 + 2 * k][slot], S[5 + 2 * k][slot]);
+            ck.cdb[k] = cmake(S[10 + 2 * k][slot], S[11 + 2 * k][slot]);
+        }
+        const cd *yi = reinterpret_cast<const cd *>(y) + (size_t)(valid ? i : 0) * NH;
+        double au = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        sweep_rows<TWOJ, true>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
+            if (!valid || !half) return;
+            const int h0 = hoff_of(l) + mb * (l + 1);
+#pragma unroll
+            for (int m = 0; m <= TWOJ; m++) {
+                if (m > l) continue;
+                const cd yv = yi[h0 + m];
+                au += cdotre(u[m], yv);
+                a0 += cdotre(du[0][m], yv);
+                a1 += cdotre(du[1][m], yv);
+                a2 += cdotre(du[2][m], yv);
+            }
+        });
+        red[lane][0] = au;
+        red[lane][1] = a0;
+        red[lane][2] = a1;
+        red[lane][3] = a2;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (valid && mb == 0) {
+            double sm[4] = {0, 0, 0, 0};
+            for (int q = 0; q < G; q++)
+                for (int k = 0; k < 4; k++) sm[k] += red[lane + q][k];
+            const double fc = S[16][slot], dfc = S[17][slot], w = S[18][slot];
+            const int64_t img = base + slot;
+            for (int k = 0; k < 3; k++) dedr[3 * img + k] = w * (fc * sm[1 + k] + dfc * S[19 + k][slot] * sm[0]);
+        }
+        // the next round rewrites red
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -1131,9 +1187,11 @@ static int so4_deidrj_launch(const pxff_so4_params *p, int64_t nimg, const int64
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
     const int ng = 64 / (p->twol / 2 + 1);
+    const int nb = ng * (64 / ng);  // images per wavefront (staged batch)
     const int wpb = g_tune[PXFF_TUNE_IMAGE_WAVES_PER_BLOCK];
-    const int grid = div_up(nimg, wpb * ng);
-    PXFF_DISPATCH_TWOJ(p->twol, (so4_deidrj_kernel<TJ><<<grid, 64 * wpb, 0, s>>>(nimg, d_img_atom, d_img_j, d_img_r, d_atom_w,
+    const int grid = div_up(div_up(nimg, nb), wpb);
+    const size_t lds = sizeof(double) * (size_t)wpb * (DEIDRJ_NST + 5) * 64;  // per-wavefront stage
+    PXFF_DISPATCH_TWOJ(p->twol, (so4_deidrj_kernel<TJ><<<grid, 64 * wpb, lds, s>>>(nimg, d_img_atom, d_img_j, d_img_r, d_atom_w,
                                                                            d_y, T->rootpq, p->rcut, p->rfac0,
                                                                            p->cutoff_id, d_dedr, d_count)));
     PXFF_LAUNCH_CHECK();
