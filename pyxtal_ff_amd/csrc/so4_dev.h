@@ -32,7 +32,7 @@ struct So4Dev {
 // A Y task is {l, loc0, count <= YTASK_ROWS, first block, end block, h0}: the
 // half elements h0 .. h0 + count - 1 (locations loc0.. of level l) summed
 // over the blocks [first, end) of level l.
-constexpr int YTASK_ROWS = 8;
+constexpr int YTASK_ROWS = 4;
 constexpr int YTASK_STRIDE = 6;
 
 int get_so4_tables(int twol, int norm, const So4Dev **out);
