@@ -333,6 +333,123 @@ __global__ __launch_bounds__(256) void so3_radial_kernel(So3Args a, int64_t nato
     }
 }
 
+// i_0(x) .. i_{LC-1}(x) at once (LC <= 8 orders), for the radial kernel
+// below: the positive series for x < 0.1 (h = x^2/2 < 0.005: eight terms
+// leave < 1e-22 of the sum), Miller's downward recurrence for x < 20
+// (f_{N+1} = 0, f_N = 1, f_{k-1} = f_{k+1} + (2k+1)/x f_k from
+// N = LC + 12 + x, normalised by sum_k (2k+1) i_k(x) = e^x, the x = 1 case
+// of the generating function e^{x cos t} = sum_k (2k+1) i_k(x) P_k(cos t)),
+// and the closed forms of i_0, i_1 with the upward recurrence at x >= 20,
+// where it is stable.  Checked against scipy.special.spherical_in over
+// x in [1e-3, 100], orders 0..8: <= 7e-15 relative.
+template <int LC>
+__device__ inline void sph_bessel_i_all(double x, double *f) {
+    if (x < 0.1) {
+        const double h = 0.5 * x * x;
+        double pre = 1.0;
+#pragma unroll
+        for (int l = 0; l < LC; l++) {
+            if (l > 0) pre *= x * g_so3_rodd[l];
+            const double *rk = g_so3_rk + l * SO3_KMAX;
+            double t = 1.0, sm = 1.0;
+#pragma unroll
+            for (int k = 1; k <= 8; k++) {
+                t *= h * rk[k];
+                sm += t;
+            }
+            f[l] = pre * sm;
+        }
+        return;
+    }
+    if (x < 20.0) {
+        const double ix = 1.0 / x;
+        const int N = LC + 12 + (int)x;
+        double fkp1 = 0.0, fk = 1.0, sum = 0.0;
+        for (int k = N; k >= LC; k--) {
+            sum += (2 * k + 1) * fk;
+            const double fkm1 = fkp1 + (2 * k + 1) * ix * fk;
+            fkp1 = fk;
+            fk = fkm1;
+        }
+#pragma unroll
+        for (int k = LC - 1; k >= 1; k--) {
+            f[k] = fk;
+            sum += (2 * k + 1) * fk;
+            const double fkm1 = fkp1 + (2 * k + 1) * ix * fk;
+            fkp1 = fk;
+            fk = fkm1;
+        }
+        f[0] = fk;
+        sum += fk;
+        const double sc = exp(x) / sum;
+#pragma unroll
+        for (int k = 0; k < LC; k++) f[k] *= sc;
+        return;
+    }
+    const double e = exp(x), ei = 1.0 / e;
+    const double sh = 0.5 * (e - ei), ch = 0.5 * (e + ei);
+    const double ix = 1.0 / x;
+    f[0] = sh * ix;
+    if (LC > 1) f[1] = (x * ch - sh) * ix * ix;
+#pragma unroll
+    for (int k = 1; k + 1 < LC; k++) f[k + 1] = f[k - 1] - (2 * k + 1) * ix * f[k];
+}
+
+// Radial integrals of every image, one image per lane, all orders l <= lmax
+// (lmax + 1 <= LC) at once: per quadrature node one evaluation of
+// i_0..i_{LC-1} feeds the I / dI sums of every (l, n), accumulated in node
+// order as in radial().  Same output as so3_radial_kernel.
+template <int NMAX, int LC>
+__global__ __launch_bounds__(256) void so3_radial_all_kernel(So3Args a, int64_t natoms,
+                                                             const int64_t *__restrict__ img_start,
+                                                             const double *__restrict__ img_r,
+                                                             double *__restrict__ rad) {
+    const int L = a.lmax + 1;
+    const int64_t nimg = img_start[natoms];
+    const double ta = 2.0 * a.alpha;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nimg; e += (int64_t)gridDim.x * blockDim.x) {
+        const double x0 = img_r[3 * e], y0 = img_r[3 * e + 1], z0 = img_r[3 * e + 2];
+        const double R = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+        double I[LC][NMAX], dI[LC][NMAX];
+#pragma unroll
+        for (int l = 0; l < LC; l++)
+#pragma unroll
+            for (int n = 0; n < NMAX; n++) {
+                I[l][n] = 0.0;
+                dI[l][n] = 0.0;
+            }
+        for (int q = 0; q < a.nq; q++) {
+            const double rq = a.rq[q];
+            const double x = ta * R * rq;
+            double f[LC];
+            sph_bessel_i_all<LC>(x, f);
+#pragma unroll
+            for (int l = 0; l < LC; l++) {
+                if (l >= L) continue;  // uniform
+                // i_0' = i_1; i_l' = i_{l-1} - (l+1)/x i_l
+                const double dil = (l == 0) ? f[1] : f[l - 1] - (l + 1) / x * f[l];
+                const double fd = ta * rq * dil;
+#pragma unroll
+                for (int n = 0; n < NMAX; n++) {
+                    const double g = a.G[n * a.nq + q];
+                    I[l][n] += g * f[l];
+                    dI[l][n] += g * fd;
+                }
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < LC; l++) {
+            if (l >= L) continue;
+            double *ro = rad + ((size_t)e * L + l) * 2 * NMAX;
+#pragma unroll
+            for (int n = 0; n < NMAX; n++) {
+                ro[n] = I[l][n];
+                ro[NMAX + n] = dI[l][n];
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------ accumulate
 // ctot[i][n][l][m] = sum_e c_e  (MODE 0)  or its tangent
 // sum_e sum_k dr_e,k dc_e/dr_k (MODE 1).  One workgroup (64 threads) per
@@ -736,8 +853,17 @@ extern "C" int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int
         });
     } else {
         if (d_radial) {
-            SO3_DISPATCH_NMAX(p->nmax, (so3_radial_kernel<NM><<<cus() * 16, 256, 0, s>>>(a, natoms, d_img_start,
-                                                                                          d_img_r, d_radial)));
+            if (p->nmax <= 6 && p->lmax <= 6) {
+                // every order of an image from one Bessel evaluation per node
+                // (i_0..i_6: l <= 6, and i_1 for l = 0)
+                SO3_DISPATCH_NMAX(p->nmax, if constexpr (NM <= 6) {
+                    so3_radial_all_kernel<NM, 7><<<cus() * 16, 256, 0, s>>>(a, natoms, d_img_start, d_img_r,
+                                                                           d_radial);
+                });
+            } else {
+                SO3_DISPATCH_NMAX(p->nmax, (so3_radial_kernel<NM><<<cus() * 16, 256, 0, s>>>(
+                                               a, natoms, d_img_start, d_img_r, d_radial)));
+            }
             PXFF_LAUNCH_CHECK();
         }
         SO3_DISPATCH_NMAX(p->nmax, {
