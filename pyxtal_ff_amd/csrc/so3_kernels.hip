@@ -38,6 +38,19 @@ constexpr int SO3_KMAX = 400;  // series terms
 __device__ double g_so3_rk[(SO3_LMAXC + 2) * SO3_KMAX];
 __device__ double g_so3_rodd[SO3_LMAXC + 3];
 
+// Coefficients of the normalised associated Legendre recurrence (same
+// expressions as evaluated in place before; filled once per device with
+// the series tables): for k >= 2, m <= k - 2
+//   g_so3_lega[k][m] = sqrt((4k^2 - 1) / (k^2 - m^2)),
+//   g_so3_legb[k][m] = sqrt(((k-1)^2 - m^2) / (4 (k-1)^2 - 1)),
+// and per m: g_so3_pmm[m] = -sqrt((2m + 1) / (2m)) (m >= 1),
+// g_so3_p1[m] = sqrt(2m + 3).
+constexpr int SO3_LEGD = SO3_LMAXC + 2;
+__device__ double g_so3_lega[SO3_LEGD * SO3_LEGD];
+__device__ double g_so3_legb[SO3_LEGD * SO3_LEGD];
+__device__ double g_so3_pmm[SO3_LEGD];
+__device__ double g_so3_p1[SO3_LEGD];
+
 // i_l(x) and i_{l-1}(x) (i_1 when l == 0: needed for i_0' = i_1).
 __device__ inline void sph_bessel_i_pair(int l, double x, double &il, double &ilm1) {
     const double thr = 16.0 + 0.5 * l * (l + 1);
@@ -146,7 +159,7 @@ __device__ inline void ylm_row(int l, double t, double s, cd eiphi, cd *Y, int L
     double pmm = sqrt(inv4pi);  // Pbar_m^m, m = 0
     for (int m = 0; m <= l; m++) {
         if (m > 0) {
-            pmm = -sqrt((2.0 * m + 1.0) / (2.0 * m)) * s * pmm;
+            pmm = g_so3_pmm[m] * s * pmm;
             em = cmul(em, eiphi);
         }
         double p;
@@ -154,10 +167,10 @@ __device__ inline void ylm_row(int l, double t, double s, cd eiphi, cd *Y, int L
             p = pmm;
         } else {
             double pm2 = pmm;
-            double pm1 = sqrt(2.0 * m + 3.0) * t * pmm;
+            double pm1 = g_so3_p1[m] * t * pmm;
             for (int k = m + 2; k <= l; k++) {
-                double a = sqrt((4.0 * k * k - 1.0) / ((double)k * k - (double)m * m));
-                double b = sqrt(((k - 1.0) * (k - 1.0) - (double)m * m) / (4.0 * (k - 1.0) * (k - 1.0) - 1.0));
+                const double a = g_so3_lega[k * SO3_LEGD + m];
+                const double b = g_so3_legb[k * SO3_LEGD + m];
                 double pk = a * (t * pm1 - b * pm2);
                 pm2 = pm1;
                 pm1 = pk;
@@ -793,6 +806,22 @@ int so3_series_init() {
     for (int k = 0; k < SO3_LMAXC + 3; k++) rodd[k] = 1.0 / (2 * k + 1);
     PXFF_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_so3_rk), rk.data(), rk.size() * sizeof(double)));
     PXFF_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_so3_rodd), rodd.data(), rodd.size() * sizeof(double)));
+    std::vector<double> la(SO3_LEGD * SO3_LEGD, 0.0), lb(SO3_LEGD * SO3_LEGD, 0.0), pm(SO3_LEGD, 0.0),
+        p1(SO3_LEGD, 0.0);
+    for (int k = 2; k < SO3_LEGD; k++)
+        for (int m = 0; m + 2 <= k; m++) {
+            la[k * SO3_LEGD + m] = std::sqrt((4.0 * k * k - 1.0) / ((double)k * k - (double)m * m));
+            lb[k * SO3_LEGD + m] =
+                std::sqrt(((k - 1.0) * (k - 1.0) - (double)m * m) / (4.0 * (k - 1.0) * (k - 1.0) - 1.0));
+        }
+    for (int m = 0; m < SO3_LEGD; m++) {
+        if (m > 0) pm[m] = -std::sqrt((2.0 * m + 1.0) / (2.0 * m));
+        p1[m] = std::sqrt(2.0 * m + 3.0);
+    }
+    PXFF_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_so3_lega), la.data(), la.size() * sizeof(double)));
+    PXFF_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_so3_legb), lb.data(), lb.size() * sizeof(double)));
+    PXFF_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_so3_pmm), pm.data(), pm.size() * sizeof(double)));
+    PXFF_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_so3_p1), p1.data(), p1.size() * sizeof(double)));
     g_series_done.insert(dev);
     return 0;
 }
