@@ -348,15 +348,16 @@ __global__ __launch_bounds__(256) void so3_radial_kernel(So3Args a, int64_t nato
 
 // i_0(x) .. i_{LC-1}(x) at once (LC <= 8 orders), for the radial kernel
 // below: the positive series for x < 0.1 (h = x^2/2 < 0.005: eight terms
-// leave < 1e-22 of the sum), Miller's downward recurrence for x < 20
+// leave < 1e-22 of the sum), Miller's downward recurrence below XUP
 // (f_{N+1} = 0, f_N = 1, f_{k-1} = f_{k+1} + (2k+1)/x f_k from
 // N = LC + 12 + x, normalised by sum_k (2k+1) i_k(x) = e^x, the x = 1 case
 // of the generating function e^{x cos t} = sum_k (2k+1) i_k(x) P_k(cos t)),
-// and the closed forms of i_0, i_1 with the upward recurrence at x >= 20,
-// where it is stable.  Checked against scipy.special.spherical_in over
-// x in [1e-3, 100], orders 0..8: <= 7e-15 relative.
+// and the closed forms of i_0, i_1 with the upward recurrence from XUP on,
+// where it is stable (XUP = 10 for orders <= 6, 20 above).  Checked against
+// scipy.special.spherical_in over x in [1e-3, 100]: <= 1e-14 relative.
 template <int LC>
 __device__ inline void sph_bessel_i_all(double x, double *f) {
+    constexpr double XUP = LC <= 7 ? 10.0 : 20.0;
     if (x < 0.1) {
         const double h = 0.5 * x * x;
         double pre = 1.0;
@@ -374,7 +375,7 @@ __device__ inline void sph_bessel_i_all(double x, double *f) {
         }
         return;
     }
-    if (x < 20.0) {
+    if (x < XUP) {
         const double ix = 1.0 / x;
         const int N = LC + 12 + (int)x;
         double fkp1 = 0.0, fk = 1.0, sum = 0.0;

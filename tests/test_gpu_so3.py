@@ -131,7 +131,7 @@ def test_so3_stored_radial_equals_recomputed():
 def test_so3_radial_integrals_vs_scipy_quadrature(nmax, lmax):
     """The stored radial integrals I_nl(R) = sum_q G[n][q] i_l(2 alpha R r_q)
     and dI_nl/dR against the same quadrature with scipy.special.spherical_in
-    on the host (amorphous SiO2 images, 2 alpha R r_q from ~0.01 to 100:
+    on the host (amorphous SiO2 images, 2 alpha R r_q from ~0.001 to 100:
     the series, downward-recurrence and upward-recurrence ranges)."""
     from scipy.special import spherical_in
     s = workloads.amorphous_sio2(40, 80, seed=1)
