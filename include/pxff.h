@@ -300,6 +300,16 @@ int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img
                   const double *d_img_r, const int64_t *d_atom_z, const double *d_G, const double *d_rq,
                   const double *d_lnorm, const double *d_dr, double *d_out, double *d_radial, void *stream);
 
+/* pxff_so3_ctot with the radial store (d_radial required, no tangent),
+ * where the separations d_img_r are the neighbour list's own, so that image
+ * e and its reverse d_img_rev[e] (pxff_nlist_rev) have bitwise the same |r|:
+ * the radial integrals are evaluated once per image pair and written to
+ * both rows (same values as pxff_so3_ctot). */
+int pxff_so3_ctot_pairs(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start,
+                        const int64_t *d_img_j, const double *d_img_r, const int64_t *d_img_rev,
+                        const int64_t *d_atom_z, const double *d_G, const double *d_rq, const double *d_lnorm,
+                        double *d_out, double *d_radial, void *stream);
+
 /* Power spectrum x [natoms*ncoef]; with d_dctot non-NULL the tangent of x. */
 int pxff_so3_power(const pxff_so3_params *p, int64_t natoms, const double *d_ctot, const double *d_dctot,
                    double *d_x, void *stream);

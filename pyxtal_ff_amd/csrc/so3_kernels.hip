@@ -412,55 +412,103 @@ __device__ inline void sph_bessel_i_all(double x, double *f) {
 // Radial integrals of every image, one image per lane, all orders l <= lmax
 // (lmax + 1 <= LC) at once: per quadrature node one evaluation of
 // i_0..i_{LC-1} feeds the I / dI sums of every (l, n), accumulated in node
-// order as in radial().  Same output as so3_radial_kernel.
+// order as in radial().  Same output as so3_radial_kernel.  With img_rev
+// (the separations being the list's own, so an image and its reverse have
+// bitwise the same |r|), only images with rev(e) >= e are evaluated and
+// their integrals are written to both rows: each wavefront queues the
+// images it owns that qualify in LDS and runs them 64 at a time.
 template <int NMAX, int LC>
-__global__ __launch_bounds__(256) void so3_radial_all_kernel(So3Args a, int64_t natoms,
-                                                             const int64_t *__restrict__ img_start,
-                                                             const double *__restrict__ img_r,
-                                                             double *__restrict__ rad) {
+__device__ inline void so3_radial_image(const So3Args &a, int64_t e, int64_t re, const double *__restrict__ img_r,
+                                        double *__restrict__ rad) {
     const int L = a.lmax + 1;
-    const int64_t nimg = img_start[natoms];
     const double ta = 2.0 * a.alpha;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nimg; e += (int64_t)gridDim.x * blockDim.x) {
-        const double x0 = img_r[3 * e], y0 = img_r[3 * e + 1], z0 = img_r[3 * e + 2];
-        const double R = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
-        double I[LC][NMAX], dI[LC][NMAX];
+    const double x0 = img_r[3 * e], y0 = img_r[3 * e + 1], z0 = img_r[3 * e + 2];
+    const double R = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+    double I[LC][NMAX], dI[LC][NMAX];
 #pragma unroll
-        for (int l = 0; l < LC; l++)
+    for (int l = 0; l < LC; l++)
+#pragma unroll
+        for (int n = 0; n < NMAX; n++) {
+            I[l][n] = 0.0;
+            dI[l][n] = 0.0;
+        }
+    for (int q = 0; q < a.nq; q++) {
+        const double rq = a.rq[q];
+        const double x = ta * R * rq;
+        double f[LC];
+        sph_bessel_i_all<LC>(x, f);
+#pragma unroll
+        for (int l = 0; l < LC; l++) {
+            if (l >= L) continue;  // uniform
+            // i_0' = i_1; i_l' = i_{l-1} - (l+1)/x i_l
+            const double dil = (l == 0) ? f[1] : f[l - 1] - (l + 1) / x * f[l];
+            const double fd = ta * rq * dil;
 #pragma unroll
             for (int n = 0; n < NMAX; n++) {
-                I[l][n] = 0.0;
-                dI[l][n] = 0.0;
-            }
-        for (int q = 0; q < a.nq; q++) {
-            const double rq = a.rq[q];
-            const double x = ta * R * rq;
-            double f[LC];
-            sph_bessel_i_all<LC>(x, f);
-#pragma unroll
-            for (int l = 0; l < LC; l++) {
-                if (l >= L) continue;  // uniform
-                // i_0' = i_1; i_l' = i_{l-1} - (l+1)/x i_l
-                const double dil = (l == 0) ? f[1] : f[l - 1] - (l + 1) / x * f[l];
-                const double fd = ta * rq * dil;
-#pragma unroll
-                for (int n = 0; n < NMAX; n++) {
-                    const double g = a.G[n * a.nq + q];
-                    I[l][n] += g * f[l];
-                    dI[l][n] += g * fd;
-                }
+                const double g = a.G[n * a.nq + q];
+                I[l][n] += g * f[l];
+                dI[l][n] += g * fd;
             }
         }
+    }
+#pragma unroll
+    for (int w = 0; w < 2; w++) {
+        if (w == 1 && re == e) break;
+        const int64_t row = w == 0 ? e : re;
 #pragma unroll
         for (int l = 0; l < LC; l++) {
             if (l >= L) continue;
-            double *ro = rad + ((size_t)e * L + l) * 2 * NMAX;
+            double *ro = rad + ((size_t)row * L + l) * 2 * NMAX;
 #pragma unroll
             for (int n = 0; n < NMAX; n++) {
                 ro[n] = I[l][n];
                 ro[NMAX + n] = dI[l][n];
             }
         }
+    }
+}
+
+template <int NMAX, int LC>
+__global__ __launch_bounds__(256) void so3_radial_all_kernel(So3Args a, int64_t natoms,
+                                                             const int64_t *__restrict__ img_start,
+                                                             const double *__restrict__ img_r,
+                                                             const int64_t *__restrict__ img_rev,
+                                                             double *__restrict__ rad) {
+    __shared__ int64_t s_q[4][128];  // per wavefront: queued images (at most 63 + 64)
+    const int64_t nimg = img_start[natoms];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t wid = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
+    int64_t *q = s_q[wv];
+    int pend = 0;  // uniform over the wavefront
+    for (int64_t c0 = wid * 64; c0 < nimg; c0 += nw * 64) {
+        const int64_t e = c0 + lane;
+        bool take = false;
+        if (e < nimg) take = img_rev ? img_rev[e] >= e : true;
+        const uint64_t mask = __ballot(take);
+        if (take) q[pend + (int)__popcll(mask & ((1ull << lane) - 1ull))] = e;
+        pend += (int)__popcll(mask);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (pend >= 64) {
+            const int64_t my = q[lane];
+            const int rest = pend - 64;
+            const int64_t mv = lane < rest ? q[64 + lane] : 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < rest) q[lane] = mv;
+            pend = rest;
+            so3_radial_image<NMAX, LC>(a, my, img_rev ? img_rev[my] : my, img_r, rad);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    if (lane < pend) {
+        const int64_t my = q[lane];
+        so3_radial_image<NMAX, LC>(a, my, img_rev ? img_rev[my] : my, img_r, rad);
     }
 }
 
@@ -862,10 +910,10 @@ static int launch_set_lds(K kern, size_t lds) {
     return 0;
 }
 
-extern "C" int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start,
-                             const int64_t *d_img_j, const double *d_img_r, const int64_t *d_atom_z,
-                             const double *d_G, const double *d_rq, const double *d_lnorm, const double *d_dr,
-                             double *d_out, double *d_radial, void *stream) {
+static int so3_ctot_impl(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start,
+                         const int64_t *d_img_j, const double *d_img_r, const int64_t *d_img_rev,
+                         const int64_t *d_atom_z, const double *d_G, const double *d_rq, const double *d_lnorm,
+                         const double *d_dr, double *d_out, double *d_radial, void *stream) {
     int rc = so3_check(p);
     if (rc) return rc;
     if (natoms <= 0) return 0;
@@ -886,9 +934,13 @@ extern "C" int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int
             if (p->nmax <= 6 && p->lmax <= 6) {
                 // every order of an image from one Bessel evaluation per node
                 // (i_0..i_6: l <= 6, and i_1 for l = 0)
+                // with pairs, one resident round of wavefronts (2 per SIMD at
+                // 256 registers), each queueing several chunks of 64 images so
+                // that its batches of 64 qualifying images are full
+                const int grid = d_img_rev ? cus() * 2 : cus() * 16;
                 SO3_DISPATCH_NMAX(p->nmax, if constexpr (NM <= 6) {
-                    so3_radial_all_kernel<NM, 7><<<cus() * 16, 256, 0, s>>>(a, natoms, d_img_start, d_img_r,
-                                                                           d_radial);
+                    so3_radial_all_kernel<NM, 7><<<grid, 256, 0, s>>>(a, natoms, d_img_start, d_img_r, d_img_rev,
+                                                                     d_radial);
                 });
             } else {
                 SO3_DISPATCH_NMAX(p->nmax, (so3_radial_kernel<NM><<<cus() * 16, 256, 0, s>>>(
@@ -904,6 +956,23 @@ extern "C" int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int
     }
     PXFF_LAUNCH_CHECK();
     return 0;
+}
+
+extern "C" int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start,
+                             const int64_t *d_img_j, const double *d_img_r, const int64_t *d_atom_z,
+                             const double *d_G, const double *d_rq, const double *d_lnorm, const double *d_dr,
+                             double *d_out, double *d_radial, void *stream) {
+    return so3_ctot_impl(p, natoms, d_img_start, d_img_j, d_img_r, nullptr, d_atom_z, d_G, d_rq, d_lnorm, d_dr,
+                         d_out, d_radial, stream);
+}
+
+extern "C" int pxff_so3_ctot_pairs(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start,
+                                   const int64_t *d_img_j, const double *d_img_r, const int64_t *d_img_rev,
+                                   const int64_t *d_atom_z, const double *d_G, const double *d_rq,
+                                   const double *d_lnorm, double *d_out, double *d_radial, void *stream) {
+    PXFF_REQUIRE(d_img_rev != nullptr && d_radial != nullptr, "SO3: pairs need the reverse images and the radial store");
+    return so3_ctot_impl(p, natoms, d_img_start, d_img_j, d_img_r, d_img_rev, d_atom_z, d_G, d_rq, d_lnorm, nullptr,
+                         d_out, d_radial, stream);
 }
 
 extern "C" int pxff_so3_power(const pxff_so3_params *p, int64_t natoms, const double *d_ctot, const double *d_dctot,

@@ -92,6 +92,13 @@ def ctot(nl, cfg, atom_z, img_r=None, dr=None, radial=False):
     r = nl.img_r if img_r is None else img_r.contiguous()
     d = None if dr is None else dr.contiguous()
     rad = _f64(nl.img_capacity * (cfg.lmax + 1) * 2 * cfg.nmax, nl.pos.device) if radial else None
+    if radial and d is None and r.data_ptr() == nl.img_r.data_ptr():
+        # the list's own separations: an image and its reverse share |r|,
+        # so the radial integrals are evaluated once per pair
+        nat.check(nat.lib().pxff_so3_ctot_pairs(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r),
+                                                p(nl.img_rev), p(atom_z), p(cfg.G), p(cfg.rq), p(cfg.lnorm),
+                                                p(out), p(rad), nat.stream()), "pxff_so3_ctot_pairs")
+        return out, rad
     nat.check(nat.lib().pxff_so3_ctot(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r), p(atom_z),
                                       p(cfg.G), p(cfg.rq), p(cfg.lnorm), p(d), p(out), p(rad), nat.stream()),
               "pxff_so3_ctot")

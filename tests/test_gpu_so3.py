@@ -138,8 +138,12 @@ def test_so3_radial_integrals_vs_scipy_quadrature(nmax, lmax):
     nl = DeviceNeighborList([s.positions], [s.cell], [s.pbc], 5.0)
     cfg = dso3.So3Config(nmax, lmax, 5.0, 2.0)
     z = torch.as_tensor(s.numbers, dtype=torch.int64, device="cuda")
-    _, rad = dso3.ctot(nl, cfg, z, radial=True)
+    ct, rad = dso3.ctot(nl, cfg, z, radial=True)  # once per image pair (the list's own separations)
     n = nl.nimg
+    # every image evaluated (a copy of the separations): bitwise the same
+    ct2, rad2 = dso3.ctot(nl, cfg, z, img_r=nl.img_r.clone(), radial=True)
+    m = n * (lmax + 1) * 2 * nmax
+    assert torch.equal(rad[:m], rad2[:m]) and torch.equal(ct, ct2)
     rad = rad[: n * (lmax + 1) * 2 * nmax].view(n, lmax + 1, 2, nmax).cpu().numpy()
     G, Q, _ = dso3.radial_tables(nmax, lmax, 5.0, 2.0)
     r = nl.img_r[: 3 * n].view(n, 3).cpu().numpy()
