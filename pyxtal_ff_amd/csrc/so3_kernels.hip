@@ -544,9 +544,21 @@ __global__ __launch_bounds__(64) void so3_accum_kernel(So3Args a, int64_t natoms
     cd *X = Yt + NG * ysz;                                   // [NG][L][M2]
     cd *Z2 = X + NG * L * M2;                                // [NG][L][M2] (tangent)
     cd *acc = Z2 + (TAN ? NG * L * M2 : 0);                  // [csz]
+    // the entries with |m| <= l, nval = NMAX L^2 of the csz: kv[2i] = entry,
+    // kv[2i+1] = (l M2 + m + lmax) | (n L + l) << 16 (X and radial offsets)
+    int *kv = reinterpret_cast<int *>(acc + csz);
+    const int nval = NMAX * L * L;
     const int lane = threadIdx.x;
     const int g = lane / L, l = lane % L;
     for (int k = lane; k < L; k += 64) lnorm[k] = a.lnorm[k];
+    for (int i = lane; i < nval; i += 64) {
+        const int n = i / (L * L), r = i - n * L * L;
+        int ll = 0;
+        while ((ll + 1) * (ll + 1) <= r) ll++;
+        const int mi = r - ll * ll - ll + a.lmax;
+        kv[2 * i] = (n * L + ll) * M2 + mi;
+        kv[2 * i + 1] = (ll * M2 + mi) | ((n * L + ll) << 16);
+    }
     So3Args al = a;
     al.lnorm = lnorm;
     for (int64_t c = blockIdx.x; c < natoms; c += gridDim.x) {
@@ -600,15 +612,15 @@ __global__ __launch_bounds__(64) void so3_accum_kernel(So3Args a, int64_t natoms
             }
             __syncthreads();
             const int nvalid = (int)((s1 - base) < NG ? (s1 - base) : NG);
-            for (int k = lane; k < csz; k += 64) {
-                const int n = k / (L * M2), rem = k - n * L * M2, ll = rem / M2, mi = rem - ll * M2;
-                if (mi - a.lmax < -ll || mi - a.lmax > ll) continue;
+            for (int i = lane; i < nval; i += 64) {
+                const int k = kv[2 * i], w = kv[2 * i + 1];
+                const int oX = w & 0xffff, oI = w >> 16;
                 cd s = acc[k];
                 for (int gg = 0; gg < nvalid; gg++) {
-                    const int o = (gg * L + ll) * M2 + mi;
-                    const double iv = Ir[(gg * NMAX + n) * L + ll];
+                    const int o = gg * L * M2 + oX;
+                    const double iv = Ir[gg * NMAX * L + oI];
                     s = cadd(s, cscale(X[o], iv));
-                    if (TAN) s = cadd(s, cscale(Z2[o], dIr[(gg * NMAX + n) * L + ll]));
+                    if (TAN) s = cadd(s, cscale(Z2[o], dIr[gg * NMAX * L + oI]));
                 }
                 acc[k] = s;
             }
@@ -830,6 +842,7 @@ size_t accum_lds(const So3Args &a, int nmax, bool tangent) {
     const int ysz = (a.lmax + 2) * (2 * LP + 1);
     size_t d = (size_t)L + (tangent ? 2 : 1) * (size_t)NG * nmax * L + 1;
     d += 2 * ((size_t)NG * ysz + (tangent ? 2 : 1) * (size_t)NG * L * M2 + (size_t)nmax * L * M2);
+    d += (size_t)nmax * L * L;  // the valid-entry table: two ints per entry
     return d * sizeof(double);
 }
 
