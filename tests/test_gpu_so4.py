@@ -153,7 +153,7 @@ def test_nlist_multi_kernel_binning_equals_single_workgroup():
     assert at == nl.nimg
 
 
-@pytest.mark.parametrize("lmax,rc", [(3, 4.9), (4, 4.9)])
+@pytest.mark.parametrize("lmax,rc", [(0, 4.9), (1, 4.9), (3, 4.9), (4, 4.9), (5, 4.9)])
 def test_snap_diamond64_parity(oracle_mod, lmax, rc):
     s = workloads.diamond_si(reps=2, sigma=0.05, seed=0)
     _print("snap diamond64 lmax", lmax)
@@ -266,6 +266,21 @@ def test_forces_entry_is_exact_negation_of_gather():
     f, vir2 = dso4.gather(nl, dedr, virial=True, forces=True)
     assert torch.equal(f, -grad) and torch.equal(vir, vir2)
     nl.verify()  # the reverse images the gather used are complete
+
+
+@pytest.mark.parametrize("lmax", [1, 3, 5])
+def test_fused_step_other_orders(lmax):
+    """2J = 2, 6, 10 (the benchmark is 2J = 8): the fused step (U_tot, Z with
+    B rows, network kernel, Y, staged dE/dr, forces entry) equals the plain
+    materialised-dxdr path's energy and forces."""
+    s = workloads.diamond_si(reps=2, sigma=0.08, seed=6)
+    nl, cfg, aw, sw, model, species, state = _engine(s, lmax=lmax)
+    sp_atom = potential.species_atom(s.symbols, model.elements, "cuda")
+    f = potential.energy_forces_fused(nl, state, model, sp_atom)
+    p = potential.energy_forces_plain(nl, cfg, aw, sw, model, species)
+    _print("2J", 2 * lmax, "energy", float(f["energy"]), float(p["energy"]))
+    assert abs(float(f["energy"] - p["energy"])) <= 1e-12 * abs(float(p["energy"]))
+    assert rel_err(f["forces"].cpu().numpy(), p["forces"].cpu().numpy()) < 1e-9
 
 
 def test_adjoint_energy_central_fd():
