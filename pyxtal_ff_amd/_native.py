@@ -99,10 +99,16 @@ def load(require_gpu=True):
     return L
 
 
+_gpu_seen = False
+
+
 def lib():
+    global _gpu_seen
     L = load()
-    if not torch.cuda.is_available():
-        raise PxffError("pyxtal_ff_amd needs a visible gfx950 GPU (torch.cuda.is_available() is False)")
+    if not _gpu_seen:  # checked until a GPU has been seen once (it does not go away)
+        if not torch.cuda.is_available():
+            raise PxffError("pyxtal_ff_amd needs a visible gfx950 GPU (torch.cuda.is_available() is False)")
+        _gpu_seen = True
     return L
 
 
@@ -129,7 +135,10 @@ def ptr(t):
 
 
 def stream():
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    """torch's current stream on the current device (the raw handle; the
+    same value as torch.cuda.current_stream().cuda_stream, fetched without
+    the Python stream object)."""
+    return ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice()))
 
 
 def call(name, *args):

@@ -99,7 +99,13 @@ class NNPotential(torch.nn.Module):
         device layout, params, dmin [nelem, nin], scale [nelem, nin]).
         Rebuilt when any parameter or buffer changed in place (optimiser
         steps bump tensor versions)."""
-        ts = list(self.parameters()) + list(self.buffers())
+        # the module tree is walked once; the key then costs one pair per
+        # tensor (walking it on every call took most of the host time of an
+        # E+F step)
+        ts = self.__dict__.get("_packed_tensors")
+        if ts is None:
+            ts = list(self.parameters()) + list(self.buffers())
+            self.__dict__["_packed_tensors"] = ts
         key = tuple((t.data_ptr(), t._version) for t in ts)
         if self._packed is not None and self._packed[0] == key:
             return self._packed[1]
