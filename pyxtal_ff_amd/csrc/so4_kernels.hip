@@ -311,7 +311,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TANGENT ? 1 
     __shared__ cd part[NP][NH];
     __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
     __shared__ double lnorm[TWOJ + 1];
-    __shared__ double stg[TANGENT ? 1 : 5][64];  // staged ca, cb, fc w (0: no image) of 64 images
+    // staged per image (64 at a time): ca, cb, fc w (0: no image); with
+    // TANGENT ca, cb, conj(da/dr_k), conj(db/dr_k), fc, dfc (rhat . dr), w (0: no image), dr
+    __shared__ double stg[TANGENT ? 22 : 5][64];
     const int lane = threadIdx.x;
     const int g = lane / G, mb = lane % G;
     for (int k = lane; k < (TWOJ + 1) * (TWOJ + 1); k += 64) rpq[k] = rootpq_g[k];
@@ -400,36 +402,70 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TANGENT ? 1 
                 __syncthreads();  // the next 64 images reuse the stage
             }
         } else {
-            for (int64_t base = s; base < e; base += NG) {
+            for (int64_t b64 = s; b64 < e; b64 += 64) {
+                const int64_t e64 = b64 + 64 < e ? b64 + 64 : e;
+                {
+                    const int64_t img = b64 + lane;
+                    bool ok = img < e64;
+                    double x = 0, y = 0, z = 0, r = 1.0, w = 0.0;
+                    if (ok) {
+                        x = img_r[3 * img];
+                        y = img_r[3 * img + 1];
+                        z = img_r[3 * img + 2];
+                        r = sqrt(x * x + y * y + z * z);
+                        w = atom_w[img_j[img]];
+                        if (r < 1e-8) ok = false;  // SNAP.py:1487 (padding rows)
+                    }
+                    if (!ok) {
+                        x = 0;
+                        y = 0;
+                        z = 1.0;
+                        r = 1.0;
+                    }
+                    double fc, dfc;
+                    cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
+                    CK ck;
+                    cayley_klein(x, y, z, r, rfac0 * M_PI * r / rcut, true, ck);
+                    double d0 = 0, d1 = 0, d2 = 0;
+                    if (ok) {
+                        d0 = dr[3 * img];
+                        d1 = dr[3 * img + 1];
+                        d2 = dr[3 * img + 2];
+                    }
+                    stg[0][lane] = ck.ca.re;
+                    stg[1][lane] = ck.ca.im;
+                    stg[2][lane] = ck.cb.re;
+                    stg[3][lane] = ck.cb.im;
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        stg[4 + 2 * k][lane] = ck.cda[k].re;
+                        stg[5 + 2 * k][lane] = ck.cda[k].im;
+                        stg[10 + 2 * k][lane] = ck.cdb[k].re;
+                        stg[11 + 2 * k][lane] = ck.cdb[k].im;
+                    }
+                    // D . dr = fc (dU . dr) + dfc (rhat . dr) U
+                    stg[16][lane] = fc;
+                    stg[17][lane] = dfc * (x * d0 + y * d1 + z * d2) / r;
+                    stg[18][lane] = ok ? w : 0.0;  // a zero weight adds nothing either way
+                    stg[19][lane] = d0;
+                    stg[20][lane] = d1;
+                    stg[21][lane] = d2;
+                }
+                __syncthreads();
+                for (int64_t base = b64; base < e64; base += NG) {
                 const int64_t img = base + g;
-                bool valid = (g < NG) && (img < e);
-                double x = 0, y = 0, z = 0, r = 1.0, w = 0.0;
-                if (valid) {
-                    x = img_r[3 * img];
-                    y = img_r[3 * img + 1];
-                    z = img_r[3 * img + 2];
-                    r = sqrt(x * x + y * y + z * z);
-                    w = atom_w[img_j[img]];
-                    if (r < 1e-8) valid = false;  // SNAP.py:1487 (padding rows)
-                }
-                if (!valid) {
-                    x = 0;
-                    y = 0;
-                    z = 1.0;
-                    r = 1.0;
-                }
-                double fc, dfc;
-                cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
+                const int si = (g < NG && img < e64) ? (int)(img - b64) : 0;
+                const bool valid = (g < NG) && (img < e64) && stg[18][si] != 0.0;
                 CK ck;
-                cayley_klein(x, y, z, r, rfac0 * M_PI * r / rcut, TANGENT, ck);
-                double d0 = 0, d1 = 0, d2 = 0;
-                if (valid) {
-                    d0 = dr[3 * img];
-                    d1 = dr[3 * img + 1];
-                    d2 = dr[3 * img + 2];
+                ck.ca = cmake(stg[0][si], stg[1][si]);
+                ck.cb = cmake(stg[2][si], stg[3][si]);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    ck.cda[k] = cmake(stg[4 + 2 * k][si], stg[5 + 2 * k][si]);
+                    ck.cdb[k] = cmake(stg[10 + 2 * k][si], stg[11 + 2 * k][si]);
                 }
-                // D . dr = fc (dU . dr) + dfc (rhat . dr) U
-                const double gu = dfc * (x * d0 + y * d1 + z * d2) / r;
+                const double fc = stg[16][si], gu = stg[17][si], w = stg[18][si];
+                const double d0 = stg[19][si], d1 = stg[20][si], d2 = stg[21][si];
                 const int pidx = g < NG ? g % NP : 0;  // index the __shared__ array itself: LDS atomics
                 sweep_rows<TWOJ, true>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
                     const bool act = valid && half;
@@ -454,6 +490,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TANGENT ? 1 
                         if (NPASS > 1) __syncthreads();
                     }
                 });
+                }
+                __syncthreads();  // the next 64 images reuse the stage
             }
         }
         __syncthreads();
