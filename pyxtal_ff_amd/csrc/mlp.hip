@@ -4,9 +4,11 @@
 // (pyxtal_ff/models/neuralnetwork.py:80-200: nn.Linear layers with one
 // activation each, descriptors scaled by the element's drange,
 // neuralnetwork.py:938-1029) evaluated forward, then backward to dE/dx.
-// One wavefront (= one workgroup) per atom: lane n holds neuron n of a
-// layer (widths <= 64), layer inputs are broadcast from LDS, weights stream
-// from L2.
+// Three kernels, by what fits in LDS: 16-atom tiles on the FP64 matrix
+// cores (mlp_mfma_kernel, scratch and one element's weights within 64 KB);
+// else one wavefront per atom with lane n holding neuron n of a layer
+// (widths <= 64), the weights of every element in LDS (mlp_lds_kernel) or
+// streamed from L2 (mlp_energy_grad_kernel).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
