@@ -217,7 +217,8 @@ def energy_forces_fused(nl, state, model, species_atom, virial=False):
     if nl.generation != gen:
         return energy_forces_fused(nl, state, model, species_atom, virial)
     dedr = dev_so4.deidrj(nl, cfg, state.atom_w, y)
-    out = {"energy": e_atom.sum(), "atom_energies": e_atom}
+    out = {"energy": _segment_total(e_atom, _whole_segment(e_atom.shape[0], e_atom.device))[0],
+           "atom_energies": e_atom}
     if virial:
         f, vir = dev_so4.gather(nl, dedr, virial=True, forces=True)
         out["virial"] = vir.sum(0)
@@ -242,6 +243,19 @@ def energy_forces_plain(nl, cfg, atom_w, site_w, model, species, stress=False):
         rv = dev_so4.rdxdr_reduce(nl, cfg.nb, rd)  # [N, nb, 6]
         out["stress_voigt"] = -torch.einsum("ik,ikl->l", beta, rv)
     return out
+
+
+_SEGS = {}
+
+
+def _whole_segment(n, device):
+    """[0, n] on the device (one segment over all rows), made once per size."""
+    key = (n, str(device))
+    seg = _SEGS.get(key)
+    if seg is None:
+        seg = torch.tensor([0, n], dtype=torch.int64, device=device)
+        _SEGS[key] = seg
+    return seg
 
 
 def _segment_total(v, seg):
