@@ -11,7 +11,7 @@ The same step captured once as a HIP graph and replayed
 materialised-dxdr path are timed beside it.
 
 Prints one JSON line (see the keys below).  ``--gpus N`` with N > 1 exits:
-this build measures one GPU.
+only one GPU's share of the work is measured.
 """
 import argparse
 import json
@@ -400,8 +400,9 @@ def main():
                     help="first run a 256 MiB device copy (scale reference for rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only (every box in this build has one GPU); "
-              "multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+        print("bench.py measures one GPU only: the workload is one GPU's share, and "
+              "the multi-GPU split (spatial slabs with a ghost-atom halo for one "
+              "large structure, whole structures for a batch) is not built", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
