@@ -153,6 +153,29 @@ def test_nlist_multi_kernel_binning_equals_single_workgroup():
     assert at == nl.nimg
 
 
+def test_nlist_batch_of_large_cells_equals_single_builds():
+    """Five 4096-atom structures in one batch (20480 atoms, 2560 bins, all
+    counted by one workgroup): the images equal those of each structure
+    built alone, with the batch's atom offsets."""
+    structs = [workloads.diamond_si(reps=8, sigma=0.08, seed=40 + k) for k in range(5)]
+    nl = DeviceNeighborList([s.positions for s in structs], [s.cell for s in structs],
+                            [s.pbc for s in structs], 4.9)
+    assert nl.natoms == 20480
+    gi, gj, gT, gr = [t.cpu().numpy() for t in nl.images()]
+    off = at = 0
+    for s in structs:
+        one = DeviceNeighborList([s.positions], [s.cell], [s.pbc], 4.9)
+        ii, jj, TT, rr = [t.cpu().numpy() for t in one.images()]
+        n = len(ii)
+        _print("batched large-cell structure", at, n)
+        assert np.array_equal(gi[at:at + n] - off, ii) and np.array_equal(gj[at:at + n] - off, jj)
+        assert np.array_equal(gT[at:at + n], TT) and np.array_equal(gr[at:at + n], rr)
+        at += n
+        off += len(s)
+    assert at == nl.nimg
+    nl.verify()
+
+
 @pytest.mark.parametrize("lmax,rc", [(0, 4.9), (1, 4.9), (3, 4.9), (4, 4.9), (5, 4.9)])
 def test_snap_diamond64_parity(oracle_mod, lmax, rc):
     s = workloads.diamond_si(reps=2, sigma=0.05, seed=0)
