@@ -634,37 +634,57 @@ __global__ void nl_zero_i32_kernel(int32_t *__restrict__ p, int64_t n) {
 }
 
 // ------------------------------------------------------------ device scan
-// Exclusive scan by one workgroup of 1024 threads (chunked).  out has n+1
-// entries; out[n] = total.  With cap >= 0 every output is clamped to cap
-// (so ranges read through it stay inside arrays of cap entries) and the
-// unclamped total goes to total_out[0].
+// Exclusive scan by one workgroup of 1024 threads (16 wavefronts) over
+// tiles of 4096 entries: each thread sums four consecutive entries, the
+// wavefronts scan those sums by shuffles, and one LDS exchange of the 16
+// wavefront totals per tile gives every thread its offset (two barriers
+// per tile).  out has n+1 entries; out[n] = total.  With cap >= 0 every
+// output is clamped to cap (so ranges read through it stay inside arrays
+// of cap entries) and the unclamped total goes to total_out[0].
+constexpr int SCAN_ITEMS = 4;
 template <typename T>
 __global__ __launch_bounds__(1024) void scan_kernel(int64_t n, const T *__restrict__ in, T *__restrict__ out,
                                                     T cap = -1, T *__restrict__ total_out = nullptr) {
-    __shared__ T part[1024];
-    const int tid = threadIdx.x;
-    const int64_t chunk = (n + 1023) / 1024;
-    const int64_t b = tid * chunk, e = (b + chunk < n) ? b + chunk : n;
-    T s = 0;
-    for (int64_t k = b; k < e; k++) s += in[k];
-    part[tid] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        T v = (tid >= off) ? part[tid - off] : 0;
+    __shared__ T wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    T carry = 0;
+    for (int64_t base = 0; base < n; base += 1024 * SCAN_ITEMS) {
+        T v[SCAN_ITEMS];
+        T own = 0;
+#pragma unroll
+        for (int u = 0; u < SCAN_ITEMS; u++) {
+            const int64_t k = base + (int64_t)SCAN_ITEMS * tid + u;
+            v[u] = k < n ? in[k] : 0;
+            own += v[u];
+        }
+        T x = own;  // inclusive scan of the thread sums within the wavefront
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const T y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
         __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+        T before = carry, tile = 0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const T t = wsum[q];
+            if (q < w) before += t;
+            tile += t;
+        }
+        T run = before + x - own;
+#pragma unroll
+        for (int u = 0; u < SCAN_ITEMS; u++) {
+            const int64_t k = base + (int64_t)SCAN_ITEMS * tid + u;
+            if (k < n) out[k] = (cap >= 0 && run > cap) ? cap : run;
+            run += v[u];
+        }
+        carry += tile;
+        __syncthreads();  // wsum is rewritten by the next tile
     }
-    T run = (tid > 0) ? part[tid - 1] : 0;
-    for (int64_t k = b; k < e; k++) {
-        T v = in[k];
-        out[k] = (cap >= 0 && run > cap) ? cap : run;
-        run += v;
-    }
-    if (tid == 1023) {
-        const T tot = part[1023];
-        out[n] = (cap >= 0 && tot > cap) ? cap : tot;
-        if (total_out) total_out[0] = tot;
+    if (tid == 0) {
+        out[n] = (cap >= 0 && carry > cap) ? cap : carry;
+        if (total_out) total_out[0] = carry;
     }
 }
 

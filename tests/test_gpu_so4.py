@@ -153,6 +153,30 @@ def test_nlist_multi_kernel_binning_equals_single_workgroup():
     assert at == nl.nimg
 
 
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 1023, 4095, 4096, 4097, 12289, 100003])
+def test_scan_i64_plain_and_capped(n):
+    """The device exclusive scans against numpy's cumsum: ragged lengths
+    around the wavefront (64) and tile (4096) sizes; the capped variant
+    clamps every offset and out[n] to the cap and reports the unclamped
+    total."""
+    from pyxtal_ff_amd import _native as nat
+    L = nat.lib()
+    rng = np.random.default_rng(n)
+    host = rng.integers(0, 60, n).astype(np.int64)
+    ref = np.concatenate([[0], np.cumsum(host)]).astype(np.int64)
+    d_in = torch.from_numpy(host).cuda()
+    out = torch.full((n + 1,), -7, dtype=torch.int64, device="cuda")
+    p = lambda t: nat.ptr(t)
+    nat.check(L.pxff_scan_i64(p(d_in), n, p(out), nat.stream()), "pxff_scan_i64")
+    assert np.array_equal(out.cpu().numpy(), ref)
+    cap = int(ref[-1] // 2)
+    total = torch.full((1,), -7, dtype=torch.int64, device="cuda")
+    out.fill_(-7)
+    nat.check(L.pxff_scan_i64_capped(p(d_in), n, cap, p(out), p(total), nat.stream()), "pxff_scan_i64_capped")
+    assert np.array_equal(out.cpu().numpy(), np.minimum(ref, cap))
+    assert int(total.item()) == int(ref[-1])
+
+
 def test_nlist_batch_of_large_cells_equals_single_builds():
     """Five 4096-atom structures in one batch (20480 atoms, 2560 bins, all
     counted by one workgroup): the images equal those of each structure
