@@ -130,8 +130,8 @@ __global__ __launch_bounds__(1024) void nl_bin_small_kernel(int64_t natoms, int 
                                                              int32_t *__restrict__ bin_start,
                                                              int32_t *__restrict__ bin_atoms) {
     __shared__ int cnt[NL_SMALL_BINS];
-    __shared__ int part[1024];
-    const int tid = threadIdx.x;
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     for (int k = tid; k < nbins; k += 1024) cnt[k] = 0;
     __syncthreads();
     // four atoms per thread per step, their loads in flight together
@@ -149,22 +149,28 @@ __global__ __launch_bounds__(1024) void nl_bin_small_kernel(int64_t natoms, int 
     const int b0 = tid * chunk, b1 = b0 + chunk < nbins ? b0 + chunk : nbins;
     int sum = 0;
     for (int k = b0; k < b1; k++) sum += cnt[k];
-    part[tid] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+    int x = sum;  // inclusive scan of the chunk sums within the wavefront
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
     }
-    int run = tid > 0 ? part[tid - 1] : 0;
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int run = x - sum, total = 0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int t = wsum[q];
+        if (q < w) run += t;
+        total += t;
+    }
     for (int k = b0; k < b1; k++) {
         const int c = cnt[k];
         bin_start[k] = run;
         cnt[k] = run;  // from here on the insertion cursor of bin k
         run += c;
     }
-    if (tid == 1023) bin_start[nbins] = part[1023];
+    if (tid == 0) bin_start[nbins] = total;
     __syncthreads();
     for (int64_t a0 = tid; a0 < natoms; a0 += 4 * 1024) {
         int gb[4];
