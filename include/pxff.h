@@ -205,6 +205,15 @@ int pxff_so4_gather(int64_t natoms, const int64_t *d_img_start, const int64_t *d
 int pxff_so4_forces(int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_rev, const double *d_img_r,
                     const double *d_dedr, double *d_forces, double *d_virial, void *stream);
 
+/* pxff_so4_forces plus the total energy in the same launch: d_energy[0] =
+ * sum of d_e_atom[0, n_e) (the structure's energy, the sum of the atomic
+ * network energies, neuralnetwork.py:603-677), bit-identical to
+ * pxff_segment_sum over the one segment [0, n_e); d_forces / d_virial as
+ * pxff_so4_forces. */
+int pxff_so4_forces_energy(int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_rev,
+                           const double *d_img_r, const double *d_dedr, double *d_forces, double *d_virial,
+                           const double *d_e_atom, int64_t n_e, double *d_energy, void *stream);
+
 /* Forward-mode (tangent) of B along per-image displacements dr_e
  * [nimg*3]: dB [natoms*nb] = sum_e dB/dr_e . dr_e.  Used by the double
  * backward of the adjoint (force-matched training).  d_z is the tile-major Z

@@ -123,4 +123,34 @@ inline double cutoff_f0_host(int id) {
 
 inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
+// Sum of in[r0, r1) by one workgroup of PXFF_SUM_THREADS threads in a fixed
+// order: strided sums with four loads in flight, a butterfly per
+// wavefront, then the four wavefront sums in order by thread 0.  Thread 0
+// returns the total (the others 0.0).  Every thread of the workgroup must
+// call it; it ends with a barrier, so the workgroup may call it again.
+constexpr int PXFF_SUM_THREADS = 256;
+__device__ inline double block_sum_1col(const double *__restrict__ in, int64_t r0, int64_t r1) {
+    __shared__ double wred[PXFF_SUM_THREADS / 64];
+    const int tid = threadIdx.x;
+    double acc = 0.0;
+    for (int64_t rb = r0 + tid; rb < r1; rb += 4 * PXFF_SUM_THREADS) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t r = rb + (int64_t)u * PXFF_SUM_THREADS;
+            v[u] = r < r1 ? in[r] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc += v[u];
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((tid & 63) == 0) wred[tid >> 6] = acc;
+    __syncthreads();
+    double t = 0.0;
+    if (tid == 0)
+        for (int w = 0; w < PXFF_SUM_THREADS / 64; w++) t += wred[w];
+    __syncthreads();
+    return t;
+}
+
 }  // namespace pxff

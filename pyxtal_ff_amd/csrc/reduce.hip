@@ -10,7 +10,7 @@
 
 namespace pxff {
 
-constexpr int SEG_THREADS = 256;
+constexpr int SEG_THREADS = PXFF_SUM_THREADS;
 constexpr int SEG_COLS = 16;  // columns reduced per pass
 
 __global__ __launch_bounds__(SEG_THREADS) void segment_sum_kernel(int64_t nseg, int ncol,
@@ -20,30 +20,10 @@ __global__ __launch_bounds__(SEG_THREADS) void segment_sum_kernel(int64_t nseg, 
     __shared__ double red[SEG_THREADS][SEG_COLS + 1];
     const int tid = threadIdx.x;
     if (ncol == 1) {
-        // one column (energies): strided row sums, then a butterfly per
-        // wavefront and the four wavefront sums in order (fixed order)
+        // one column (energies): block_sum_1col per segment (fixed order)
         for (int64_t s = blockIdx.x; s < nseg; s += gridDim.x) {
-            const int64_t r0 = seg[s], r1 = seg[s + 1];
-            double acc = 0.0;
-            for (int64_t rb = r0 + tid; rb < r1; rb += 4 * SEG_THREADS) {
-                double v[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int64_t r = rb + (int64_t)u * SEG_THREADS;
-                    v[u] = r < r1 ? in[r] : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; u++) acc += v[u];
-            }
-            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-            if ((tid & 63) == 0) red[tid >> 6][0] = acc;
-            __syncthreads();
-            if (tid == 0) {
-                double t = 0.0;
-                for (int w = 0; w < SEG_THREADS / 64; w++) t += red[w][0];
-                out[s] = t;
-            }
-            __syncthreads();
+            const double t = block_sum_1col(in, seg[s], seg[s + 1]);
+            if (tid == 0) out[s] = t;
         }
         return;
     }

@@ -981,13 +981,24 @@ __device__ inline double wave_sum(double v) {
 }
 
 // NEG: the forces -dE/dR_i, summed as (dedr_e - dedr_rev(e)) -- the exact
-// negation of the dE/dR sum, term by term in the same order.
+// negation of the dE/dR sum, term by term in the same order.  With e_out the
+// grid has one more workgroup, the last, which stores the sum of
+// e_in[0, n_e) (the total energy) to e_out[0] by block_sum_1col: the same
+// value as the one-column segment sum, with no launch of its own.
 template <bool NEG>
 __global__ __launch_bounds__(256) void so4_gather_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
                                                           const int64_t *__restrict__ img_rev,
                                                           const double *__restrict__ img_r,
                                                           const double *__restrict__ dedr, double *__restrict__ grad,
-                                                          double *__restrict__ virial) {
+                                                          double *__restrict__ virial,
+                                                          const double *__restrict__ e_in = nullptr,
+                                                          int64_t n_e = 0, double *__restrict__ e_out = nullptr) {
+    static_assert(PXFF_SUM_THREADS == 256, "the energy workgroup is one gather workgroup");
+    if (e_out != nullptr && blockIdx.x == gridDim.x - 1) {
+        const double t = block_sum_1col(e_in, 0, n_e);
+        if (threadIdx.x == 0) e_out[0] = t;
+        return;
+    }
     const int lane = threadIdx.x & 63;
     const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= natoms) return;  // whole wavefront
@@ -1267,6 +1278,20 @@ extern "C" int pxff_so4_forces(int64_t natoms, const int64_t *d_img_start, const
     hipStream_t s = (hipStream_t)stream;
     so4_gather_kernel<true><<<div_up(natoms, 4), 256, 0, s>>>(natoms, d_img_start, d_img_rev, d_img_r, d_dedr,
                                                               d_forces, d_virial);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_so4_forces_energy(int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_rev,
+                                      const double *d_img_r, const double *d_dedr, double *d_forces, double *d_virial,
+                                      const double *d_e_atom, int64_t n_e, double *d_energy, void *stream) {
+    PXFF_REQUIRE(d_energy != nullptr, "null energy output");
+    PXFF_REQUIRE(n_e >= 0 && (n_e == 0 || d_e_atom != nullptr), "bad atom energies");
+    PXFF_REQUIRE(natoms >= 0, "natoms must be >= 0");
+    hipStream_t s = (hipStream_t)stream;
+    // workgroups 0 .. div_up(natoms, 4) - 1 gather forces; the last one sums the energies
+    so4_gather_kernel<true><<<div_up(natoms, 4) + 1, 256, 0, s>>>(natoms, d_img_start, d_img_rev, d_img_r, d_dedr,
+                                                                  d_forces, d_virial, d_e_atom, n_e, d_energy);
     PXFF_LAUNCH_CHECK();
     return 0;
 }

@@ -217,14 +217,11 @@ def energy_forces_fused(nl, state, model, species_atom, virial=False):
     if nl.generation != gen:
         return energy_forces_fused(nl, state, model, species_atom, virial)
     dedr = dev_so4.deidrj(nl, cfg, state.atom_w, y)
-    out = {"energy": _segment_total(e_atom, _whole_segment(e_atom.shape[0], e_atom.device))[0],
-           "atom_energies": e_atom}
+    # forces and the energy total (one extra workgroup) in one launch
+    f, energy, vir = dev_so4.forces_energy(nl, dedr, e_atom, virial=virial)
+    out = {"energy": energy[0], "atom_energies": e_atom, "forces": f}
     if virial:
-        f, vir = dev_so4.gather(nl, dedr, virial=True, forces=True)
         out["virial"] = vir.sum(0)
-    else:
-        f = dev_so4.gather(nl, dedr, forces=True)
-    out["forces"] = f
     return out
 
 
@@ -243,19 +240,6 @@ def energy_forces_plain(nl, cfg, atom_w, site_w, model, species, stress=False):
         rv = dev_so4.rdxdr_reduce(nl, cfg.nb, rd)  # [N, nb, 6]
         out["stress_voigt"] = -torch.einsum("ik,ikl->l", beta, rv)
     return out
-
-
-_SEGS = {}
-
-
-def _whole_segment(n, device):
-    """[0, n] on the device (one segment over all rows), made once per size."""
-    key = (n, str(device))
-    seg = _SEGS.get(key)
-    if seg is None:
-        seg = torch.tensor([0, n], dtype=torch.int64, device=device)
-        _SEGS[key] = seg
-    return seg
 
 
 def _segment_total(v, seg):
@@ -303,13 +287,10 @@ class FusedEFGraph:
         e_atom, beta = self.model.energy_grad(b, self.species_atom)
         y = dev_so4.yi(nl, cfg, z, beta)
         dedr = dev_so4.deidrj(nl, cfg, self.atom_w, y, counted=True)
-        out = {"atom_energies": e_atom, "energy": _segment_total(e_atom, self._seg)[0]}
+        f, energy, vir = dev_so4.forces_energy(nl, dedr, e_atom, virial=self.virial)
+        out = {"atom_energies": e_atom, "energy": energy[0], "forces": f}
         if self.virial:
-            f, vir = dev_so4.gather(nl, dedr, virial=True, forces=True)
             out["virial"] = _segment_total(vir.reshape(n, 9), self._seg).view(3, 3)
-        else:
-            f = dev_so4.gather(nl, dedr, forces=True)
-        out["forces"] = f
         self._nl = nl
         return out
 

@@ -147,6 +147,23 @@ def gather(nl, dedr, virial=False, img_r=None, forces=False):
     return (g, vir[: nl.natoms * 9].view(nl.natoms, 3, 3)) if virial else g
 
 
+def forces_energy(nl, dedr, e_atom, virial=False):
+    """The forces -dE/dR [N, 3] (as ``gather(..., forces=True)``), the
+    total energy [1] = sum of e_atom (as the one-segment ``pxff_segment_sum``)
+    and (optionally) the per-centre virial [N, 3, 3], in one launch."""
+    p = nat.ptr
+    dev = nl.pos.device
+    f = _f64(nl.natoms * 3, dev)
+    vir = _f64(nl.natoms * 9, dev) if virial else None
+    e = e_atom.contiguous()
+    energy = _f64(1, dev)
+    nat.check(nat.lib().pxff_so4_forces_energy(nl.natoms, p(nl.img_start), p(nl.img_rev), p(nl.img_r),
+                                               p(dedr.contiguous()), p(f), p(vir), p(e), e.numel(), p(energy),
+                                               nat.stream()), "pxff_so4_forces_energy")
+    fv = f[: nl.natoms * 3].view(nl.natoms, 3)
+    return fv, energy, (vir[: nl.natoms * 9].view(nl.natoms, 3, 3) if virial else None)
+
+
 def tangent(nl, cfg, atom_w, z, dr, img_r=None):
     """dB [N, nb] = sum_e dB/dr_e . dr_e (forward-mode derivative of B)."""
     p = nat.ptr

@@ -315,6 +315,28 @@ def test_forces_entry_is_exact_negation_of_gather():
     nl.verify()  # the reverse images the gather used are complete
 
 
+@pytest.mark.parametrize("natoms_e", [0, 1, 64, 1000, 4097])
+def test_forces_energy_entry_equals_forces_and_segment_sum(natoms_e):
+    """pxff_so4_forces_energy: its forces and virial equal pxff_so4_forces
+    bit for bit, and its energy (summed by the launch's extra workgroup)
+    equals the one-segment pxff_segment_sum bit for bit, for energy counts
+    around the workgroup's 1024-row stride (0 gives 0.0)."""
+    s = workloads.diamond_si(reps=2, sigma=0.08, seed=3)
+    nl = DeviceNeighborList([s.positions], [s.cell], [s.pbc], 4.9)
+    g = torch.Generator(device="cpu").manual_seed(9)
+    dedr = torch.randn(nl.nimg, 3, generator=g, dtype=torch.float64).cuda()
+    e_atom = torch.randn(natoms_e, generator=g, dtype=torch.float64).cuda()
+    f_ref, vir_ref = dso4.gather(nl, dedr, virial=True, forces=True)
+    seg = torch.tensor([0, natoms_e], dtype=torch.int64, device="cuda")
+    e_ref = potential._segment_total(e_atom, seg)
+    f, e, vir = dso4.forces_energy(nl, dedr, e_atom, virial=True)
+    assert torch.equal(f, f_ref) and torch.equal(vir, vir_ref)
+    assert torch.equal(e, e_ref), (float(e), float(e_ref))
+    assert abs(float(e) - float(e_atom.sum())) <= 1e-12 * max(1.0, float(e_atom.abs().sum()))
+    f2, e2, none = dso4.forces_energy(nl, dedr, e_atom)
+    assert none is None and torch.equal(f2, f_ref) and torch.equal(e2, e_ref)
+
+
 @pytest.mark.parametrize("lmax", [1, 3, 5])
 def test_fused_step_other_orders(lmax):
     """2J = 2, 6, 10 (the benchmark is 2J = 8): the fused step (U_tot, Z with
