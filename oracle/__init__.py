@@ -66,6 +66,8 @@ def lib():
         L.so4_oracle_sizes.argtypes = [ctypes.c_int, P, P, P, P]
         L.so4_oracle_tables.argtypes = [ctypes.c_int, P, P, P, P]
         L.so4_oracle_uarray.argtypes = [ctypes.c_int] + [ctypes.c_double] * 5 + [P, P]
+        L.so4_oracle_cutoff.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, P, P]
+        L.so4_oracle_cutoff.restype = None
         L.nlist_oracle.argtypes = [P, i64, P, P, ctypes.c_double, P, P, P, P]
         L.nlist_oracle.restype = i64
         _lib = L
@@ -103,6 +105,14 @@ def uarray(twol, x, y, z, psi, r, derivative=False):
         du = (du[0::2] + 1j * du[1::2]).reshape(nu, 3)
         return u, du
     return u
+
+
+def cutoff(name, r, rc):
+    """(fc, dfc) of the SNAP cutoff ``name`` at separation r (SNAP.py:512-588)."""
+    f = ctypes.c_double()
+    df = ctypes.c_double()
+    lib().so4_oracle_cutoff(CUTOFF_IDS[name], float(r), float(rc), ctypes.byref(f), ctypes.byref(df))
+    return f.value, df.value
 
 
 # ---------------------------------------------------------- neighbour list

@@ -71,10 +71,60 @@ static void canon_sep(const double *pos, const double *cell, int64_t i, int64_t 
  * arrays of that length it fills them.  Outputs per image: centre i, neighbour
  * j, translation T[3], separation r[3].  Ordered by (i, j, T).
  */
+static void cross3(const double *a, const double *b, double *c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static void unit3(double *a) {
+    double s = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    for (int k = 0; k < 3; k++) a[k] /= s;
+}
+
+/*
+ * Lattice for the fractional coordinates: the cell with its non-periodic
+ * rows replaced by unit vectors normal to the periodic rows (by cross
+ * products).  Only periodic directions use fractional coordinates, and a
+ * cluster may carry a zero cell along its non-periodic directions, as ASE's
+ * NeighborList allows (SNAP.py:250-252).  Separations use the cell itself.
+ */
+static void frac_lattice(const double *cell, const int *pbc, double *lat) {
+    int per[3], non[3], np_ = 0, nn = 0;
+    memcpy(lat, cell, 9 * sizeof(double));
+    for (int d = 0; d < 3; d++) {
+        if (pbc[d])
+            per[np_++] = d;
+        else
+            non[nn++] = d;
+    }
+    if (np_ == 3) return;
+    if (np_ == 2) {
+        cross3(cell + 3 * per[0], cell + 3 * per[1], lat + 3 * non[0]);
+        unit3(lat + 3 * non[0]);
+    } else if (np_ == 1) {
+        const double *p = cell + 3 * per[0];
+        /* the axis least aligned with p gives a well-conditioned first normal */
+        int k = 0;
+        for (int d = 1; d < 3; d++)
+            if (fabs(p[d]) < fabs(p[k])) k = d;
+        double e[3] = {0, 0, 0};
+        e[k] = 1.0;
+        cross3(p, e, lat + 3 * non[0]);
+        unit3(lat + 3 * non[0]);
+        cross3(p, lat + 3 * non[0], lat + 3 * non[1]);
+        unit3(lat + 3 * non[1]);
+    } else {
+        memset(lat, 0, 9 * sizeof(double));
+        lat[0] = lat[4] = lat[8] = 1.0;
+    }
+}
+
 int64_t nlist_oracle(const double *pos, int64_t n, const double *cell, const int *pbc, double rc,
                      int64_t *out_i, int64_t *out_j, int *out_T, double *out_r) {
-    double inv[9];
-    inv3(cell, inv);
+    double inv[9], lat[9];
+    frac_lattice(cell, pbc, lat);
+    inv3(lat, inv);
     /* plane spacings h_d = 1/|row d of inverse-transpose| */
     double h[3];
     for (int d = 0; d < 3; d++) {

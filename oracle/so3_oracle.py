@@ -182,12 +182,23 @@ def compute_dcs(pos, nmax, lmax, rcut, alpha, cutoff, derivative=True):
 
 
 def calculate(positions, cell, pbc, numbers, nmax=3, lmax=3, rcut=3.5, alpha=2.0, derivative=True, stress=False,
-              cutoff="cosine", weight_on=False, chunk=4096):
-    """SO3.calculate (SO3.py:187-291) -> dict x, dxdr, rdxdr, elements, seq."""
+              cutoff="cosine", weight_on=False, chunk=4096, centres=None):
+    """SO3.calculate (SO3.py:187-291) -> dict x, dxdr, rdxdr, elements, seq.
+
+    centres (derivative=False only): evaluate only the images of these
+    centre atoms and return x of these centres, in the given order (the
+    per-centre sums are independent, so the values are those of the full
+    calculation)."""
     positions = np.asarray(positions, dtype=np.float64)
     numbers = np.asarray(numbers)
     N = len(positions)
     ii, jj, TT, rr = _oracle.nlist(positions, cell, pbc, rcut)
+    if centres is not None:
+        if derivative:
+            raise ValueError("centres is for derivative=False")
+        centres = np.asarray(centres, dtype=np.int64)
+        keep = np.isin(ii, centres)
+        ii, jj, rr = ii[keep], jj[keep], rr[keep]
     wts = numbers[jj].astype(np.int64)
     if weight_on:
         wts = np.where(numbers[jj] != numbers[ii], -wts, wts)
@@ -212,6 +223,8 @@ def calculate(positions, cell, pbc, numbers, nmax=3, lmax=3, rcut=3.5, alpha=2.0
         np.add.at(ctot, ii[e], C)
     P = np.einsum('aijk,aljk->ailj', ctot, np.conj(ctot)).real
     x = P[:, tri[0], tri[1], :].reshape(N, ncoefs)
+    if centres is not None:
+        return {"x": x[centres], "dxdr": None, "rdxdr": None, "elements": None, "seq": None}
     if not derivative:
         return {"x": x, "dxdr": None, "rdxdr": None, "elements": None, "seq": seq}
     dxdr = np.zeros((len(seq), ncoefs, 3))

@@ -478,6 +478,12 @@ static void compute_dbidrj(const so4_tables *t, const cplx *dulist, const cplx *
 
 /* ------------------------------------------------------------ public API */
 
+/* compute_sfac / compute_dsfac of one separation (SNAP.py:512-588). */
+void so4_oracle_cutoff(int cutoff_id, double r, double rc, double *f, double *df) {
+    *f = cut_f(r, rc, cutoff_id);
+    *df = cut_df(r, rc, cutoff_id);
+}
+
 int so4_oracle_sizes(int twol, int *ncg, int *nu, int *nz, int *nb) {
     so4_tables t;
     build_tables(twol, 0, &t);

@@ -63,7 +63,7 @@ class StructureBatch:
         self.device = dev
         self.pos = torch.as_tensor(np.concatenate([a[0] for a in arrs]), dtype=torch.float64, device=dev)
         self.builder = NeighborListBuilder(self.counts, cells, pbcs, rc, device)
-        self.nl = self.builder.build(self.pos)
+        self.nl = self.builder.build(self.pos).resolve()  # counts final before the work list is sorted
         nl = self.nl
         self.atom_struct = self.builder.atom_struct
         self.natoms = nl.natoms

@@ -30,23 +30,37 @@ except ImportError:  # pragma: no cover
     _Base, _ALL = object, None
 
 
-def make_descriptor(sb, descriptor):
+def make_descriptor(sb, descriptor, semantics="database"):
     """Batched descriptor of a StructureBatch from the reference's
     descriptor dict: {'type': 'SNAP'|'SO4'|'SO3', 'Rc', 'parameters':
     {'lmax'[, 'nmax', 'alpha', 'rfac', 'normalize_U']}, 'weights' (SNAP),
-    'cutoff'}."""
+    'cutoff'}.
+
+    The reference reads the dict differently in its two callers, and so
+    does this function:
+      * ``semantics="database"`` (``Database.compute``,
+        ``utilities/__init__.py:186-220``): SNAP takes weights, lmax,
+        normalize_U, cutoff and rfac; SO4 lmax, normalize_U and cutoff; SO3
+        nmax, lmax and alpha (cutoff stays cosine);
+      * ``semantics="calculator"`` (``compute_descriptor``, ``:276-308``,
+        which the ASE calculator calls): the same, except that SNAP keeps
+        the default rfac0 (0.99363) and SO3 the default alpha (2.0)."""
+    if semantics not in ("database", "calculator"):
+        raise ValueError("semantics must be 'database' or 'calculator'")
     t = descriptor["type"]
     rc = float(descriptor["Rc"])
     p = descriptor.get("parameters", {})
     cut = descriptor.get("cutoff", "cosine")
     if t in ("SNAP", "snap"):
-        return bt.BatchedSNAP(sb, descriptor["weights"], lmax=p["lmax"], rcut=rc, rfac0=p.get("rfac", 0.99363),
+        rfac0 = p.get("rfac", 0.99363) if semantics == "database" else 0.99363
+        return bt.BatchedSNAP(sb, descriptor["weights"], lmax=p["lmax"], rcut=rc, rfac0=rfac0,
                               cutoff=cut, normalize_U=p.get("normalize_U", False))
     if t in ("SO4", "Bispectrum", "bispectrum"):
         return bt.BatchedSNAP(sb, None, lmax=p["lmax"], rcut=rc, rfac0=1.0, cutoff=cut,
                               normalize_U=p.get("normalize_U", False), site_weights="Z")
     if t in ("SO3", "SOAP", "soap"):
-        return bt.BatchedSO3(sb, nmax=p["nmax"], lmax=p["lmax"], rcut=rc, alpha=p.get("alpha", 2.0), cutoff=cut)
+        alpha = p.get("alpha", 2.0) if semantics == "database" else 2.0
+        return bt.BatchedSO3(sb, nmax=p["nmax"], lmax=p["lmax"], rcut=rc, alpha=alpha, cutoff="cosine")
     raise NotImplementedError(f"{t} is not implemented")
 
 
@@ -75,7 +89,8 @@ class PyXtalFFCalculator(_Base):
         key = (cell.tobytes(), numbers.tobytes(), tuple(np.asarray(atoms.pbc, dtype=bool).reshape(-1)))
         if key != self._key:
             self._sb = bt.StructureBatch([atoms], float(self.descriptor["Rc"]), device=self.device)
-            self._desc = make_descriptor(self._sb, self.descriptor)
+            # the descriptor is built as the reference's compute_descriptor builds it
+            self._desc = make_descriptor(self._sb, self.descriptor, semantics="calculator")
             self._species = pot.species_index(self._sb.symbols, self.model.elements, self.device)
             self._key = key
         else:

@@ -768,6 +768,11 @@ __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int
             const double fc = S[16][slot], dfc = S[17][slot], w = S[18][slot];
             const int64_t img = base + slot;
             for (int k = 0; k < 3; k++) dedr[3 * img + k] = w * (fc * sm[1 + k] + dfc * S[19 + k][slot] * sm[0]);
+        } else if (g < NG && mb == 0 && base + slot < nimg) {
+            // an image of coincident atoms (r < 1e-8) contributes nothing, as
+            // the reference skips it (SNAP.py:1487)
+            const int64_t img = base + slot;
+            for (int k = 0; k < 3; k++) dedr[3 * img + k] = 0.0;
         }
         // the next round rewrites red
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -27,7 +27,7 @@ from .calculator import make_descriptor
 
 def _plain_arrays(desc):
     """x, dxdr rows and per-atom Voigt rdxdr of a batch (device arrays)."""
-    nl = desc.batch.nl
+    nl = desc.batch.nl.resolve()  # count settled before any kernel
     if isinstance(desc, bt.BatchedSO3):
         x, ct, rad = dso3.forward(nl, desc.cfg, desc.state.atom_z, radial=True)
         nf = desc.cfg.ncoef

@@ -162,7 +162,7 @@ class SO3:
         pos, cell, numbers, symbols, pbc = as_arrays(atoms)
         n = len(pos)
         k = prefix_count(atom_ids, n)
-        nl = DeviceNeighborList([pos], [cell], [pbc], self.rcut)
+        nl = DeviceNeighborList([pos], [cell], [pbc], self.rcut).resolve()  # count settled before any kernel
         cfg = dev_so3.So3Config(self.nmax, self.lmax, self.rcut, self.alpha, self._cutoff_function, self.weight_on)
         d = nl.pos.device
         z = torch.as_tensor(numbers, dtype=torch.int64, device=d)

@@ -38,7 +38,9 @@ def run_so4(atoms, twol, rcut, rfac0, cutoff_id, normalize_u, derivative, stress
     n = len(pos)
     k = prefix_count(atom_ids, n)
     vol = abs(float(np.linalg.det(cell)))
-    nl = DeviceNeighborList([pos], [cell], [pbc], rcut)
+    # the image count is settled before the first kernel (a build that
+    # overflowed its capacity is redone first, never computed on)
+    nl = DeviceNeighborList([pos], [cell], [pbc], rcut).resolve()
     cfg = dev_so4.So4Config(twol, rcut, rfac0, cutoff_id, normalize_u)
     d = nl.pos.device
     aw = torch.as_tensor(np.asarray(atom_w, dtype=np.float64), device=d)

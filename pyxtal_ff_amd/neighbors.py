@@ -21,11 +21,37 @@ import torch
 from . import _native as nat
 
 
+def binning_lattice(cell, pbc):
+    """The lattice the atoms are binned in: the cell with the rows of its
+    non-periodic directions replaced by unit vectors orthogonal to the
+    periodic rows.  Along a non-periodic direction there is one bin and no
+    translation, so those rows only have to make the matrix invertible --
+    a cluster may come with a zero or degenerate cell there, which ASE's
+    NeighborList accepts (SNAP.py:250-252, SO3.py:326-332).  Separations
+    are still formed with the structure's own cell."""
+    c = np.array(cell, dtype=np.float64).reshape(3, 3)
+    per = [d for d in range(3) if pbc[d]]
+    if len(per) == 3:
+        return c
+    non = [d for d in range(3) if not pbc[d]]
+    if per:
+        basis = c[per]
+        if np.linalg.matrix_rank(basis) < len(per):
+            raise ValueError("the cell vectors of the periodic directions are linearly dependent")
+        comp = np.linalg.svd(basis)[2][len(per):]  # orthonormal complement of the periodic rows
+    else:
+        comp = np.eye(3)
+    for k, d in enumerate(non):
+        c[d] = comp[k]
+    return c
+
+
 def _geometry(cell, pbc, rc):
     """Bin grid of one structure: nb_d = floor(h_d / rc) (>= 1) bins along
     lattice direction d, stencil half-width S_d = ceil(rc nb_d / h_d), where
-    h_d is the spacing of the lattice planes normal to direction d."""
-    inv = np.linalg.inv(cell)
+    h_d is the spacing of the lattice planes normal to direction d.  Returns
+    the inverse of the binning lattice (``binning_lattice``) and the grid."""
+    inv = np.linalg.inv(binning_lattice(cell, pbc))
     geom = np.zeros(8, dtype=np.int32)
     for d in range(3):
         if not pbc[d]:
@@ -44,6 +70,10 @@ class NeighborListBuilder:
     repeated neighbour-list builds of the same set of structures -- e.g. the
     steps of an MD run or a training epoch.  ``build(pos)`` redoes the whole
     list from the given positions; only the metadata is reused."""
+
+    # first image capacity of a builder: max(MIN_CAPACITY, CAPACITY_PER_ATOM x atoms)
+    MIN_CAPACITY = 1024
+    CAPACITY_PER_ATOM = 48
 
     def __init__(self, natoms_per_struct, cells, pbcs, rc, device="cuda"):
         counts = np.asarray(natoms_per_struct, dtype=np.int64).reshape(-1)
@@ -80,7 +110,10 @@ class NeighborListBuilder:
         self.pbc = torch.as_tensor(pbcs.astype(np.int32), **i32).contiguous()
         self.geom = torch.as_tensor(geoms, **i32).contiguous()
         self.volumes = np.abs(np.linalg.det(cells)) if nst else np.zeros(0)
-        self.inv_vol = torch.as_tensor(np.repeat(1.0 / self.volumes, counts), **f64)
+        # a cluster's zero cell has no volume: its stress rows come out as the
+        # reference's -bstress / V does (inf / nan), everything else is finite
+        with np.errstate(divide="ignore"):
+            self.inv_vol = torch.as_tensor(np.repeat(1.0 / self.volumes, counts), **f64)
         n = self.natoms
         nb = max(self.nbins, 1)
         self._ws = dict(bin_count=torch.empty(nb, **i32), bin_start=torch.empty(nb + 1, **i32),
@@ -90,7 +123,7 @@ class NeighborListBuilder:
         # image capacity of the next build: the arrays are sized before the
         # image count is known on the host (no synchronisation in build);
         # it grows when a build overflows it
-        self.img_capacity = max(1024, 48 * n)
+        self.img_capacity = max(self.MIN_CAPACITY, self.CAPACITY_PER_ATOM * n)
 
     def build(self, pos, host_count=True):
         """A DeviceNeighborList of float64 device positions pos [natoms, 3].
@@ -223,6 +256,16 @@ class DeviceNeighborList:
                 self._total_ev.synchronize()
             self._nimg = total
         return self._nimg
+
+    def resolve(self):
+        """Settle the image count before anything is launched on the list:
+        wait for it and, if the build overflowed its capacity, redo the
+        build with a larger one (``nimg``).  Callers that read no image
+        count before their first kernel call this first, so no kernel ever
+        runs on a cut list.  Returns the list."""
+        if self._host_count:
+            self.nimg
+        return self
 
     @property
     def img_capacity(self):

@@ -227,6 +227,7 @@ def energy_forces_fused(nl, state, model, species_atom, virial=False):
 
 def energy_forces_plain(nl, cfg, atom_w, site_w, model, species, stress=False):
     """Same quantities through the materialised dxdr path."""
+    nl.resolve()  # count settled before any kernel
     b, z, _ = dev_so4.forward(nl, cfg, atom_w, site_w)
     dbi = dev_so4.dbdr(nl, cfg, atom_w, z)
     dx, rd = dev_so4.rows(nl, cfg.nb, dbi, stress=stress)

@@ -1,14 +1,14 @@
 """Force-matched training through the adjoint, with the step captured in a
 HIP graph.
 
-Mirrors the reference's loss (``pyxtal_ff/models/neuralnetwork.py:466-545``):
-energy, force and stress terms per atom with user coefficients.  The force
-term is differentiated through the adjoint's double backward
-(``pxff_so4_tangent`` / the SO3 tangent), so dB/dR is never materialised at
-any order.  Structures are fixed during training, so the neighbour list, Z
-(or the SO3 ctot) and the descriptors are computed once; each step runs the
-network, the adjoint sweep (dE/dr per image, recomputed every step), its
-tangent, and the optimiser.
+The loss is the reference's (``pyxtal_ff/models/neuralnetwork.py:466-545``,
+``calculate_loss``), see ``reference_loss``.  The force term is
+differentiated through the adjoint's double backward (``pxff_so4_tangent`` /
+the SO3 tangent), so dB/dR is never materialised at any order.  Structures
+are fixed during training, so the neighbour list, Z (or the SO3 ctot) and
+the descriptors are computed once; each step runs the network, the adjoint
+sweep (dE/dr per image, recomputed every step), its tangent, and the
+optimiser.
 
 Per-structure sums (energies, force errors, virials) are segment sums over
 the contiguous atoms / images of each structure (``pxff_segment_sum``:
@@ -20,15 +20,55 @@ import torch
 from . import adjoint
 from .batch import VOIGT
 
+EV2GPA = 160.21766208  # neuralnetwork.py: eV2GPa
+
+
+def reference_loss(E, E_ref, n_atoms, f_sq, sf, force_coefficient, S_gpa=None, S_ref=None, stress_mask=None,
+                   stress_coefficient=None, reg=None):
+    """The loss of ``calculate_loss`` (neuralnetwork.py:466-545) from
+    per-structure quantities of one batch of S structures:
+
+      E, E_ref [S]          predicted / label total energies (eV),
+      n_atoms [S]           atoms per structure,
+      f_sq [S]              sum over the structure's atoms of |F - F*|^2,
+      sf [S]                per-structure weights (the softmax factors,
+                            ones when no softmax is used),
+      S_gpa, S_ref [S, 6]   predicted / label stress in GPa (Voigt xx yy zz
+                            xy xz yz; the prediction is -sum dE/dx . rdxdr
+                            times eV2GPa, :494-497, 506-507),
+      stress_mask [S]       1 for structures whose group is in the stress
+                            groups, else 0,
+      reg                   alpha sum ||w||^2 over the weight matrices, or None.
+
+    energy = sum sf ((E - E*) / n)^2 / (2 S); force = sum sf cF f_sq /
+    (2 sum n); stress = sum mask sf cS |S - S*|^2 / (2 * 6 * sum mask).
+    The terms are combined as the reference's branches combine them: the
+    stress term only with a stress coefficient, the force term only with a
+    force coefficient."""
+    nst = E.shape[0]
+    energy_loss = (sf * ((E - E_ref) / n_atoms) ** 2).sum() / (2.0 * nst)
+    loss = energy_loss
+    if force_coefficient:
+        loss = loss + (sf * force_coefficient * f_sq).sum() / (2.0 * n_atoms.sum())
+    if stress_coefficient:
+        m = stress_mask if stress_mask is not None else torch.ones_like(E)
+        s_count = 6.0 * m.sum()
+        sl = (m * sf * stress_coefficient * ((S_gpa - S_ref) ** 2).sum(1)).sum()
+        loss = loss + sl / (2.0 * torch.clamp(s_count, min=1.0))
+    if reg is not None:
+        loss = loss + reg
+    return loss
+
 
 def teacher_labels(desc, teacher, species, plain=True):
-    """Energies [S], forces [N, 3], Voigt stress [S, 6] of a teacher model,
+    """Energies [S] (eV), forces [N, 3] (eV/A) and Voigt stress [S, 6] (GPa,
+    the unit of the reference's stress labels) of a teacher model,
     evaluated on the materialised (reference-layout) dxdr / rdxdr when
     ``plain`` (the reference's contraction, neuralnetwork.py:484-497), else
     through the adjoint."""
     from . import so4 as dso4, so3 as dso3
     b = desc.batch
-    nl = b.nl
+    nl = b.nl.resolve()  # count settled before any kernel
     if plain and hasattr(desc, "atom_w"):
         x, z, _ = dso4.forward(nl, desc.cfg, desc.atom_w, desc.site_w)
         dx, rd = dso4.rows(nl, desc.cfg.nb, dso4.dbdr(nl, desc.cfg, desc.atom_w, z), stress=True)
@@ -45,25 +85,46 @@ def teacher_labels(desc, teacher, species, plain=True):
         f = -dso4.dxdr_contract(nl, nf, beta, dx)
         rv = dso4.rdxdr_reduce(nl, nf, rd)  # [N, nf, 6]
         s_atom = -torch.einsum("ik,ikl->il", beta, rv)
-        return b.per_structure(e_atom.detach()), f.detach(), b.per_structure(s_atom).detach()
+        return (b.per_structure(e_atom.detach()), f.detach(),
+                (b.per_structure(s_atom) * EV2GPA).detach())
     from .batch import energy_forces_virial, stress_voigt
     o = energy_forces_virial(desc, teacher, species, virial=True)
-    return o["energy"].detach(), o["forces"].detach(), stress_voigt(o["virial"], b).detach()
+    return o["energy"].detach(), o["forces"].detach(), (stress_voigt(o["virial"], b) * EV2GPA).detach()
+
+
+def weight_penalty(model, alpha):
+    """alpha * sum of the squares of every parameter whose name contains
+    'weight' (neuralnetwork.py:533-540), or None without alpha."""
+    if not alpha:
+        return None
+    return alpha * sum((p * p).sum() for name, p in model.named_parameters() if "weight" in name)
 
 
 class ForceTrainer:
-    """Loss = mean over structures of
-         cE ((E - E*) / n)^2 + cF sum_i |F_i - F*_i|^2 / (3 n) + cS |S - S*|^2 / 6
-    (per atom), with E from the network on fixed descriptors, F and the
-    virial stress from the adjoint."""
+    """Force-matched training of a per-element network on a fixed batch of
+    structures (``NeuralNetwork.train`` with one batch of every structure).
 
-    def __init__(self, desc, model, species, labels, coef=(1.0, 0.1, 0.01), lr=1e-3, graph=False,
-                 capturable=None):
+    The loss is ``calculate_loss``'s (neuralnetwork.py:466-545, see
+    ``reference_loss``): energies per atom over 2 x structures, forces over
+    2 x atoms, stress in GPa over 2 x stress components, the per-structure
+    weights ``sf`` and the alpha ||w||^2 penalty.  Energies come from the
+    network on fixed descriptors, forces and the virial stress from the
+    adjoint.  labels = (E [S] eV, F [N, 3] eV/A, S [S, 6] GPa)."""
+
+    def __init__(self, desc, model, species, labels, force_coefficient=0.1, stress_coefficient=0.01, alpha=None,
+                 sf=None, stress_mask=None, lr=1e-3, graph=False, capturable=None):
         self.desc, self.model, self.species = desc, model, species
         b = desc.batch
         self.batch = b
         self.E_ref, self.F_ref, self.S_ref = labels
-        self.cE, self.cF, self.cS = coef
+        self.cF, self.cS, self.alpha = force_coefficient, stress_coefficient, alpha
+        dev = b.device
+        self.sf = (torch.ones(b.nstruct, dtype=torch.float64, device=dev) if sf is None
+                   else torch.as_tensor(sf, dtype=torch.float64, device=dev))
+        # structures whose group is among the stress groups (all by default,
+        # as get_stress_group collects every group, neuralnetwork.py:1032-1040)
+        self.stress_mask = (torch.ones(b.nstruct, dtype=torch.float64, device=dev) if stress_mask is None
+                            else torch.as_tensor(stress_mask, dtype=torch.float64, device=dev))
         nl = b.nl
         self.r = nl.img_r[: 3 * nl.nimg].view(nl.nimg, 3).clone()
         # fixed structures: descriptors and (for SO4) Z are computed once
@@ -76,8 +137,8 @@ class ForceTrainer:
             from . import so3 as dso3
             # fixed structures: the radial integrals are computed once too
             _, self.ct, self.rad = dso3.forward(nl, desc.cfg, desc.state.atom_z, radial=True)
-        self.n_struct = torch.as_tensor(b.counts, dtype=torch.float64, device=b.device)
-        self._voigt = torch.as_tensor([3 * i + j for i, j in zip(*VOIGT)], dtype=torch.int64, device=b.device)
+        self.n_struct = torch.as_tensor(b.counts, dtype=torch.float64, device=dev)
+        self._voigt = torch.as_tensor([3 * i + j for i, j in zip(*VOIGT)], dtype=torch.int64, device=dev)
         # Adam keeps its step count on the device when the step is captured;
         # its update then differs from the host-count variant in the last
         # bits, so eager runs meant to match graph replays pass capturable=True
@@ -92,7 +153,9 @@ class ForceTrainer:
         from .so3 import _SO3DEdR
         return _SO3DEdR.apply(beta, self.r, self.ct, self.desc.state, self.rad)
 
-    def loss(self):
+    def predict(self):
+        """Per-structure energies [S], forces [N, 3] and Voigt stress [S, 6]
+        (GPa), differentiable in the network parameters."""
         b = self.batch
         nl = b.nl
         x = self.x.requires_grad_(True)
@@ -105,13 +168,15 @@ class ForceTrainer:
         W = b.per_structure_images(w.reshape(-1, 9)).view(b.nstruct, 3, 3)
         # Voigt components by a device index (no host-to-device copy in a captured step)
         S = -W.view(b.nstruct, 9).index_select(1, self._voigt) * b.inv_vol_struct[:, None]
-        E = b.per_structure(e_atom)
-        le = ((E - self.E_ref) / self.n_struct) ** 2
-        fa = ((forces - self.F_ref) ** 2).sum(1)
-        lf = b.per_structure(fa) / (3 * self.n_struct)
-        ls = ((S - self.S_ref) ** 2).sum(1) / 6
-        per_s = self.cE * le + self.cF * lf + self.cS * ls
-        return per_s.mean(), forces
+        return b.per_structure(e_atom), forces, S * EV2GPA
+
+    def loss(self):
+        """(loss, forces) of the current parameters."""
+        E, forces, S = self.predict()
+        f_sq = self.batch.per_structure(((forces - self.F_ref) ** 2).sum(1))
+        loss = reference_loss(E, self.E_ref, self.n_struct, f_sq, self.sf, self.cF, S, self.S_ref,
+                              self.stress_mask, self.cS, weight_penalty(self.model, self.alpha))
+        return loss, forces
 
     def _step_eager(self):
         self.opt.zero_grad(set_to_none=False)

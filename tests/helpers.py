@@ -20,3 +20,15 @@ def max_rel_elem(a, b, floor):
 
 def struct_arrays(s):
     return s.positions, s.cell, s.pbc, s.numbers, s.symbols
+
+
+def contract_rows(d, beta):
+    """Forces -sum over the seq rows (i, m) of beta_i . dxdr[(i, m)] of a
+    reference-layout descriptor dict -- the reference's contraction
+    einsum("ik,ijkl->jl") over the dense dxdr (neuralnetwork.py:484-492),
+    taken row by row."""
+    seq = np.asarray(d["seq"])
+    n = d["x"].shape[0]
+    f = np.zeros((n, 3))
+    np.add.at(f, seq[:, 1], -np.einsum("rb,rbk->rk", np.asarray(beta)[seq[:, 0]], d["dxdr"]))
+    return f

@@ -43,7 +43,7 @@ def test_batched_forces_and_virial_match_plain():
     E, F, S = training.teacher_labels(d, model, sp, plain=True)
     assert rel_err(o["energy"].detach().cpu().numpy(), E.cpu().numpy()) < 1e-12
     assert rel_err(o["forces"].detach().cpu().numpy(), F.cpu().numpy()) < 1e-9
-    assert rel_err(bt.stress_voigt(o["virial"], b).detach().cpu().numpy(), S.cpu().numpy()) < 1e-9
+    assert rel_err(bt.stress_voigt(o["virial"], b).detach().cpu().numpy() * training.EV2GPA, S.cpu().numpy()) < 1e-9
 
 
 def test_batched_so3_matches_plain():
@@ -56,7 +56,7 @@ def test_batched_so3_matches_plain():
     o = bt.energy_forces_virial(d, model, sp)
     E, F, S = training.teacher_labels(d, model, sp, plain=True)
     assert rel_err(o["forces"].detach().cpu().numpy(), F.cpu().numpy()) < 1e-9
-    assert rel_err(bt.stress_voigt(o["virial"], b).detach().cpu().numpy(), S.cpu().numpy()) < 1e-9
+    assert rel_err(bt.stress_voigt(o["virial"], b).detach().cpu().numpy() * training.EV2GPA, S.cpu().numpy()) < 1e-9
 
 
 def _trainer(structs, graph=False, seed_student=5, capturable=None):
@@ -67,8 +67,8 @@ def _trainer(structs, graph=False, seed_student=5, capturable=None):
     teacher = potential.NNPotential(el, d.nfeat, (30, 30), seed=100).cuda()
     labels = training.teacher_labels(d, teacher, sp, plain=True)
     student = potential.NNPotential(el, d.nfeat, (30, 30), seed=seed_student).cuda()
-    tr = training.ForceTrainer(d, student, sp, labels, coef=(1.0, 0.1, 0.01), lr=1e-3, graph=graph,
-                               capturable=capturable)
+    tr = training.ForceTrainer(d, student, sp, labels, force_coefficient=0.1, stress_coefficient=0.01, lr=1e-3,
+                               graph=graph, capturable=capturable)
     return tr, b, d, sp
 
 

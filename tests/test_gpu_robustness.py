@@ -1,0 +1,213 @@
+"""Drop-in robustness: a neighbour list that overflows its image capacity
+is rebuilt before any kernel runs on it (calculate, records, plain path,
+teacher labels); clusters with a zero cell along non-periodic directions;
+atoms with no neighbour; coincident atoms (r < 1e-8, skipped as
+SNAP.py:1441/1487 skip them); and the calculator's reading of the
+descriptor dict (compute_descriptor, utilities/__init__.py:276-308)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from pyxtal_ff_amd import adjoint, batch as bt, database, lammps_io, potential, training, workloads
+from pyxtal_ff_amd import so4 as dso4
+from pyxtal_ff_amd.calculator import PyXtalFFCalculator
+from pyxtal_ff_amd.descriptors.SNAP import SO4_Bispectrum as SNAPB
+from pyxtal_ff_amd.descriptors.SO3 import SO3
+from pyxtal_ff_amd.neighbors import DeviceNeighborList, NeighborListBuilder
+from pyxtal_ff_amd.structure import Structure
+from tests.helpers import contract_rows, rel_err
+from tests.test_gpu_so4 import check_layout
+
+pytestmark = pytest.mark.gpu
+
+W = {"Si": 1.0, "O": 0.8}
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture
+def tiny_capacity(monkeypatch):
+    """First builds get an image capacity far below the image count."""
+    monkeypatch.setattr(NeighborListBuilder, "MIN_CAPACITY", 16)
+    monkeypatch.setattr(NeighborListBuilder, "CAPACITY_PER_ATOM", 1)
+
+
+def _outputs():
+    rng = np.random.default_rng(8)
+    structs = [workloads.random_sio_structure(rng, 20, 40) for _ in range(3)]
+    s = structs[0]
+    out = {}
+    out["snap"] = SNAPB(W, lmax=3, rcut=4.9, derivative=True, stress=True).calculate(s)
+    out["snap_x"] = SNAPB(W, lmax=3, rcut=4.9, derivative=False, stress=False).calculate(s)
+    out["so3"] = SO3(nmax=3, lmax=3, rcut=4.9, derivative=True, stress=True).calculate(s)
+    data = [{"structure": t, "energy": 0.0, "force": np.zeros((len(t), 3)), "stress": None, "group": "g"}
+            for t in structs]
+    f = {"type": "SNAP", "Rc": 4.9, "parameters": {"lmax": 3, "rfac": 0.99363, "normalize_U": False},
+         "weights": W, "cutoff": "cosine", "force": True, "stress": True}
+    out["records"] = database.compute_records(f, data)
+    nl = DeviceNeighborList([s.positions], [s.cell], [s.pbc], 4.9)
+    cfg = dso4.So4Config(6, 4.9, 0.99363)
+    w = torch.as_tensor([W[e] for e in s.symbols], dtype=torch.float64, device="cuda")
+    el = ["O", "Si"]
+    model = potential.NNPotential(el, cfg.nb, (8, 8), seed=2).cuda()
+    sp = potential.species_index(s.symbols, el, "cuda")
+    out["plain_forces"] = potential.energy_forces_plain(nl, cfg, w, torch.ones_like(w), model, sp)["forces"].cpu()
+    b = bt.StructureBatch(structs, 4.9)
+    d = bt.BatchedSNAP(b, W, lmax=3, rcut=4.9)
+    out["labels"] = [t.cpu() for t in training.teacher_labels(d, potential.NNPotential(el, cfg.nb, (8, 8), seed=4)
+                                                              .cuda(), potential.species_index(b.symbols, el, "cuda"))]
+    return out
+
+
+def test_capacity_overflow_never_computes_on_a_cut_list(tiny_capacity):
+    small = _outputs()
+    NeighborListBuilder.MIN_CAPACITY, NeighborListBuilder.CAPACITY_PER_ATOM = 1024, 48
+    full = _outputs()
+    for k in ("snap", "snap_x", "so3"):
+        for key in ("x", "dxdr", "rdxdr", "seq"):
+            a, b = small[k].get(key), full[k].get(key)
+            assert (a is None) == (b is None), (k, key)
+            if a is not None:
+                assert np.array_equal(a, b), (k, key)
+    for ra, rb in zip(small["records"], full["records"]):
+        for key in ("x", "dxdr", "rdxdr", "seq"):
+            assert np.array_equal(ra[key], rb[key]), key
+    assert torch.equal(small["plain_forces"], full["plain_forces"])
+    for a, b in zip(small["labels"], full["labels"]):
+        assert torch.equal(a, b)
+
+
+def _cluster(n=12, seed=0, cell=None, pbc=False):
+    rng = np.random.default_rng(seed)
+    pts = []
+    while len(pts) < n:
+        p = rng.random(3) * 6.0
+        if all(np.linalg.norm(p - q) > 1.8 for q in pts):
+            pts.append(p)
+    nums = [14] * (n // 2) + [8] * (n - n // 2)
+    return Structure(np.array(pts), np.zeros((3, 3)) if cell is None else cell, numbers=nums, pbc=pbc)
+
+
+def test_zero_cell_cluster_vs_oracle(oracle_mod):
+    """A molecule-like cluster, pbc False with a zero cell: SNAP and SO3
+    (x, dxdr, seq) against the oracle; images bit-exact."""
+    from oracle import so3_oracle
+    s = _cluster()
+    nl = DeviceNeighborList([s.positions], [s.cell], [s.pbc], 4.9)
+    ii, jj, TT, rr = oracle_mod.nlist(s.positions, s.cell, s.pbc, 4.9)
+    gi, gj, gT, gr = (t.cpu().numpy() for t in nl.images())
+    assert np.array_equal(gi, ii) and np.array_equal(gj, jj) and np.array_equal(gT, TT) and np.array_equal(gr, rr)
+    g = SNAPB(W, lmax=3, rcut=4.9, derivative=True, stress=False).calculate(s)
+    r = oracle_mod.snap_calculate(s.positions, s.cell, s.pbc, s.symbols, W, lmax=3, rcut=4.9, derivative=True,
+                                  stress=False)
+    check_layout(g, r)
+    g = SO3(nmax=3, lmax=3, rcut=4.9, derivative=True, stress=False).calculate(s)
+    r = so3_oracle.calculate(s.positions, s.cell, s.pbc, s.numbers, nmax=3, lmax=3, rcut=4.9, derivative=True)
+    check_layout(g, r)
+
+
+def test_slab_with_zero_normal_vector_vs_oracle(oracle_mod):
+    """Periodic in x and y, not in z, with a zero third cell row."""
+    s0 = workloads.diamond_si(reps=2, sigma=0.05, seed=6)
+    cell = s0.cell.copy()
+    cell[2] = 0.0
+    s = Structure(s0.positions, cell, numbers=s0.numbers, pbc=[True, True, False])
+    nl = DeviceNeighborList([s.positions], [s.cell], [s.pbc], 4.9)
+    ii, jj, TT, rr = oracle_mod.nlist(s.positions, s.cell, s.pbc, 4.9)
+    gi, gj, gT, gr = (t.cpu().numpy() for t in nl.images())
+    assert np.array_equal(gi, ii) and np.array_equal(gj, jj) and np.array_equal(gT, TT) and np.array_equal(gr, rr)
+    g = SNAPB({"Si": 1.0}, lmax=3, rcut=4.9, derivative=True, stress=False).calculate(s)
+    r = oracle_mod.snap_calculate(s.positions, s.cell, s.pbc, s.symbols, {"Si": 1.0}, lmax=3, rcut=4.9,
+                                  derivative=True, stress=False)
+    check_layout(g, r)
+
+
+def test_isolated_atoms(oracle_mod):
+    """One atom alone (no image at all), then a structure where one atom
+    has no neighbour: x = (l + 1) for the lone atom (U_tot = I), its row
+    (i, i) zero; the adjoint E+F path gives it zero force."""
+    s = Structure([[1.0, 2.0, 3.0]], np.eye(3) * 20.0, symbols=["Si"], pbc=True)
+    g = SNAPB({"Si": 1.0}, lmax=4, rcut=4.9, derivative=True, stress=True).calculate(s)
+    idxb = oracle_mod.so4_tables(8)["idxb"]
+    assert np.allclose(g["x"][0], idxb[:, 2] + 1.0, rtol=1e-14, atol=0)
+    assert g["seq"].tolist() == [[0, 0]] and not g["dxdr"].any() and not g["rdxdr"].any()
+    pos = np.array([[1.0, 1.0, 1.0], [3.1, 1.2, 0.9], [2.0, 2.9, 1.4], [12.0, 12.0, 12.0]])
+    s = Structure(pos, np.eye(3) * 24.0, numbers=[14, 14, 8, 14], pbc=True)
+    g = SNAPB(W, lmax=3, rcut=4.9, derivative=True, stress=True).calculate(s)
+    r = oracle_mod.snap_calculate(s.positions, s.cell, s.pbc, s.symbols, W, lmax=3, rcut=4.9, derivative=True,
+                                  stress=True)
+    check_layout(g, r)
+    nl = DeviceNeighborList([s.positions], [s.cell], [s.pbc], 4.9)
+    cfg = dso4.So4Config(6, 4.9, 0.99363)
+    w = torch.as_tensor([W[e] for e in s.symbols], dtype=torch.float64, device="cuda")
+    one = torch.ones_like(w)
+    el = ["O", "Si"]
+    model = potential.NNPotential(el, cfg.nb, (8, 8), seed=2).cuda()
+    sa = potential.species_atom(s.symbols, el, "cuda")
+    sp = potential.species_index(s.symbols, el, "cuda")
+    fu = potential.energy_forces_fused(nl, adjoint.SO4State(nl, cfg, w, one), model, sa)["forces"].cpu().numpy()
+    fp = potential.energy_forces_plain(nl, cfg, w, one, model, sp)["forces"].cpu().numpy()
+    assert rel_err(fu, fp) < 1e-12 and not fu[3].any()
+
+
+def test_coincident_atoms_are_skipped(oracle_mod):
+    """Two atoms on the same site: their images (r = 0) are skipped by the
+    descriptor (SNAP.py:1441, 1487); x/dxdr equal the oracle's and the
+    adjoint forces (whose dE/dr rows of those images are written as zero)
+    equal the plain path's and are finite."""
+    rng = np.random.default_rng(9)
+    s = workloads.random_sio_structure(rng, 12, 12)
+    s.positions[5] = s.positions[2]
+    g = SNAPB(W, lmax=3, rcut=4.9, derivative=True, stress=True).calculate(s)
+    r = oracle_mod.snap_calculate(s.positions, s.cell, s.pbc, s.symbols, W, lmax=3, rcut=4.9, derivative=True,
+                                  stress=True)
+    check_layout(g, r)
+    nl = DeviceNeighborList([s.positions], [s.cell], [s.pbc], 4.9)
+    cfg = dso4.So4Config(6, 4.9, 0.99363)
+    w = torch.as_tensor([W[e] for e in s.symbols], dtype=torch.float64, device="cuda")
+    one = torch.ones_like(w)
+    el = ["O", "Si"]
+    model = potential.NNPotential(el, cfg.nb, (8, 8), seed=2).cuda()
+    y = dso4.yi(nl, cfg, dso4.forward(nl, cfg, w, one)[1], torch.ones((len(s), cfg.nb), dtype=torch.float64,
+                                                                       device="cuda"))
+    # garbage in the output buffer must not survive for the skipped images
+    torch.cuda.empty_cache()
+    dedr = dso4.deidrj(nl, cfg, w, y).cpu().numpy()
+    ii, jj, _, rr = (t.cpu().numpy() for t in nl.images())
+    zero = np.linalg.norm(rr, axis=1) < 1e-8
+    assert zero.sum() == 2 and np.all(dedr[zero] == 0.0) and np.isfinite(dedr).all()
+    fu = potential.energy_forces_fused(nl, adjoint.SO4State(nl, cfg, w, one), model,
+                                       potential.species_atom(s.symbols, el, "cuda"))["forces"].cpu().numpy()
+    fp = potential.energy_forces_plain(nl, cfg, w, one, model,
+                                       potential.species_index(s.symbols, el, "cuda"))["forces"].cpu().numpy()
+    assert np.isfinite(fu).all() and rel_err(fu, fp) < 1e-12
+
+
+def test_calculator_reads_the_dict_as_compute_descriptor(oracle_mod):
+    """compute_descriptor builds SNAP without rfac0 (default 0.99363) and SO3
+    without alpha: the calculator gives the oracle's result at the default,
+    whatever the dict's rfac says."""
+    pot = lammps_io.potential_from_weights(
+        lammps_io.read_nn_weights(os.path.join(HERE, "golden", "lammps", "Si-snap-zbl_NN_weights.txt")))
+    desc = {"type": "SNAP", "Rc": 5.0, "parameters": {"lmax": 3, "rfac": 0.8}, "weights": {"Si": 1.0}}
+    s = workloads.diamond_si(reps=2, sigma=0.05, seed=12)
+    calc = PyXtalFFCalculator(desc, pot)
+    e = calc.get_potential_energy(s)
+    f = calc.get_forces(s)
+    pc = potential.NNPotential(pot.elements, pot.nin, drange=None)
+    pc.load_state_dict(pot.state_dict())
+    pc = pc.cpu()
+    sp = potential.species_index(s.symbols, pot.elements, "cpu")
+    for rfac0, same in ((0.99363, True), (0.8, False)):
+        d = oracle_mod.snap_calculate(s.positions, s.cell, s.pbc, s.symbols, {"Si": 1.0}, lmax=3, rcut=5.0,
+                                      derivative=True, stress=False, rfac0=rfac0)
+        x = torch.as_tensor(d["x"]).requires_grad_(True)
+        ea = pc.atom_energies(x, sp)
+        (beta,) = torch.autograd.grad(ea.sum(), [x])
+        close = abs(e - float(ea.sum())) < 1e-10 * abs(e) and rel_err(f, contract_rows(d, beta.numpy())) < 1e-9
+        assert close == same, rfac0
