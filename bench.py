@@ -54,21 +54,24 @@ def so4_counts(twol):
 
 
 def flops(twol, k_mean):
-    """Algorithmic FP64 flops per atom-environment (conventions: complex
-    multiply 6, real x complex 2, complex add 2).
+    """FP64 flops per atom-environment (conventions: complex multiply 6,
+    real x complex 2, complex add 2, multiply-add 2).
       Z: 10 per inner term (cg * u1 * u2 accumulated) + 4 per outer term;
       U per neighbour: 18 per half element (two scaled complex products);
-      dU per neighbour: 3 x 34 per half element; adjoint contraction 16 per
-      half element; plain dB: 3 x 4 per (Z element, role)."""
+      the reference's dU per neighbour: 3 x 34 per half element, and its
+      adjoint contraction 16 per half element (SNAP.py:958-1048);
+      the adjoint sweep as run here (U alone, dU = U M(xi)): the U recursion
+      and 30 per half element for the four generator sums (A_u 4, A1 6,
+      A2 and A3 10 each); plain dB: 3 x 4 per (Z element, role)."""
     c = so4_counts(twol)
     z = 10 * c["zterms"] + 4 * c["zpairs"]
     hu = c["nhalf"] - 1
     u = 18 * hu
     du = 102 * hu
     adj_contract = 16 * c["nhalf"]
+    sweep = u + 30 * c["nhalf"]
     return dict(z_per_atom=z, u_per_nbr=u, du_per_nbr=du + u, adj_contract_per_nbr=adj_contract,
-                adjoint_per_env=z + k_mean * (u + u + du + adj_contract),
-                counts=c)
+                adj_sweep_per_nbr=sweep, adjoint_per_env=z + k_mean * (u + sweep), counts=c)
 
 
 # ------------------------------------------------------------------ GPU
@@ -113,6 +116,7 @@ ADJ_KERNELS = ("ui", "z", "bi", "network", "yi", "deidrj", "gather")
 PLAIN_KERNELS = ("dbdr", "rows", "dxdr_contract")
 # Python wrapper -> the one kernel it launches (rocprof names)
 KERNEL_SYMBOL = {"ui": "so4_accum_kernel", "z": "so4_z_tile_kernel", "bi": "so4_b_kernel",
+                 "zb": "so4_z_tile_kernel+so4_brow_sum_kernel",
                  "network": "mlp_energy_grad_kernel", "yi": "so4_yi_kernel",
                  "deidrj": "so4_deidrj_kernel", "gather": "so4_gather_kernel", "dbdr": "so4_dbdr_kernel",
                  "rows": "so4_rows_kernel", "dxdr_contract": "dxdr_contract_kernel"}
@@ -235,7 +239,9 @@ def run_gpu(steps, warmup):
     # dominant kernel's launches inside the timed region
     td = Timers()
     owner = potential.NNPotential if dom == "network" else dso4
-    name = "energy_grad" if dom == "network" else dom
+    # the function of the fused step that launches the dominant kernel (the
+    # step calls zb: the Z kernel and the short B row-partial sum)
+    name = {"network": "energy_grad", "z": "zb"}.get(dom, dom)
     td.wrap(owner, [name])
     t_adj, out_a = timed(step_adjoint, steps, warmup, on_start=td.acc.clear)
     dom_ms = td.summary()[name]
@@ -243,7 +249,7 @@ def run_gpu(steps, warmup):
     # the same E+F step captured once as a HIP graph and replayed; every
     # replay waits for its image count (the capacity check), as an MD
     # driver would before using the forces
-    gstep = potential.FusedEFGraph(builder, cfg, w, w, model, species_atom, pos_dev)
+    gstep = potential.FusedEFGraph(builder, cfg, (w, w), model, species_atom, pos_dev)
     t_graph, out_gr = timed(gstep, steps, warmup)
     out_gr = {k: v.clone() for k, v in out_gr.items()}
     t_pl, out_p = timed(step_plain, max(1, steps), max(1, warmup))
@@ -324,54 +330,161 @@ def run_training(steps=10):
     return out
 
 
-def run_so3_sio2(steps=3):
+def so3_flops(nmax, lmax):
+    """Algorithmic FP64 flops per image of the SO3 kernels, counted from the
+    reference's compute_cs / compute_dcs loop bounds (SO3.py:504-694) with
+    NQ = 10 (nmax + lmax + 1) quadrature nodes (a multiply-add is 2 flops,
+    complex operations as for SO4): the radial integrals I and dI are
+    nmax x NQ x (lmax + 1) multiply-adds each; the expansion c = e fc I Y is
+    8 flops per (n, l, m); the adjoint image term forms dc_k for the 3
+    directions (I grad Y + dI rhat Y + grad(e fc) I Y, 8 flops each) and
+    contracts it with conj(A) (4 flops) per (n, l, m, k)."""
+    nq = 10 * (nmax + lmax + 1)
+    nlm = nmax * (lmax + 1) ** 2
+    return dict(radial=2 * 2 * nmax * nq * (lmax + 1), ctot=8 * nlm, image_adjoint=3 * (8 + 4) * nlm)
+
+
+SO3_KERNELS = {"expansion": "so3_radial_all_kernel+so3_accum_kernel", "power": "so3_power_kernel",
+               "network": "mlp_energy_grad_kernel", "adjoint_a": "so3_adj_a_kernel",
+               "deidrj": "so3_image_kernel", "gather": "so4_gather_kernel"}
+
+
+def run_so3_sio2(steps=5):
     """Amorphous SiO2, 4000 Si + 8000 O at 2.2 g/cm^3 (1.6 A minimum distance),
-    SO3 nmax = lmax = 6, rc 5.0, alpha 2.0, Z weights: adjoint E+F (the
-    expansion is recomputed per image in the backward, never stored)."""
+    SO3 nmax = lmax = 6, rc 5.0, alpha 2.0, Z weights: adjoint E+F per step
+    (neighbour list rebuilt from the HBM positions; per-image expansions
+    recomputed in the backward, only the radial integrals kept): the fused
+    kernels called directly, the same step as one captured HIP graph, and
+    the autograd variant (torch network)."""
     import torch
     from pyxtal_ff_amd import batch as bt, potential, workloads
+    from pyxtal_ff_amd import so3 as dso3
+    from pyxtal_ff_amd import so4 as dso4
+    from pyxtal_ff_amd.neighbors import NeighborListBuilder
     s = workloads.amorphous_sio2(4000, 8000, seed=0)
+    n = len(s)
+    el = ["O", "Si"]
+    cfg = dso3.So3Config(6, 6, 5.0, 2.0)
+    pos = torch.as_tensor(s.positions, dtype=torch.float64, device="cuda")
+    z = torch.as_tensor(s.numbers, dtype=torch.int64, device="cuda")
+    model = potential.NNPotential(el, cfg.ncoef, (30, 30), seed=0).cuda()
+    spa = potential.species_atom(s.symbols, el, "cuda")
+    sp = potential.species_index(s.symbols, el, "cuda")
+    builder = NeighborListBuilder([n], [s.cell], [s.pbc], 5.0)
+
+    def fused():
+        nl = builder.build(pos)
+        return potential.energy_forces_fused(nl, dso3.SO3State(nl, cfg, z), model, spa)
+
     b = bt.StructureBatch([s], 5.0)
     d = bt.BatchedSO3(b, nmax=6, lmax=6, rcut=5.0, alpha=2.0)
-    el = ["O", "Si"]
-    model = potential.NNPotential(el, d.nfeat, (30, 30), seed=0).cuda()
-    sp = potential.species_index(b.symbols, el, "cuda")
 
-    def step():
+    def autograd():
         b.nl = b.builder.build(b.pos)
         d.state.nl = b.nl
         return bt.energy_forces_virial(d, model, sp, virial=False)
-    step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
-    torch.cuda.synchronize()
-    t = (time.perf_counter() - t0) / steps
-    return {"atoms": b.natoms, "images": b.nl.nimg, "ms_per_step": t * 1e3, "atom_env_per_s": b.natoms / t}
+
+    def timed(fn, k):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k, out
+
+    t_fused, o_f = timed(fused, steps)
+    g = potential.FusedEFGraph(builder, cfg, (z,), model, spa, pos)
+    t_graph, o_g = timed(g, steps)
+    t_ag, o_a = timed(autograd, max(1, steps // 2))
+    ferr = float(torch.linalg.norm(o_a["forces"] - o_f["forces"]) / torch.linalg.norm(o_f["forces"]))
+    geq = bool(torch.equal(o_g["forces"], o_f["forces"]))
+    # per-kernel durations, each launched back to back on the same inputs
+    nl = builder.build(pos)
+    nimg = nl.nimg
+
+    def t(fn, reps=5):
+        fn()
+        a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a0.record()
+        for _ in range(reps):
+            fn()
+        a1.record()
+        a1.synchronize()
+        return a0.elapsed_time(a1) / reps
+
+    kt = {}
+    x, ct, rad = dso3.forward(nl, cfg, z, radial=True)
+    # the expansion: radial integrals once per image pair, then the per-centre sums
+    kt["expansion"] = t(lambda: dso3.ctot(nl, cfg, z, radial=True))
+    kt["power"] = t(lambda: dso3.power(nl, cfg, ct))
+    e, beta = model.energy_grad(x, spa)
+    kt["network"] = t(lambda: model.energy_grad(x, spa))
+    A = dso3.adjoint_a(nl, cfg, ct, beta)
+    kt["adjoint_a"] = t(lambda: dso3.adjoint_a(nl, cfg, ct, beta))
+    kt["deidrj"] = t(lambda: dso3.deidrj(nl, cfg, z, A, radial=rad))
+    dedr = dso3.deidrj(nl, cfg, z, A, radial=rad)
+    kt["forces_energy"] = t(lambda: dso4.forces_energy(nl, dedr, e))
+    fl = so3_flops(6, 6)
+    alg = {"expansion": (fl["radial"] + fl["ctot"]) * nimg, "deidrj": fl["image_adjoint"] * nimg}
+    dom = max(alg, key=lambda k: kt[k])
+    ach = alg[dom] / (kt[dom] * 1e-3)
+    return {"atoms": n, "images": nimg, "ms_per_step": t_fused * 1e3, "atom_env_per_s": n / t_fused,
+            "graph_ms_per_step": t_graph * 1e3, "graph_atom_env_per_s": n / t_graph,
+            "graph_forces_equal_eager": geq, "autograd_ms_per_step": t_ag * 1e3,
+            "force_rel_diff_autograd_vs_fused": ferr, "kernel_ms_isolated": kt,
+            "roofline": {"bound": "valu", "kernel": SO3_KERNELS[dom], "achieved": ach / 1e12, "peak": PEAK_FP64 / 1e12,
+                         "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "flops_per_launch": alg[dom],
+                         "kernel_ms_isolated": kt[dom],
+                         "note": "algorithmic flops from the reference loop bounds (bench.so3_flops); the radial "
+                                 "kernel evaluates each image pair once (an image and its reverse share |r|), so "
+                                 "its algorithmic rate counts both images"}}
 
 
 # ------------------------------------------------------------ CPU baseline
-def run_cpu_baseline(reps_sample=5):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_cpu_baseline(reps_one=3, reps_all=5):
     """The oracle (C restatement of the reference's numba kernels: two passes,
-    U recomputed per pair, materialised dB/dr) on one host core, on the same
-    4096-atom structure, plus the NumPy force contraction."""
+    U recomputed per pair, materialised dB/dr and per-row rdxdr, stress=True
+    as the reference's Database.compute / compute_descriptor call it) on the
+    same 4096-atom structure, with the reference's force contraction over
+    the rows: on one host core (the reference runs one structure serially)
+    and on every host core (OpenMP over centres; the reference fans out over
+    structures with a process pool instead)."""
     import oracle
     oracle.build()
     s = build_case()
-    ts = []
-    for _ in range(reps_sample):
+    ncores = len(os.sched_getaffinity(0))
+
+    def once(threads):
         t0 = time.perf_counter()
         d = oracle.snap_calculate(s.positions, s.cell, s.pbc, s.symbols, {"Si": 1.0}, lmax=4, rcut=4.9,
-                                  derivative=True, stress=False)
+                                  derivative=True, stress=True, threads=threads)
         beta = np.ones_like(d["x"])
         f = np.zeros((len(s), 3))
         np.add.at(f, d["seq"][:, 1], -np.einsum("rb,rbk->rk", beta[d["seq"][:, 0]], d["dxdr"]))
-        ts.append(time.perf_counter() - t0)
-    t = float(np.median(ts))
-    return dict(value=len(s) / t, unit="atom-env/s", cores=1, kind="port",
-                sample="full 4096-atom diamond Si 2J=8 structure, descriptor+dxdr+force contraction, "
-                       "median of %d runs (%.1f s each), 1 thread" % (reps_sample, t))
+        return time.perf_counter() - t0
+
+    once(ncores)  # warm-up (tables, page faults)
+    t_all = float(np.median([once(ncores) for _ in range(reps_all)]))
+    t_one = float(np.median([once(1) for _ in range(reps_one)]))
+    n = len(s)
+    return dict(value=n / t_all, unit="atom-env/s", cores=ncores, kind="port",
+                sample="full 4096-atom diamond Si 2J=8 structure: neighbour layout, descriptor, dxdr, per-row "
+                       "rdxdr (stress=True) and the force contraction; median of %d runs of %.2f s on %d threads "
+                       "(OpenMP over centres), %s" % (reps_all, t_all, ncores, _cpu_model()),
+                one_core={"value": n / t_one, "cores": 1,
+                          "sample": "same, median of %d runs of %.2f s on 1 thread" % (reps_one, t_one)})
 
 
 def committed_traffic(symbol):
@@ -419,7 +532,7 @@ def main():
     kt = r["ktimes"]
     dom = r["dom"]
     alg = {"z": fl["z_per_atom"] * n,
-           "deidrj": (fl["du_per_nbr"] + fl["adj_contract_per_nbr"]) * r["nimg"],
+           "deidrj": fl["adj_sweep_per_nbr"] * r["nimg"],
            "ui": fl["u_per_nbr"] * r["nimg"],
            "yi": 8 * so4_counts(8)["nhalf"] * n,
            "network": 4 * (55 * 30 + 30 * 30 + 30) * n}.get(dom)

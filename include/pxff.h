@@ -334,6 +334,16 @@ int pxff_so3_deidrj(const pxff_so3_params *p, int64_t nimg, const int64_t *d_img
                     const double *d_lnorm, const double *d_a, const double *d_radial, double *d_dedr,
                     void *stream);
 
+/* pxff_so3_deidrj over an image capacity: images e < min(cap, *d_count),
+ * the count read on the device (as pxff_so4_deidrj_counted), for a HIP
+ * graph capture of a whole SO3 E+F step.  Replaces the same reference
+ * contraction as pxff_so3_deidrj (SO3.py:572-694 with neuralnetwork.py:
+ * 484-497); d_dedr [cap*3], entries at and beyond the count not written. */
+int pxff_so3_deidrj_counted(const pxff_so3_params *p, int64_t cap, const int64_t *d_count,
+                            const int64_t *d_img_atom, const int64_t *d_img_j, const double *d_img_r,
+                            const int64_t *d_atom_z, const double *d_G, const double *d_rq, const double *d_lnorm,
+                            const double *d_a, const double *d_radial, double *d_dedr, void *stream);
+
 /* Plain path: dP/dr per image [nimg*ncoef*3] (then pxff_so4_rows builds the
  * reference's dxdr / rdxdr rows, whose row rules SO3.py:243-265 shares). */
 int pxff_so3_dpdr(const pxff_so3_params *p, int64_t nimg, const int64_t *d_img_atom, const int64_t *d_img_j,

@@ -42,12 +42,12 @@ def build(force=False):
         if all(os.path.getmtime(s) <= mt for s in srcs):
             return _LIB_PATH
     objs = []
-    for s, flags in ((srcs[0], ["-O3", "-march=x86-64-v3"]), (srcs[1], ["-O2", "-ffp-contract=off"])):
+    for s, flags in ((srcs[0], ["-O3", "-march=x86-64-v3", "-fopenmp"]), (srcs[1], ["-O2", "-ffp-contract=off"])):
         o = os.path.join(_BUILD, os.path.basename(s)[:-2] + ".o")
         subprocess.check_call(["gcc", "-std=gnu99", "-fPIC", "-c", s, "-o", o] + flags)
         objs.append(o)
     tmp = _LIB_PATH + ".tmp%d" % os.getpid()
-    subprocess.check_call(["gcc", "-shared", "-o", tmp] + objs + ["-lm"])
+    subprocess.check_call(["gcc", "-shared", "-fopenmp", "-o", tmp] + objs + ["-lm"])
     os.replace(tmp, _LIB_PATH)
     return _LIB_PATH
 
@@ -63,6 +63,8 @@ def lib():
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_double, P, P, P, P]
         L.so4_oracle_compute.restype = ctypes.c_int
+        L.so4_oracle_compute_mt.argtypes = L.so4_oracle_compute.argtypes + [ctypes.c_int]
+        L.so4_oracle_compute_mt.restype = ctypes.c_int
         L.so4_oracle_sizes.argtypes = [ctypes.c_int, P, P, P, P]
         L.so4_oracle_tables.argtypes = [ctypes.c_int, P, P, P, P]
         L.so4_oracle_uarray.argtypes = [ctypes.c_int] + [ctypes.c_double] * 5 + [P, P]
@@ -170,8 +172,9 @@ CUTOFF_IDS = {"cosine": 1, "tanh": 2, "poly1": 3, "poly2": 4, "poly3": 5, "poly4
 
 
 def so4_components(layout, twol, rcut, norm=False, derivative=True, stress=True, cutoff="cosine",
-                   rfac0=0.99363, want_z=False):
-    """get_bispectrum_components (SNAP.py:1289-1762) on a reference layout."""
+                   rfac0=0.99363, want_z=False, threads=1):
+    """get_bispectrum_components (SNAP.py:1289-1762) on a reference layout;
+    threads > 1 processes centres on that many OpenMP threads (same values)."""
     ca = layout["center_atoms"]
     nl = layout["neighborlist"]
     seq = layout["seq"]
@@ -182,16 +185,16 @@ def so4_components(layout, twol, rcut, norm=False, derivative=True, stress=True,
     s = so4_sizes(twol)
     nb, nz = s["nb"], s["nz"]
     bl = np.zeros((N, nb))
-    db = np.zeros((P, nb, 3)) if derivative else None
-    bs = np.zeros((P, nb, 3, 3)) if (derivative and stress) else None
+    db = np.empty((P, nb, 3)) if derivative else None  # zeroed (in parallel) by the C code
+    bs = np.empty((P, nb, 3, 3)) if (derivative and stress) else None
     zl = np.zeros((N, nz, 2)) if want_z else None
     nlc = np.ascontiguousarray(nl if M > 0 else np.zeros((P, 1, 3)))
     nwc = np.ascontiguousarray(nw if M > 0 else np.zeros((P, 1)))
-    rc = lib().so4_oracle_compute(_ptr(np.ascontiguousarray(ca)), N, _ptr(nlc), P, max(M, 1),
+    rc = lib().so4_oracle_compute_mt(_ptr(np.ascontiguousarray(ca)), N, _ptr(nlc), P, max(M, 1),
                                   _ptr(np.ascontiguousarray(seq)), _ptr(nwc),
                                   _ptr(np.ascontiguousarray(sw)), int(twol), float(rcut), int(bool(norm)),
                                   int(bool(derivative)), int(bool(stress)), CUTOFF_IDS[cutoff], float(rfac0),
-                                  _ptr(bl), _ptr(db), _ptr(bs), _ptr(zl))
+                                  _ptr(bl), _ptr(db), _ptr(bs), _ptr(zl), int(threads))
     if rc != 0:
         raise ValueError("not implemented")
     out = dict(blist=bl, dblist=db, bstress=bs)
@@ -201,13 +204,13 @@ def so4_components(layout, twol, rcut, norm=False, derivative=True, stress=True,
 
 
 def snap_calculate(positions, cell, pbc, symbols, weights, lmax=3, rcut=3.5, derivative=True, stress=False,
-                   normalize_U=False, cutoff_function="cosine", rfac0=0.99363):
+                   normalize_U=False, cutoff_function="cosine", rfac0=0.99363, threads=1):
     """SNAP.py::SO4_Bispectrum.calculate (SNAP.py:200-238): dict x, dxdr,
     rdxdr (per pair row, -bstress/V), elements, seq."""
     twol = int(round(2 * lmax))
     w_atom = np.array([weights[s] for s in symbols], dtype=np.float64)
     lay = reference_layout(positions, cell, pbc, rcut, w_atom, np.ones(len(positions)))
-    out = so4_components(lay, twol, rcut, normalize_U, derivative, stress, cutoff_function, rfac0)
+    out = so4_components(lay, twol, rcut, normalize_U, derivative, stress, cutoff_function, rfac0, threads=threads)
     vol = abs(np.linalg.det(np.asarray(cell, dtype=np.float64)))
     return _assemble(out, lay, symbols, derivative, stress, vol)
 

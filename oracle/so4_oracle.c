@@ -534,104 +534,44 @@ int so4_oracle_uarray(int twol, double x, double y, double z, double psi, double
     return 0;
 }
 
-/* get_bispectrum_components, SNAP.py:1289-1762 (all three branches).
- *   center_atoms [N][3], neighborlist [P][M][3], seq [P][2], neighbor_w [P][M],
- *   site_w [N]; outputs (real parts): blist [N][nb], dblist [P][nb][3],
- *   bstress [P][nb][3][3]; zlist_out (optional) [N][nz] complex interleaved.
- * Returns 0, or -1 for an unknown cutoff id (the reference raises). */
-int so4_oracle_compute(const double *center_atoms, int64_t N, const double *neighborlist, int64_t P,
-                       int64_t M, const int64_t *seq, const double *neighbor_w, const double *site_w,
-                       int twol, double rcut, int norm, int derivative, int stress, int cutoff_id,
-                       double rfac0, double *blist_out, double *dblist_out, double *bstress_out,
-                       double *zlist_out) {
-    if (cutoff_id < 1 || cutoff_id > 7) return -1;
-    if (P <= 0) return 0;
-    so4_tables t;
-    build_tables(twol, norm, &t);
-    const int nu = t.nu, nz = t.nz, nb = t.nb;
-    cplx *ulisttot = calloc(nu, sizeof(cplx));
-    cplx *zlist = calloc(nz, sizeof(cplx));
-    cplx *ulist = calloc(nu, sizeof(cplx));
-    cplx *dulist = calloc(3 * (size_t)nu, sizeof(cplx));
-    cplx *tempdb = calloc(3 * (size_t)nb, sizeof(cplx));
-    cplx *bl = calloc(nb, sizeof(cplx));
-    cplx *dbl = derivative ? calloc((size_t)P * nb * 3, sizeof(cplx)) : NULL;
-    cplx *bst = (derivative && stress) ? calloc((size_t)P * nb * 9, sizeof(cplx)) : NULL;
-    if (blist_out) memset(blist_out, 0, sizeof(double) * N * nb);
+typedef struct {
+    cplx *ulisttot, *zlist, *ulist, *dulist, *tempdb, *bl;
+} so4_scratch;
 
-    int64_t isite = seq[0], nstart = 0, nsite = 0;
-    for (int64_t n = 0; n <= P; n++) {
-        int64_t i = 0, j = 0;
-        int last = (n == P);
-        if (!last) {
-            i = seq[2 * n];
-            j = seq[2 * n + 1];
-        }
-        if (last || i != isite) {
-            /* finish centre isite */
-            memset(ulist, 0, sizeof(cplx) * nu);
-            addself_uarraytot(&t, ulist, cutoff_id, rcut);
-            for (int u = 0; u < nu; u++) ulist[u] *= site_w[isite];
-            for (int u = 0; u < nu; u++) {
-                ulisttot[u] += ulist[u];
-                ulisttot[u] *= t.u_norm[u];
-            }
-            compute_zi(&t, ulisttot, zlist);
-            compute_bi(&t, ulisttot, zlist, bl);
-            if (blist_out)
-                for (int k = 0; k < nb; k++) blist_out[isite * nb + k] = creal(bl[k]);
-            if (zlist_out)
-                for (int k = 0; k < nz; k++) {
-                    zlist_out[2 * ((size_t)isite * nz + k)] = creal(zlist[k]);
-                    zlist_out[2 * ((size_t)isite * nz + k) + 1] = cimag(zlist[k]);
-                }
-            if (derivative) {
-                for (int64_t NN = nstart; NN < n; NN++) {
-                    int64_t II = seq[2 * NN], JJ = seq[2 * NN + 1];
-                    const double *Ri = center_atoms + 3 * II;
-                    double Weight = neighbor_w[NN * M + 0];
-                    for (int64_t nb_ = 0; nb_ < M; nb_++) {
-                        const double *v = neighborlist + 3 * (NN * M + nb_);
-                        double x = v[0], y = v[1], z = v[2];
-                        double r = sqrt(x * x + y * y + z * z);
-                        if (r < 1e-8) continue;
-                        double psi = rfac0 * M_PI * r / rcut;
-                        memset(ulist, 0, sizeof(cplx) * nu);
-                        memset(dulist, 0, sizeof(cplx) * 3 * nu);
-                        compute_uarray_recursive(&t, x, y, z, psi, r, ulist);
-                        compute_duarray_recursive(&t, x, y, z, psi, r, ulist, dulist);
-                        dudr(&t, x, y, z, r, rcut, ulist, dulist, cutoff_id);
-                        for (int u = 0; u < nu; u++)
-                            for (int k = 0; k < 3; k++) {
-                                dulist[3 * u + k] *= Weight;
-                                dulist[3 * u + k] *= t.u_norm[u];
-                            }
-                        memset(tempdb, 0, sizeof(cplx) * 3 * nb);
-                        compute_dbidrj(&t, dulist, zlist, tempdb);
-                        if (II != JJ)
-                            for (int q = 0; q < 3 * nb; q++) {
-                                dbl[(size_t)NN * nb * 3 + q] += tempdb[q];
-                                dbl[(size_t)nsite * nb * 3 + q] -= tempdb[q];
-                            }
-                        if (stress) {
-                            double Rj[3] = {x + Ri[0], y + Ri[1], z + Ri[2]};
-                            for (int k = 0; k < nb; k++)
-                                for (int a = 0; a < 3; a++)
-                                    for (int c = 0; c < 3; c++) {
-                                        bst[((size_t)nsite * nb + k) * 9 + 3 * a + c] += Ri[a] * tempdb[3 * k + c];
-                                        bst[((size_t)NN * nb + k) * 9 + 3 * a + c] -= Rj[a] * tempdb[3 * k + c];
-                                    }
-                        }
-                    }
-                }
-            }
-            if (last) break;
-            isite = i;
-            nstart = n;
-            memset(ulisttot, 0, sizeof(cplx) * nu);
-            memset(zlist, 0, sizeof(cplx) * nz);
-        }
-        if (i == j) nsite = n;
+static void scratch_alloc(const so4_tables *t, so4_scratch *w) {
+    w->ulisttot = calloc(t->nu, sizeof(cplx));
+    w->zlist = calloc(t->nz, sizeof(cplx));
+    w->ulist = calloc(t->nu, sizeof(cplx));
+    w->dulist = calloc(3 * (size_t)t->nu, sizeof(cplx));
+    w->tempdb = calloc(3 * (size_t)t->nb, sizeof(cplx));
+    w->bl = calloc(t->nb, sizeof(cplx));
+}
+
+static void scratch_free(so4_scratch *w) {
+    free(w->ulisttot);
+    free(w->zlist);
+    free(w->ulist);
+    free(w->dulist);
+    free(w->tempdb);
+    free(w->bl);
+}
+
+/* One centre: the rows [nstart, nend) of seq all have centre isite
+ * (SNAP.py:1428-1548 for one value of isite).  It writes only the outputs of
+ * its own rows and atom, so centres are independent. */
+static void so4_centre(const so4_tables *t, int64_t isite, int64_t nstart, int64_t nend, const double *center_atoms,
+                       const double *neighborlist, int64_t M, const int64_t *seq, const double *neighbor_w,
+                       const double *site_w, double rcut, int derivative, int stress, int cutoff_id, double rfac0,
+                       so4_scratch *w, double *blist_out, double *dbl, double *bst, double *zlist_out) {
+    const int nu = t->nu, nz = t->nz, nb = t->nb;
+    cplx *ulisttot = w->ulisttot, *zlist = w->zlist, *ulist = w->ulist, *dulist = w->dulist;
+    cplx *tempdb = w->tempdb, *bl = w->bl;
+    memset(ulisttot, 0, sizeof(cplx) * nu);
+    memset(zlist, 0, sizeof(cplx) * nz);
+    int64_t nsite = nstart;
+    /* accumulate U_tot over the centre's images (SNAP.py:1479-1500) */
+    for (int64_t n = nstart; n < nend; n++) {
+        if (seq[2 * n] == seq[2 * n + 1]) nsite = n;
         double weight = neighbor_w[n * M + 0];
         for (int64_t nb_ = 0; nb_ < M; nb_++) {
             const double *v = neighborlist + 3 * (n * M + nb_);
@@ -640,25 +580,136 @@ int so4_oracle_compute(const double *center_atoms, int64_t N, const double *neig
             if (r < 1e-8) continue;
             double psi = rfac0 * M_PI * r / rcut;
             memset(ulist, 0, sizeof(cplx) * nu);
-            compute_uarray_recursive(&t, x, y, z, psi, r, ulist);
+            compute_uarray_recursive(t, x, y, z, psi, r, ulist);
             double sfac = cut_f(r, rcut, cutoff_id);
             for (int u = 0; u < nu; u++) ulisttot[u] += sfac * (ulist[u] * weight);
         }
     }
+    memset(ulist, 0, sizeof(cplx) * nu);
+    addself_uarraytot(t, ulist, cutoff_id, rcut);
+    for (int u = 0; u < nu; u++) ulist[u] *= site_w[isite];
+    for (int u = 0; u < nu; u++) {
+        ulisttot[u] += ulist[u];
+        ulisttot[u] *= t->u_norm[u];
+    }
+    compute_zi(t, ulisttot, zlist);
+    compute_bi(t, ulisttot, zlist, bl);
+    if (blist_out)
+        for (int k = 0; k < nb; k++) blist_out[isite * nb + k] = creal(bl[k]);
+    if (zlist_out)
+        for (int k = 0; k < nz; k++) {
+            zlist_out[2 * ((size_t)isite * nz + k)] = creal(zlist[k]);
+            zlist_out[2 * ((size_t)isite * nz + k) + 1] = cimag(zlist[k]);
+        }
+    if (!derivative) return;
+    /* the derivative pass, U recomputed per image (SNAP.py:1502-1548) */
+    for (int64_t NN = nstart; NN < nend; NN++) {
+        int64_t II = seq[2 * NN], JJ = seq[2 * NN + 1];
+        const double *Ri = center_atoms + 3 * II;
+        double Weight = neighbor_w[NN * M + 0];
+        for (int64_t nb_ = 0; nb_ < M; nb_++) {
+            const double *v = neighborlist + 3 * (NN * M + nb_);
+            double x = v[0], y = v[1], z = v[2];
+            double r = sqrt(x * x + y * y + z * z);
+            if (r < 1e-8) continue;
+            double psi = rfac0 * M_PI * r / rcut;
+            memset(ulist, 0, sizeof(cplx) * nu);
+            memset(dulist, 0, sizeof(cplx) * 3 * nu);
+            compute_uarray_recursive(t, x, y, z, psi, r, ulist);
+            compute_duarray_recursive(t, x, y, z, psi, r, ulist, dulist);
+            dudr(t, x, y, z, r, rcut, ulist, dulist, cutoff_id);
+            for (int u = 0; u < nu; u++)
+                for (int k = 0; k < 3; k++) {
+                    dulist[3 * u + k] *= Weight;
+                    dulist[3 * u + k] *= t->u_norm[u];
+                }
+            memset(tempdb, 0, sizeof(cplx) * 3 * nb);
+            compute_dbidrj(t, dulist, zlist, tempdb);
+            /* the outputs are real parts: Re of a sum is the sum of Re, and
+             * Re(R z) = R Re(z) for real R, so the real parts are kept alone */
+            if (II != JJ)
+                for (int q = 0; q < 3 * nb; q++) {
+                    dbl[(size_t)NN * nb * 3 + q] += creal(tempdb[q]);
+                    dbl[(size_t)nsite * nb * 3 + q] -= creal(tempdb[q]);
+                }
+            if (stress) {
+                double Rj[3] = {x + Ri[0], y + Ri[1], z + Ri[2]};
+                for (int k = 0; k < nb; k++)
+                    for (int a = 0; a < 3; a++)
+                        for (int c = 0; c < 3; c++) {
+                            bst[((size_t)nsite * nb + k) * 9 + 3 * a + c] += Ri[a] * creal(tempdb[3 * k + c]);
+                            bst[((size_t)NN * nb + k) * 9 + 3 * a + c] -= Rj[a] * creal(tempdb[3 * k + c]);
+                        }
+            }
+        }
+    }
+}
 
-    if (dblist_out && dbl)
-        for (size_t q = 0; q < (size_t)P * nb * 3; q++) dblist_out[q] = creal(dbl[q]);
-    if (bstress_out && bst)
-        for (size_t q = 0; q < (size_t)P * nb * 9; q++) bstress_out[q] = creal(bst[q]);
-
-    free(ulisttot);
-    free(zlist);
-    free(ulist);
-    free(dulist);
-    free(tempdb);
-    free(bl);
-    free(dbl);
-    free(bst);
+/*
+ * get_bispectrum_components (SNAP.py:1289-1762).  The reference walks the
+ * seq rows once, finishing a centre when the row's centre changes; here the
+ * centre ranges are found first and each centre is processed by
+ * so4_centre (same arithmetic in the same order per centre).  nthreads > 1
+ * runs centres on that many OpenMP threads (results are identical: centres
+ * share no output).
+ *
+ * Inputs: center_atoms [N][3], neighborlist [P][M][3], seq [P][2],
+ * neighbor_w [P][M], site_w [N].  Outputs (real parts, as the reference
+ * returns them): blist [N][nb]; dblist [P][nb][3]; bstress [P][nb][3][3];
+ * zlist_out (optional) [N][nz] complex interleaved.  Returns 0, or -1 for an
+ * unknown cutoff id (the reference raises), -2 for a missing derivative
+ * output.
+ */
+int so4_oracle_compute_mt(const double *center_atoms, int64_t N, const double *neighborlist, int64_t P, int64_t M,
+                          const int64_t *seq, const double *neighbor_w, const double *site_w, int twol, double rcut,
+                          int norm, int derivative, int stress, int cutoff_id, double rfac0, double *blist_out,
+                          double *dblist_out, double *bstress_out, double *zlist_out, int nthreads) {
+    if (cutoff_id < 1 || cutoff_id > 7) return -1;
+    if (P <= 0) return 0;
+    so4_tables t;
+    build_tables(twol, norm, &t);
+    const int nb = t.nb;
+    if (blist_out) memset(blist_out, 0, sizeof(double) * N * nb);
+    double *dbl = derivative ? dblist_out : NULL;
+    double *bst = (derivative && stress) ? bstress_out : NULL;
+    if ((derivative && !dbl) || (derivative && stress && !bst)) {
+        free_tables(&t);
+        return -2;
+    }
+    /* centre ranges of the (centre-contiguous) seq rows */
+    int64_t *cstart = malloc(sizeof(int64_t) * (P + 1));
+    int64_t nc = 0;
+    for (int64_t n = 0; n < P; n++)
+        if (n == 0 || seq[2 * n] != seq[2 * (n - 1)]) cstart[nc++] = n;
+    cstart[nc] = P;
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+    {
+        /* zero the row outputs in parallel (first touch of their pages) */
+#pragma omp for schedule(static)
+        for (int64_t n = 0; n < P; n++) {
+            if (dbl) memset(dbl + (size_t)n * nb * 3, 0, sizeof(double) * nb * 3);
+            if (bst) memset(bst + (size_t)n * nb * 9, 0, sizeof(double) * nb * 9);
+        }
+        so4_scratch w;
+        scratch_alloc(&t, &w);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t c = 0; c < nc; c++)
+            so4_centre(&t, seq[2 * cstart[c]], cstart[c], cstart[c + 1], center_atoms, neighborlist, M, seq,
+                       neighbor_w, site_w, rcut, derivative, stress, cutoff_id, rfac0, &w, blist_out, dbl, bst,
+                       zlist_out);
+        scratch_free(&w);
+    }
+    free(cstart);
     free_tables(&t);
     return 0;
+}
+
+int so4_oracle_compute(const double *center_atoms, int64_t N, const double *neighborlist, int64_t P, int64_t M,
+                       const int64_t *seq, const double *neighbor_w, const double *site_w, int twol, double rcut,
+                       int norm, int derivative, int stress, int cutoff_id, double rfac0, double *blist_out,
+                       double *dblist_out, double *bstress_out, double *zlist_out) {
+    return so4_oracle_compute_mt(center_atoms, N, neighborlist, P, M, seq, neighbor_w, site_w, twol, rcut, norm,
+                                 derivative, stress, cutoff_id, rfac0, blist_out, dblist_out, bstress_out, zlist_out,
+                                 1);
 }

@@ -69,6 +69,7 @@ _SIGS = {
     "pxff_so3_power": ([P, I64, P, P, P, P], I32),
     "pxff_so3_adjoint_a": ([P, I64, P, P, P, P], I32),
     "pxff_so3_deidrj": ([P, I64, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "pxff_so3_deidrj_counted": ([P, I64, P, P, P, P, P, P, P, P, P, P, P, P], I32),
     "pxff_so3_dpdr": ([P, I64, P, P, P, P, P, P, P, P, P, P, P], I32),
 }
 

@@ -642,8 +642,15 @@ __global__ __launch_bounds__(64) void so3_image_kernel(So3Args a, int64_t nimg, 
                                                         const double *__restrict__ img_r,
                                                         const int64_t *__restrict__ atom_z,
                                                         const double *__restrict__ cin, double *__restrict__ out,
-                                                        const double *__restrict__ rad_in) {
+                                                        const double *__restrict__ rad_in,
+                                                        const int64_t *__restrict__ d_count) {
     extern __shared__ double smem[];
+    // with d_count the grid covers an image capacity and the images in use
+    // are counted on the device
+    if (d_count) {
+        const int64_t c = *d_count;
+        nimg = c < nimg ? c : nimg;
+    }
     const int L = a.lmax + 1;
     const int LP = a.lmax + 1;
     const int NG = 64 / L;
@@ -1015,7 +1022,7 @@ extern "C" int pxff_so3_adjoint_a(const pxff_so3_params *p, int64_t natoms, cons
 static int so3_image_launch(const pxff_so3_params *p, int mode, int64_t nimg, const int64_t *d_img_atom,
                             const int64_t *d_img_j, const double *d_img_r, const int64_t *d_atom_z, const double *d_G,
                             const double *d_rq, const double *d_lnorm, const double *d_cin, double *d_out,
-                            const double *d_radial, void *stream) {
+                            const double *d_radial, void *stream, const int64_t *d_count = nullptr) {
     int rc = so3_check(p);
     if (rc) return rc;
     if (nimg <= 0) return 0;
@@ -1029,13 +1036,13 @@ static int so3_image_launch(const pxff_so3_params *p, int mode, int64_t nimg, co
         SO3_DISPATCH_NMAX(p->nmax, {
             auto k = so3_image_kernel<NM, SO3_ADJOINT>;
             if ((rc = launch_set_lds(k, lds))) return rc;
-            k<<<grid, 64, lds, s>>>(a, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_cin, d_out, d_radial);
+            k<<<grid, 64, lds, s>>>(a, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_cin, d_out, d_radial, d_count);
         });
     } else {
         SO3_DISPATCH_NMAX(p->nmax, {
             auto k = so3_image_kernel<NM, SO3_PLAIN>;
             if ((rc = launch_set_lds(k, lds))) return rc;
-            k<<<grid, 64, lds, s>>>(a, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_cin, d_out, d_radial);
+            k<<<grid, 64, lds, s>>>(a, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_cin, d_out, d_radial, d_count);
         });
     }
     PXFF_LAUNCH_CHECK();
@@ -1048,6 +1055,16 @@ extern "C" int pxff_so3_deidrj(const pxff_so3_params *p, int64_t nimg, const int
                                const double *d_radial, double *d_dedr, void *stream) {
     return so3_image_launch(p, SO3_ADJOINT, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_G, d_rq, d_lnorm, d_a,
                             d_dedr, d_radial, stream);
+}
+
+extern "C" int pxff_so3_deidrj_counted(const pxff_so3_params *p, int64_t cap, const int64_t *d_count,
+                                       const int64_t *d_img_atom, const int64_t *d_img_j, const double *d_img_r,
+                                       const int64_t *d_atom_z, const double *d_G, const double *d_rq,
+                                       const double *d_lnorm, const double *d_a, const double *d_radial,
+                                       double *d_dedr, void *stream) {
+    PXFF_REQUIRE(d_count != nullptr, "null image count");
+    return so3_image_launch(p, SO3_ADJOINT, cap, d_img_atom, d_img_j, d_img_r, d_atom_z, d_G, d_rq, d_lnorm, d_a,
+                            d_dedr, d_radial, stream, d_count);
 }
 
 extern "C" int pxff_so3_dpdr(const pxff_so3_params *p, int64_t nimg, const int64_t *d_img_atom,

@@ -279,6 +279,55 @@ __device__ inline void sweep_rows(int mb, const double *__restrict__ rpq, const 
     }
 }
 
+// ------------------------------------------------------ su(2) generators
+// U^l is a representation of SU(2): U^l(g h) = U^l(g) U^l(h) for the
+// Cayley-Klein matrix g = [[conj a, -conj b], [b, a]] of an image.  Hence
+// dU^l/dr_k = U^l M^l(xi_k) with xi_k = g^-1 dg/dr_k = [[i p, q], [-conj q,
+// -i p]] in su(2), and M^l(xi) is tridiagonal in the column index:
+//   M[m][m] = i p (l - 2m),  M[m][m+1] = c_m q,  M[m+1][m] = -c_m conj q,
+//   c_m = sqrt((m + 1)(l - m)).
+// Right multiplication keeps the row, so on the half rows the adjoint
+// contraction of compute_duarray_recursive / compute_dbidrj (SNAP.py:
+// 958-1017, 1143-1257) with Y becomes
+//   Re sum conj(dU_k) Y = p_k A1 + Re q_k A2 + Im q_k A3,
+//   A1 = sum (l-2m) Im(conj U_m Y_m),
+//   A2 = sum c_m Re(conj U_m Y_m+1 - conj U_m+1 Y_m),
+//   A3 = sum c_m Im(conj U_m Y_m+1 + conj U_m+1 Y_m),
+// so the sweep needs U alone (no dU recursion, no derivative registers).
+// p_k, q_k from a, b and their gradients: i p = a conj(da) + conj(b) db,
+// q = conj(b) da - a conj(db).
+__device__ inline void su2_generators(const CK &ck, double p[3], cd q[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const cd ca = ck.ca, cb = ck.cb, cda = ck.cda[k], cdb = ck.cdb[k];  // conjugates of a, b, da, db
+        p[k] = (ca.re * cda.im - ca.im * cda.re) + (cb.im * cdb.re - cb.re * cdb.im);
+        q[k] = cmake((cb.re * cda.re + cb.im * cda.im) - (ca.re * cdb.re + ca.im * cdb.im),
+                     (cb.im * cda.re - cb.re * cda.im) - (ca.re * cdb.im - ca.im * cdb.re));
+    }
+}
+
+// Accumulate A_u = sum Re(conj U_m Y_m) and A1..A3 (above) of one half row
+// of level l (u: the row, y: Y of the same row).
+template <int TWOJ>
+__device__ inline void generator_dots(int l, const cd *u, const cd *y, double &au, double &a1, double &a2,
+                                      double &a3) {
+#pragma unroll
+    for (int m = 0; m <= TWOJ; m++) {
+        if (m > l) continue;
+        au = fma(u[m].re, y[m].re, fma(u[m].im, y[m].im, au));
+        if (l != 2 * m) a1 = fma((double)(l - 2 * m), u[m].re * y[m].im - u[m].im * y[m].re, a1);
+        if (m < l) {
+            const double c = sqrt((double)((m + 1) * (l - m)));  // folded per (l, m)
+            const double t1 = fma(u[m].re, y[m + 1].re, u[m].im * y[m + 1].im) -
+                              fma(u[m + 1].re, y[m].re, u[m + 1].im * y[m].im);
+            const double t2 = fma(u[m].re, y[m + 1].im, -u[m].im * y[m + 1].re) +
+                              fma(u[m + 1].re, y[m].im, -u[m + 1].im * y[m].re);
+            a2 = fma(c, t1, a2);
+            a3 = fma(c, t2, a3);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- U_tot
 // One wavefront per centre (64 threads); NG groups of G lanes each take one
 // image per round; per-group partial sums live in LDS and are reduced in a
@@ -287,7 +336,7 @@ __device__ inline void sweep_rows(int mb, const double *__restrict__ rpq, const 
 // The U_tot sums hold no derivative rows: registers are kept within three
 // wavefronts per SIMD (the tangent variant needs more than that).
 template <int TWOJ, bool TANGENT, bool TILED, int MODE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TANGENT ? 1 : 3))) void so4_accum_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void so4_accum_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
                                                         const int64_t *__restrict__ img_j,
                                                         const double *__restrict__ img_r,
                                                         const double *__restrict__ atom_w,
@@ -312,8 +361,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TANGENT ? 1 
     __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
     __shared__ double lnorm[TWOJ + 1];
     // staged per image (64 at a time): ca, cb, fc w (0: no image); with
-    // TANGENT ca, cb, conj(da/dr_k), conj(db/dr_k), fc, dfc (rhat . dr), w (0: no image), dr
-    __shared__ double stg[TANGENT ? 22 : 5][64];
+    // TANGENT ca, cb, w fc p(eta), w fc q(eta), w dfc (rhat . dr), valid
+    __shared__ double stg[TANGENT ? 9 : 5][64];
     const int lane = threadIdx.x;
     const int g = lane / G, mb = lane % G;
     for (int k = lane; k < (TWOJ + 1) * (TWOJ + 1); k += 64) rpq[k] = rootpq_g[k];
@@ -402,6 +451,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TANGENT ? 1 
                 __syncthreads();  // the next 64 images reuse the stage
             }
         } else {
+            // forward-mode: dU . dr_e = U M(eta_e) with eta_e = sum_k dr_k xi_k
+            // (the su(2) generators of the adjoint kernel), so a row needs U
+            // alone: (U M(eta))_m = i p (l-2m) U_m + c_m-1 q U_m-1 - c_m conj(q) U_m+1
             for (int64_t b64 = s; b64 < e; b64 += 64) {
                 const int64_t e64 = b64 + 64 < e ? b64 + 64 : e;
                 {
@@ -426,70 +478,71 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TANGENT ? 1 
                     cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
                     CK ck;
                     cayley_klein(x, y, z, r, rfac0 * M_PI * r / rcut, true, ck);
+                    double p[3];
+                    cd q[3];
+                    su2_generators(ck, p, q);
                     double d0 = 0, d1 = 0, d2 = 0;
                     if (ok) {
                         d0 = dr[3 * img];
                         d1 = dr[3 * img + 1];
                         d2 = dr[3 * img + 2];
                     }
+                    const double wf = ok ? w * fc : 0.0;
                     stg[0][lane] = ck.ca.re;
                     stg[1][lane] = ck.ca.im;
                     stg[2][lane] = ck.cb.re;
                     stg[3][lane] = ck.cb.im;
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        stg[4 + 2 * k][lane] = ck.cda[k].re;
-                        stg[5 + 2 * k][lane] = ck.cda[k].im;
-                        stg[10 + 2 * k][lane] = ck.cdb[k].re;
-                        stg[11 + 2 * k][lane] = ck.cdb[k].im;
-                    }
+                    stg[4][lane] = wf * (p[0] * d0 + p[1] * d1 + p[2] * d2);
+                    stg[5][lane] = wf * (q[0].re * d0 + q[1].re * d1 + q[2].re * d2);
+                    stg[6][lane] = wf * (q[0].im * d0 + q[1].im * d1 + q[2].im * d2);
                     // D . dr = fc (dU . dr) + dfc (rhat . dr) U
-                    stg[16][lane] = fc;
-                    stg[17][lane] = dfc * (x * d0 + y * d1 + z * d2) / r;
-                    stg[18][lane] = ok ? w : 0.0;  // a zero weight adds nothing either way
-                    stg[19][lane] = d0;
-                    stg[20][lane] = d1;
-                    stg[21][lane] = d2;
+                    stg[7][lane] = ok ? w * dfc * (x * d0 + y * d1 + z * d2) / r : 0.0;
+                    stg[8][lane] = ok ? 1.0 : 0.0;
                 }
                 __syncthreads();
                 for (int64_t base = b64; base < e64; base += NG) {
-                const int64_t img = base + g;
-                const int si = (g < NG && img < e64) ? (int)(img - b64) : 0;
-                const bool valid = (g < NG) && (img < e64) && stg[18][si] != 0.0;
-                CK ck;
-                ck.ca = cmake(stg[0][si], stg[1][si]);
-                ck.cb = cmake(stg[2][si], stg[3][si]);
+                    const int64_t img = base + g;
+                    const int si = (g < NG && img < e64) ? (int)(img - b64) : 0;
+                    const bool valid = (g < NG) && (img < e64) && stg[8][si] != 0.0;
+                    CK ck;
+                    ck.ca = cmake(stg[0][si], stg[1][si]);
+                    ck.cb = cmake(stg[2][si], stg[3][si]);
+                    const double P = stg[4][si], Qr = stg[5][si], Qi = stg[6][si], Gu = stg[7][si];
+                    const int pidx = g < NG ? g % NP : 0;  // index the __shared__ array itself: LDS atomics
+                    sweep_rows<TWOJ, false>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
+                        const bool act = valid && half;
+                        const int h0 = hoff_of(l) + mb * (l + 1);
 #pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    ck.cda[k] = cmake(stg[4 + 2 * k][si], stg[5 + 2 * k][si]);
-                    ck.cdb[k] = cmake(stg[10 + 2 * k][si], stg[11 + 2 * k][si]);
-                }
-                const double fc = stg[16][si], gu = stg[17][si], w = stg[18][si];
-                const double d0 = stg[19][si], d1 = stg[20][si], d2 = stg[21][si];
-                const int pidx = g < NG ? g % NP : 0;  // index the __shared__ array itself: LDS atomics
-                sweep_rows<TWOJ, true>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
-                    const bool act = valid && half;
-                    const int h0 = hoff_of(l) + mb * (l + 1);
+                        for (int pass = 0; pass < NPASS; pass++) {
+                            if (act && (NPASS == 1 || g / NP == pass)) {
 #pragma unroll
-                    for (int pass = 0; pass < NPASS; pass++) {
-                        if (act && (NPASS == 1 || g / NP == pass)) {
-#pragma unroll
-                            for (int m = 0; m <= TWOJ; m++) {
-                                if (m > l) continue;
-                                double re = fc * (du[0][m].re * d0 + du[1][m].re * d1 + du[2][m].re * d2) + gu * u[m].re;
-                                double im = fc * (du[0][m].im * d0 + du[1][m].im * d1 + du[2][m].im * d2) + gu * u[m].im;
-                                if (LDSADD) {
-                                    atomicAdd(&part[pidx][h0 + m].re, w * re);
-                                    atomicAdd(&part[pidx][h0 + m].im, w * im);
-                                } else {
-                                    cd v = part[pidx][h0 + m];
-                                    part[pidx][h0 + m] = cmake(v.re + w * re, v.im + w * im);
+                                for (int m = 0; m <= TWOJ; m++) {
+                                    if (m > l) continue;
+                                    const double pm = P * (double)(l - 2 * m);
+                                    double re = fma(-pm, u[m].im, Gu * u[m].re);
+                                    double im = fma(pm, u[m].re, Gu * u[m].im);
+                                    if (m >= 1) {
+                                        const double c = sqrt((double)(m * (l - m + 1)));  // c_(m-1)
+                                        re = fma(c, Qr * u[m - 1].re - Qi * u[m - 1].im, re);
+                                        im = fma(c, Qr * u[m - 1].im + Qi * u[m - 1].re, im);
+                                    }
+                                    if (m < l) {
+                                        const double c = sqrt((double)((m + 1) * (l - m)));  // c_m
+                                        re = fma(-c, Qr * u[m + 1].re + Qi * u[m + 1].im, re);
+                                        im = fma(-c, Qr * u[m + 1].im - Qi * u[m + 1].re, im);
+                                    }
+                                    if (LDSADD) {
+                                        atomicAdd(&part[pidx][h0 + m].re, re);
+                                        atomicAdd(&part[pidx][h0 + m].im, im);
+                                    } else {
+                                        cd v = part[pidx][h0 + m];
+                                        part[pidx][h0 + m] = cmake(v.re + re, v.im + im);
+                                    }
                                 }
                             }
+                            if (NPASS > 1) __syncthreads();
                         }
-                        if (NPASS > 1) __syncthreads();
-                    }
-                });
+                    });
                 }
                 __syncthreads();  // the next 64 images reuse the stage
             }
@@ -634,13 +687,13 @@ __global__ __launch_bounds__(256) void so4_yi_kernel(int64_t natoms, int ntasks,
 }
 
 // -------------------------------------------------------- adjoint dE/dr
-constexpr int DEIDRJ_NST = 22;  // staged doubles per image of the dE/dr kernel
+constexpr int DEIDRJ_NST = 16;  // staged doubles per image of the dE/dr kernel
 // Flat over images: a wavefront serves NB = NR x NG images (any centres).
-// It first stages, one lane per image, the Cayley-Klein parameters with
-// their gradients, the cutoff and its derivative, the weight and r-hat of
-// its NB images in LDS (the G lanes of an image would otherwise each
-// compute them), then sweeps them in NR rounds of NG images.  Per image:
-// dE/dr_k = w Re sum_h conj(fc dU_k[h] + dfc rhat_k U[h]) Y_i[h].
+// It first stages, one lane per image, the Cayley-Klein parameters and the
+// per-image factors of the generator contraction in LDS (the G lanes of an
+// image would otherwise each compute them), then sweeps them in NR rounds
+// of NG images:
+//   dE/dr_k = w fc (p_k A1 + Re q_k A2 + Im q_k A3) + w dfc rhat_k A_u.
 template <int TWOJ>
 __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int64_t *__restrict__ img_atom,
                                                            const int64_t *__restrict__ img_j,
@@ -703,23 +756,23 @@ __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int
         cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
         CK ck;
         cayley_klein(x, yy, zz, r, rfac0 * M_PI * r / rcut, true, ck);
+        double p[3];
+        cd q[3];
+        su2_generators(ck, p, q);
         S[0][lane] = ck.ca.re;
         S[1][lane] = ck.ca.im;
         S[2][lane] = ck.cb.re;
         S[3][lane] = ck.cb.im;
+        const double wf = w * fc, wd = w * dfc / r;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            S[4 + 2 * k][lane] = ck.cda[k].re;
-            S[5 + 2 * k][lane] = ck.cda[k].im;
-            S[10 + 2 * k][lane] = ck.cdb[k].re;
-            S[11 + 2 * k][lane] = ck.cdb[k].im;
+            S[4 + k][lane] = wf * p[k];
+            S[7 + k][lane] = wf * q[k].re;
+            S[10 + k][lane] = wf * q[k].im;
         }
-        S[16][lane] = fc;
-        S[17][lane] = dfc;
-        S[18][lane] = w;
-        S[19][lane] = x / r;
-        S[20][lane] = yy / r;
-        S[21][lane] = zz / r;
+        S[13][lane] = wd * x;
+        S[14][lane] = wd * yy;
+        S[15][lane] = wd * zz;
         s_at[lane] = ok ? i : -1;
     }
     // the rounds read lanes of the same wavefront only
@@ -734,40 +787,32 @@ __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int
         CK ck;
         ck.ca = cmake(S[0][slot], S[1][slot]);
         ck.cb = cmake(S[2][slot], S[3][slot]);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            ck.cda[k] = cmake(S[4 + 2 * k][slot], S[5 + 2 * k][slot]);
-            ck.cdb[k] = cmake(S[10 + 2 * k][slot], S[11 + 2 * k][slot]);
-        }
         const cd *yi = reinterpret_cast<const cd *>(y) + (size_t)(valid ? i : 0) * NH;
-        double au = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        sweep_rows<TWOJ, true>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
+        double au = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        sweep_rows<TWOJ, false>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
             if (!valid || !half) return;
-            const int h0 = hoff_of(l) + mb * (l + 1);
+            const cd *yr = yi + hoff_of(l) + mb * (l + 1);
+            cd yv[TWOJ + 1];
 #pragma unroll
-            for (int m = 0; m <= TWOJ; m++) {
-                if (m > l) continue;
-                const cd yv = yi[h0 + m];
-                au += cdotre(u[m], yv);
-                a0 += cdotre(du[0][m], yv);
-                a1 += cdotre(du[1][m], yv);
-                a2 += cdotre(du[2][m], yv);
-            }
+            for (int m = 0; m <= TWOJ; m++)
+                if (m <= l) yv[m] = yr[m];
+            generator_dots<TWOJ>(l, u, yv, au, a1, a2, a3);
         });
         red[lane][0] = au;
-        red[lane][1] = a0;
-        red[lane][2] = a1;
-        red[lane][3] = a2;
+        red[lane][1] = a1;
+        red[lane][2] = a2;
+        red[lane][3] = a3;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (valid && mb == 0) {
             double sm[4] = {0, 0, 0, 0};
-            for (int q = 0; q < G; q++)
-                for (int k = 0; k < 4; k++) sm[k] += red[lane + q][k];
-            const double fc = S[16][slot], dfc = S[17][slot], w = S[18][slot];
+            for (int qq = 0; qq < G; qq++)
+                for (int k = 0; k < 4; k++) sm[k] += red[lane + qq][k];
             const int64_t img = base + slot;
-            for (int k = 0; k < 3; k++) dedr[3 * img + k] = w * (fc * sm[1 + k] + dfc * S[19 + k][slot] * sm[0]);
+            for (int k = 0; k < 3; k++)
+                dedr[3 * img + k] = S[4 + k][slot] * sm[1] + S[7 + k][slot] * sm[2] + S[10 + k][slot] * sm[3] +
+                                    S[13 + k][slot] * sm[0];
         } else if (g < NG && mb == 0 && base + slot < nimg) {
             // an image of coincident atoms (r < 1e-8) contributes nothing, as
             // the reference skips it (SNAP.py:1487)
@@ -1301,14 +1346,9 @@ extern "C" int pxff_so4_forces_energy(int64_t natoms, const int64_t *d_img_start
     return 0;
 }
 
-extern "C" int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start,
-                                const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
-                                const double *d_dr, const double *d_z, double *d_dutot, double *d_db, void *stream) {
-    int rc = so4_check_params(p);
-    if (rc) return rc;
-    if (natoms <= 0) return 0;
-    const So4Dev *T = nullptr;
-    if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
+static int so4_tangent_launch(const pxff_so4_params *p, const So4Dev *T, int64_t natoms, const int64_t *d_img_start,
+                              const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
+                              const double *d_dr, const double *d_z, double *d_dutot, double *d_db, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * g_tune[PXFF_TUNE_ACCUM_WAVES_PER_CU]);
 #define PXFF_ACCUM_TAN(MODE)                                                                      \
@@ -1332,6 +1372,17 @@ extern "C" int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const 
         T->blk_level_start, T->blk_zoff, T->b_src, T->b_srcf, d_db);
     PXFF_LAUNCH_CHECK();
     return 0;
+}
+
+extern "C" int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start,
+                                const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
+                                const double *d_dr, const double *d_z, double *d_dutot, double *d_db, void *stream) {
+    int rc = so4_check_params(p);
+    if (rc) return rc;
+    if (natoms <= 0) return 0;
+    const So4Dev *T = nullptr;
+    if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
+    return so4_tangent_launch(p, T, natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_dr, d_z, d_dutot, d_db, stream);
 }
 
 extern "C" int pxff_so4_z_atom_major(const pxff_so4_params *p, int64_t natoms, const double *d_z, double *d_za,

@@ -19,6 +19,7 @@ import numpy as np
 import torch
 
 from . import adjoint
+from . import so3 as dev_so3
 from . import so4 as dev_so4
 
 EV2GPA = 160.21766208
@@ -200,23 +201,46 @@ def energy_forces_adjoint(nl, state, model, species, stress=False):
     return out
 
 
-def energy_forces_fused(nl, state, model, species_atom, virial=False):
-    """E, F (and the per-centre virial sum) through the adjoint kernels with
-    no autograd: U_tot -> Z -> B -> network energies and dE/dB (fused
-    kernel) -> Y -> dE/dr per image -> forces by the deterministic gather.
-    Same quantities as ``energy_forces_adjoint``; the network parameters are
-    constants here (energy + force evaluation, e.g. MD or the calculator)."""
-    cfg = state.cfg
-    gen = nl.generation
-    b, z, _ = dev_so4.forward(nl, cfg, state.atom_w, state.site_w, state.work_order)
+def _fused_adjoint(nl, cfg, weights, model, species_atom, work_order=None, counted=False):
+    """The descriptor kernels and the fused network of one E+F step up to
+    the per-image sweep: (atomic energies, a function launching the sweep
+    that returns dE/dr per image).  weights: (atom_w, site_w) for SO(4) /
+    SNAP, (atomic numbers,) for SO(3)."""
+    if isinstance(cfg, dev_so3.So3Config):
+        (atom_z,) = weights
+        x, ct, rad = dev_so3.forward(nl, cfg, atom_z, radial=True)
+        e_atom, beta = model.energy_grad(x, species_atom)
+        a = dev_so3.adjoint_a(nl, cfg, ct, beta)
+        return e_atom, lambda: dev_so3.deidrj(nl, cfg, atom_z, a, radial=rad, counted=counted)
+    atom_w, site_w = weights
+    b, z, _ = dev_so4.forward(nl, cfg, atom_w, site_w, work_order)
     e_atom, beta = model.energy_grad(b, species_atom)
     y = dev_so4.yi(nl, cfg, z, beta)
+    return e_atom, lambda: dev_so4.deidrj(nl, cfg, atom_w, y, counted=counted)
+
+
+def state_weights(state):
+    """The weight tensors of an SO4State ((atom_w, site_w)) or SO3State ((atom_z,))."""
+    return (state.atom_z,) if isinstance(state, dev_so3.SO3State) else (state.atom_w, state.site_w)
+
+
+def energy_forces_fused(nl, state, model, species_atom, virial=False):
+    """E, F (and the per-centre virial sum) through the adjoint kernels with
+    no autograd: descriptor (SO(4): U_tot -> Z -> B; SO(3): expansion ->
+    power spectrum) -> network energies and dE/dx (fused kernel) -> Y or A
+    -> dE/dr per image -> forces by the deterministic gather.  Same
+    quantities as ``energy_forces_adjoint``; the network parameters are
+    constants here (energy + force evaluation, e.g. MD or the calculator).
+    state: an ``adjoint.SO4State`` or a ``so3.SO3State``."""
+    gen = nl.generation
+    e_atom, sweep = _fused_adjoint(nl, state.cfg, state_weights(state), model, species_atom,
+                                   getattr(state, "work_order", None))
     # the first use of the host-side image count (the build did not wait
     # for it); if the list had to be rebuilt, everything above used a cut list
     nl.nimg
     if nl.generation != gen:
         return energy_forces_fused(nl, state, model, species_atom, virial)
-    dedr = dev_so4.deidrj(nl, cfg, state.atom_w, y)
+    dedr = sweep()
     # forces and the energy total (one extra workgroup) in one launch
     f, energy, vir = dev_so4.forces_energy(nl, dedr, e_atom, virial=virial)
     out = {"energy": energy[0], "atom_energies": e_atom, "forces": f}
@@ -258,9 +282,12 @@ def _segment_total(v, seg):
 class FusedEFGraph:
     """Energy, forces (and the virial) of a fixed set of atoms as one HIP
     graph replay: the neighbour-list rebuild from the positions in HBM and
-    the fused adjoint kernels of ``energy_forces_fused``, captured once and
-    replayed per step, so no host work sits between the kernels and the step
-    never waits for the host.
+    the fused adjoint kernels of ``energy_forces_fused`` (SO(4)/SNAP or
+    SO(3)), captured once and replayed per step, so no host work sits
+    between the kernels and the step never waits for the host.
+
+    cfg: an ``So4Config`` with weights = (atom_w, site_w), or an
+    ``So3Config`` with weights = (atomic numbers,).
 
     The captured list has a fixed image capacity (the builder's).  Each
     replay's unclamped image count is copied back asynchronously; with
@@ -269,10 +296,11 @@ class FusedEFGraph:
     recovery ``DeviceNeighborList.nimg`` does for eager builds).  The
     returned tensors belong to the graph: the next replay overwrites them.
     Per-image work is launched over the capacity and bounded by the image
-    count on the device (``pxff_so4_deidrj_counted``)."""
+    count on the device (``pxff_so4_deidrj_counted`` /
+    ``pxff_so3_deidrj_counted``)."""
 
-    def __init__(self, builder, cfg, atom_w, site_w, model, species_atom, pos, virial=False, warmup=2):
-        self.builder, self.cfg, self.atom_w, self.site_w = builder, cfg, atom_w, site_w
+    def __init__(self, builder, cfg, weights, model, species_atom, pos, virial=False, warmup=2):
+        self.builder, self.cfg, self.weights = builder, cfg, tuple(weights)
         self.model, self.species_atom, self.virial = model, species_atom, virial
         self.pos = pos.detach().clone().contiguous()
         self._seg = torch.tensor([0, builder.natoms], dtype=torch.int64, device=pos.device)
@@ -284,10 +312,8 @@ class FusedEFGraph:
     def _body(self):
         n, cfg = self.builder.natoms, self.cfg
         nl = self.builder.build(self.pos, host_count=False)
-        b, z, _ = dev_so4.forward(nl, cfg, self.atom_w, self.site_w)
-        e_atom, beta = self.model.energy_grad(b, self.species_atom)
-        y = dev_so4.yi(nl, cfg, z, beta)
-        dedr = dev_so4.deidrj(nl, cfg, self.atom_w, y, counted=True)
+        e_atom, sweep = _fused_adjoint(nl, cfg, self.weights, self.model, self.species_atom, counted=True)
+        dedr = sweep()
         f, energy, vir = dev_so4.forces_energy(nl, dedr, e_atom, virial=self.virial)
         out = {"atom_energies": e_atom, "energy": energy[0], "forces": f}
         if self.virial:

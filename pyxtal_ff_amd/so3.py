@@ -130,12 +130,22 @@ def adjoint_a(nl, cfg, ct, beta):
     return a
 
 
-def deidrj(nl, cfg, atom_z, a, img_r=None, radial=None):
+def deidrj(nl, cfg, atom_z, a, img_r=None, radial=None, counted=False):
     """dE/dr per image [nimg, 3]; radial: the forward's radial integrals (or
-    None to recompute them)."""
+    None to recompute them).  counted: over the list's image capacity with
+    the image count read on the device (no host wait; returns [capacity,
+    3], rows at and beyond the count unwritten)."""
     p = nat.ptr
-    out = _f64(nl.nimg * 3, nl.pos.device)
     r = nl.img_r if img_r is None else img_r.contiguous()
+    if counted:
+        cap = nl.img_capacity
+        out = _f64(cap * 3, nl.pos.device)
+        count = nl.img_start[nl.natoms:]
+        nat.check(nat.lib().pxff_so3_deidrj_counted(cfg.params, cap, p(count), p(nl.img_atom), p(nl.img_j), p(r),
+                                                    p(atom_z), p(cfg.G), p(cfg.rq), p(cfg.lnorm), p(a), p(radial),
+                                                    p(out), nat.stream()), "pxff_so3_deidrj_counted")
+        return out[: cap * 3].view(cap, 3)
+    out = _f64(nl.nimg * 3, nl.pos.device)
     nat.check(nat.lib().pxff_so3_deidrj(cfg.params, nl.nimg, p(nl.img_atom), p(nl.img_j), p(r), p(atom_z), p(cfg.G),
                                         p(cfg.rq), p(cfg.lnorm), p(a), p(radial), p(out), nat.stream()),
               "pxff_so3_deidrj")

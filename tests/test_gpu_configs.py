@@ -185,3 +185,18 @@ def test_c5_full_cell(oracle_mod):
     fa = o["forces"].detach()
     assert rel_err(fa.cpu().numpy(), fp.cpu().numpy()) < 1e-9
     assert float(torch.linalg.norm(fa.sum(0))) < 1e-9 * float(torch.linalg.norm(fa))
+    # the whole E+F step as one HIP graph (neighbour list, expansion, power
+    # spectrum, network, adjoint sweep, gather): bit for bit the eager fused
+    # step, and the autograd path's forces
+    from pyxtal_ff_amd.neighbors import NeighborListBuilder
+    pos = torch.as_tensor(s.positions, dtype=torch.float64, device="cuda")
+    spa = potential.species_atom(b.symbols, EL, "cuda")
+    builder = NeighborListBuilder([len(s)], [s.cell], [s.pbc], 5.0)
+    g = potential.FusedEFGraph(builder, d.cfg, (d.state.atom_z,), model, spa, pos)
+    for _ in range(3):
+        got = {k: v.clone() for k, v in g(pos).items()}
+        torch.cuda.synchronize()
+    nl2 = builder.build(pos)
+    ref = potential.energy_forces_fused(nl2, dso3.SO3State(nl2, d.cfg, d.state.atom_z), model, spa)
+    assert torch.equal(got["forces"], ref["forces"]) and torch.equal(got["atom_energies"], ref["atom_energies"])
+    assert rel_err(got["forces"].cpu().numpy(), fa.cpu().numpy()) < 1e-10

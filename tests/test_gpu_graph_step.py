@@ -41,7 +41,7 @@ def _np(t):
 
 def test_graph_step_matches_eager_over_replays():
     s, cfg, pos, w, model, sp, b = _setup()
-    g = potential.FusedEFGraph(b, cfg, w, w, model, sp, pos, virial=True)
+    g = potential.FusedEFGraph(b, cfg, (w, w), model, sp, pos, virial=True)
     rng = np.random.default_rng(0)
     for it in range(3):
         p = pos + 0.02 * torch.as_tensor(rng.standard_normal(tuple(pos.shape)), device="cuda")
@@ -58,9 +58,52 @@ def test_graph_step_matches_eager_over_replays():
 
 def test_graph_step_grows_a_small_capacity():
     s, cfg, pos, w, model, sp, b = _setup(cap=100)
-    g = potential.FusedEFGraph(b, cfg, w, w, model, sp, pos)
+    g = potential.FusedEFGraph(b, cfg, (w, w), model, sp, pos)
     out = g(pos)
     assert g.captures == 2 and b.img_capacity > 100
     ref = _eager(s, cfg, pos, w, model, sp)
     assert torch.equal(out["forces"], ref["forces"])
     assert torch.equal(out["atom_energies"], ref["atom_energies"])
+
+
+def _so3_setup(s, nmax=6, lmax=6, rc=5.0):
+    from pyxtal_ff_amd import so3 as dso3
+    n = len(s)
+    cfg = dso3.So3Config(nmax, lmax, rc, 2.0)
+    pos = torch.as_tensor(s.positions, dtype=torch.float64, device="cuda")
+    z = torch.as_tensor(s.numbers, dtype=torch.int64, device="cuda")
+    el = ["O", "Si"]
+    model = potential.NNPotential(el, cfg.ncoef, (16, 16), seed=2).cuda()
+    sp = potential.species_atom(s.symbols, el, "cuda")
+    return cfg, pos, z, model, sp, NeighborListBuilder([n], [s.cell], [s.pbc], rc)
+
+
+def test_so3_graph_step_matches_eager_and_autograd():
+    """SO(3) 6/6 E+F step (amorphous SiO2, 300 atoms) as one graph replay:
+    bit for bit the eager fused step over replays with new positions; forces
+    equal the autograd adjoint path's."""
+    from pyxtal_ff_amd import batch as bt
+    from pyxtal_ff_amd import so3 as dso3
+    s = workloads.amorphous_sio2(100, 200, seed=3)
+    cfg, pos, z, model, sp, b = _so3_setup(s)
+    g = potential.FusedEFGraph(b, cfg, (z,), model, sp, pos, virial=True)
+    rng = np.random.default_rng(1)
+    for it in range(3):
+        p = pos + 0.02 * torch.as_tensor(rng.standard_normal(tuple(pos.shape)), device="cuda")
+        got = {k: v.clone() for k, v in g(p).items()}
+        nl = NeighborListBuilder([len(s)], [s.cell], [s.pbc], 5.0).build(p)
+        ref = potential.energy_forces_fused(nl, dso3.SO3State(nl, cfg, z), model, sp, virial=True)
+        print("so3 replay", it, float((got["forces"] - ref["forces"]).abs().max()), flush=True)
+        assert torch.equal(got["atom_energies"], ref["atom_energies"])
+        assert torch.equal(got["forces"], ref["forces"])
+        assert rel_err(_np(got["energy"]), _np(ref["energy"])) < 1e-14
+        assert rel_err(_np(got["virial"]), _np(ref["virial"])) < 1e-13
+    assert g.captures == 1
+    # the autograd adjoint (torch network) on the last positions
+    sb = bt.StructureBatch([s], 5.0)
+    sb.pos = p
+    sb.nl = sb.builder.build(p).resolve()
+    d = bt.BatchedSO3(sb, nmax=6, lmax=6, rcut=5.0, alpha=2.0)
+    o = bt.energy_forces_virial(d, model, potential.species_index(s.symbols, ["O", "Si"], "cuda"), virial=True)
+    assert rel_err(_np(o["forces"]), _np(got["forces"])) < 1e-10
+    assert rel_err(_np(o["virial"][0]), _np(got["virial"])) < 1e-10

@@ -4,6 +4,7 @@ teacher labels); clusters with a zero cell along non-periodic directions;
 atoms with no neighbour; coincident atoms (r < 1e-8, skipped as
 SNAP.py:1441/1487 skip them); and the calculator's reading of the
 descriptor dict (compute_descriptor, utilities/__init__.py:276-308)."""
+import copy
 import os
 
 import numpy as np
@@ -199,9 +200,7 @@ def test_calculator_reads_the_dict_as_compute_descriptor(oracle_mod):
     calc = PyXtalFFCalculator(desc, pot)
     e = calc.get_potential_energy(s)
     f = calc.get_forces(s)
-    pc = potential.NNPotential(pot.elements, pot.nin, drange=None)
-    pc.load_state_dict(pot.state_dict())
-    pc = pc.cpu()
+    pc = copy.deepcopy(calc.model).cpu()
     sp = potential.species_index(s.symbols, pot.elements, "cpu")
     for rfac0, same in ((0.99363, True), (0.8, False)):
         d = oracle_mod.snap_calculate(s.positions, s.cell, s.pbc, s.symbols, {"Si": 1.0}, lmax=3, rcut=5.0,
