@@ -509,6 +509,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched / training / SO3 workloads")
+    ap.add_argument("--extra", default="batched,training,so3",
+                    help="comma-separated secondary workloads to run (batched, training, so3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device copy (scale reference for rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
@@ -540,8 +542,11 @@ def main():
     cpu = None if args.no_cpu else run_cpu_baseline()
     extra = {}
     if not args.no_extra:
-        for name, fn in (("batched_sio_1024", run_batched), ("training_256", run_training),
-                         ("so3_sio2_12000", run_so3_sio2)):
+        chosen = set(args.extra.split(","))
+        for key, name, fn in (("batched", "batched_sio_1024", run_batched), ("training", "training_256", run_training),
+                              ("so3", "so3_sio2_12000", run_so3_sio2)):
+            if key not in chosen:
+                continue
             try:
                 extra[name] = fn()
             except Exception as e:  # a secondary workload must not hide the headline line
