@@ -222,6 +222,22 @@ int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const int64_t *d_
                      const double *d_img_r, const double *d_atom_w, const double *d_dr, const double *d_z,
                      double *d_dutot, double *d_db, void *stream);
 
+/* pxff_so4_tangent without the d_dutot workspace argument, allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving or
+ * training loops, harnesses that forbid hipMalloc).  The workspace is a
+ * static device array of 32 MiB per GPU (held until the process ends), so
+ * ceil(natoms/64)*64*nhalf*16 bytes must fit in it (about 13,500 atoms at
+ * 2J=8; larger inputs return PXFF_EINVAL with a message).  The 2J tables must
+ * already exist on the device (built by an earlier call of an allocating SO4
+ * entry point with the same 2J and normalisation, e.g. pxff_so4_tangent);
+ * otherwise PXFF_EINVAL.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (different streams) race.  Same results as
+ * pxff_so4_tangent. */
+int pxff_so4_tangent_noalloc(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start,
+                             const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w, const double *d_dr,
+                             const double *d_z, double *d_db, void *stream);
+
 /* Atom-major copy of the tile-major Z of pxff_so4_zb: d_za [natoms*nz*2],
  * element e of atom i at i*nz + e (complex). */
 int pxff_so4_z_atom_major(const pxff_so4_params *p, int64_t natoms, const double *d_z, double *d_za, void *stream);

@@ -50,6 +50,20 @@ int upload(const std::vector<T> &v, T **dst) {
 }
 }  // namespace
 
+int find_so4_tables(int twol, int norm, const So4Dev **out) {
+    int dev = 0;
+    PXFF_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_tab_mutex);
+    auto it = g_tabs.find(std::make_pair(dev, twol * 2 + (norm ? 1 : 0)));
+    if (it == g_tabs.end()) {
+        set_error("the 2J=" + std::to_string(twol) + " tables are not built on this device yet: call any allocating "
+                  "SO4 entry point (e.g. pxff_so4_tangent) with the same 2J and normalisation once first");
+        return PXFF_EINVAL;
+    }
+    *out = &it->second;
+    return 0;
+}
+
 int get_so4_tables(int twol, int norm, const So4Dev **out) {
     int dev = 0;
     PXFF_HIP_CHECK(hipGetDevice(&dev));
@@ -1383,6 +1397,52 @@ extern "C" int pxff_so4_tangent(const pxff_so4_params *p, int64_t natoms, const 
     const So4Dev *T = nullptr;
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     return so4_tangent_launch(p, T, natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_dr, d_z, d_dutot, d_db, stream);
+}
+
+// ------------------------------------------------- no-allocation tangent
+// The dU_tot . dr workspace of pxff_so4_tangent in a file-scope device array
+// (one copy per GPU, held until the process ends), for callers that cannot
+// allocate device memory (HIP graph capture, tight loops).  Host code must
+// not use the array's name as a device address: its address on the current
+// device comes from hipGetSymbolAddress, resolved once per device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+static std::mutex g_scratch_mutex;
+static std::map<int, double *> g_scratch_ptr;  // device -> address of scratch there
+
+static int scratch_on_device(double **out) {
+    int dev = 0;
+    PXFF_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        double *ptr = nullptr;
+        PXFF_HIP_CHECK(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+        PXFF_REQUIRE(ptr != nullptr, "no device address for the no-allocation scratch array");
+        it = g_scratch_ptr.emplace(dev, ptr).first;
+    }
+    *out = it->second;
+    return 0;
+}
+
+extern "C" int pxff_so4_tangent_noalloc(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start,
+                                        const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
+                                        const double *d_dr, const double *d_z, double *d_db, void *stream) {
+    int rc = so4_check_params(p);
+    if (rc) return rc;
+    if (natoms <= 0) return 0;
+    const So4Dev *T = nullptr;
+    if ((rc = find_so4_tables(p->twol, p->normalize_u, &T))) return rc;
+    const size_t need = (size_t)((natoms + 63) / 64) * 64 * (size_t)T->nhalf * 2 * sizeof(double);
+    if (need > NOALLOC_SCRATCH_BYTES) {
+        set_error("pxff_so4_tangent_noalloc: " + std::to_string(natoms) + " atoms at 2J=" + std::to_string(p->twol) +
+                  " need " + std::to_string(need) + " bytes of workspace, more than the " +
+                  std::to_string(NOALLOC_SCRATCH_BYTES) + " of the static array (use pxff_so4_tangent)");
+        return PXFF_EINVAL;
+    }
+    double *du = nullptr;
+    if ((rc = scratch_on_device(&du))) return rc;
+    return so4_tangent_launch(p, T, natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_dr, d_z, du, d_db, stream);
 }
 
 extern "C" int pxff_so4_z_atom_major(const pxff_so4_params *p, int64_t natoms, const double *d_z, double *d_za,

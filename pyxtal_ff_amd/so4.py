@@ -177,6 +177,23 @@ def tangent(nl, cfg, atom_w, z, dr, img_r=None):
     return db[: nl.natoms * cfg.nb].view(nl.natoms, cfg.nb)
 
 
+def tangent_noalloc(nl, cfg, atom_w, z, dr, db, img_r=None):
+    """``tangent`` into a caller-owned db [N, nb] with no device allocation
+    (pxff_so4_tangent_noalloc: the dU_tot workspace is the library's static
+    32 MiB array, so ceil(N/64)*64*nhalf*16 bytes must fit; the 2J tables
+    must already exist on the device, e.g. after one ``tangent`` call).
+    Calls that overlap on one GPU share the workspace and race."""
+    p = nat.ptr
+    r = nl.img_r if img_r is None else img_r.contiguous()
+    d = dr.contiguous()
+    if not (db.is_contiguous() and db.dtype == torch.float64 and db.numel() >= nl.natoms * cfg.nb):
+        raise nat.PxffError("db must be a contiguous float64 device tensor of natoms x nb elements")
+    nat.check(nat.lib().pxff_so4_tangent_noalloc(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r),
+                                                 p(atom_w), p(d), p(z), p(db), nat.stream()),
+              "pxff_so4_tangent_noalloc")
+    return db
+
+
 def z_atom_major(nl, cfg, z):
     """Atom-major copy [N, nz, 2] of the tile-major Z."""
     za = _f64(nl.natoms * cfg.nz * 2, nl.pos.device)
