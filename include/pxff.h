@@ -40,7 +40,9 @@ extern "C" {
  *   PXFF_TUNE_ACCUM_LDSADD          per-centre U_tot sums: 0 = per-group LDS partials, 1 = one LDS array
  *                                   with LDS atomic adds (issued by one wavefront in a fixed order),
  *                                   2 (3) = two (three) groups per partial array, added in two
- *                                   (three) ordered passes */
+ *                                   (three) ordered passes, 4 (default) = each lane's sums over
+ *                                   all of a centre's images in registers, reduced across the
+ *                                   lane groups once per centre (2J <= 8; 2J = 10 uses 0) */
 #define PXFF_TUNE_Z_CHUNKS 0
 #define PXFF_TUNE_ACCUM_WAVES_PER_CU 1
 #define PXFF_TUNE_IMAGE_WAVES_PER_BLOCK 2
@@ -329,11 +331,15 @@ int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img
  * where the separations d_img_r are the neighbour list's own, so that image
  * e and its reverse d_img_rev[e] (pxff_nlist_rev) have bitwise the same |r|:
  * the radial integrals are evaluated once per image pair and written to
- * both rows (same values as pxff_so3_ctot). */
+ * both rows (same values as pxff_so3_ctot).  With d_work (int64 workspace
+ * of 3*512 + 1 + cap entries, cap = the image capacity of the arrays) the
+ * pairs are first ordered by |r| (a counting sort into 512 bins) so that
+ * the images a wavefront evaluates together take the same branches; the
+ * values do not depend on it.  d_work = NULL: the unsorted order. */
 int pxff_so3_ctot_pairs(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start,
                         const int64_t *d_img_j, const double *d_img_r, const int64_t *d_img_rev,
                         const int64_t *d_atom_z, const double *d_G, const double *d_rq, const double *d_lnorm,
-                        double *d_out, double *d_radial, void *stream);
+                        double *d_out, double *d_radial, int64_t *d_work, int64_t cap, void *stream);
 
 /* Power spectrum x [natoms*ncoef]; with d_dctot non-NULL the tangent of x. */
 int pxff_so3_power(const pxff_so3_params *p, int64_t natoms, const double *d_ctot, const double *d_dctot,

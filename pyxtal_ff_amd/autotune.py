@@ -3,8 +3,9 @@
 Knobs (``pxff_tune_set``, include/pxff.h):
   * Z-row chunks per 64-atom tile of the Z kernel (wavefronts per tile),
   * persistent-grid size of the per-centre kernels (wavefronts per CU),
-  * how the per-centre kernels sum over images (per-group LDS partials,
-    LDS atomic adds, or partials shared by two groups),
+  * how the per-centre kernels sum over images (register sums reduced once
+    per centre, per-group LDS partials, LDS atomic adds, or LDS partials
+    shared by two or three groups),
   * wavefronts per workgroup of the per-image adjoint kernel.
 Each knob is timed on the caller's own workload with HIP events recorded on
 the stream the kernels run on (torch's current stream), median of a few
@@ -21,7 +22,7 @@ from . import so4 as dso4
 
 Z_CHUNKS, ACCUM_WAVES_PER_CU, IMAGE_WAVES_PER_BLOCK, ACCUM_LDSADD = 0, 1, 2, 3
 CANDIDATES = {Z_CHUNKS: (4, 8, 16, 24, 32, 48, 64), ACCUM_WAVES_PER_CU: (4, 8, 16, 32, 64),
-              IMAGE_WAVES_PER_BLOCK: (1, 2, 4), ACCUM_LDSADD: (0, 1, 2, 3)}
+              IMAGE_WAVES_PER_BLOCK: (1, 2, 4), ACCUM_LDSADD: (4, 0, 1, 2, 3)}
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "pyxtal_ff_amd", "tune.json")
 
 

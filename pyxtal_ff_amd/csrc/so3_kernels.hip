@@ -512,6 +512,85 @@ __global__ __launch_bounds__(256) void so3_radial_all_kernel(So3Args a, int64_t 
     }
 }
 
+// Scheduling of the radial kernel by |r|: a counting sort of the images
+// that the pair store evaluates (rev(e) >= e) into SO3_RBINS bins of |r| over
+// (0, rc], so that the 64 lanes of a wavefront hold images of nearly equal
+// |r|.  At each quadrature node their Bessel arguments x = 2 alpha |r| r_q
+// then fall in the same branch of sph_bessel_i_all (series, Miller,
+// closed form), which otherwise diverge and cost the wavefront all three.
+// Only the order in which images are processed changes: every image's
+// integrals are the same computation as in so3_radial_all_kernel.
+// Workspace (int64): count [NB], start [NB + 1] (start[NB] = number of
+// sorted images), cursor [NB], order [image capacity].
+constexpr int SO3_RBINS = 512;
+__device__ inline int so3_rbin(const double *__restrict__ img_r, int64_t e, double rc) {
+    const double x = img_r[3 * e], y = img_r[3 * e + 1], z = img_r[3 * e + 2];
+    const double R = sqrt(x * x + y * y + z * z);
+    int b = (int)(R / rc * SO3_RBINS);
+    return b < 0 ? 0 : (b >= SO3_RBINS ? SO3_RBINS - 1 : b);
+}
+
+__global__ void so3_rbin_zero_kernel(int64_t *__restrict__ work) {
+    for (int k = threadIdx.x; k < SO3_RBINS; k += blockDim.x) work[k] = 0;
+}
+
+__global__ __launch_bounds__(256) void so3_rbin_count_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
+                                                             const double *__restrict__ img_r,
+                                                             const int64_t *__restrict__ img_rev, double rc,
+                                                             int64_t *__restrict__ work) {
+    const int64_t nimg = img_start[natoms];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nimg; e += (int64_t)gridDim.x * blockDim.x)
+        if (img_rev[e] >= e)
+            atomicAdd(reinterpret_cast<unsigned long long *>(work + so3_rbin(img_r, e, rc)), 1ull);
+}
+
+__global__ __launch_bounds__(SO3_RBINS) void so3_rbin_scan_kernel(int64_t *__restrict__ work) {
+    __shared__ int64_t sc[SO3_RBINS];
+    const int t = threadIdx.x;
+    sc[t] = work[t];
+    __syncthreads();
+    for (int o = 1; o < SO3_RBINS; o <<= 1) {
+        const int64_t v = t >= o ? sc[t - o] : 0;
+        __syncthreads();
+        sc[t] += v;
+        __syncthreads();
+    }
+    const int64_t excl = sc[t] - work[t];
+    work[SO3_RBINS + t] = excl;                  // start
+    work[2 * SO3_RBINS + 1 + t] = excl;          // cursor
+    if (t == SO3_RBINS - 1) work[2 * SO3_RBINS] = sc[t];  // start[NB]: sorted images
+}
+
+__global__ __launch_bounds__(256) void so3_rbin_fill_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
+                                                            const double *__restrict__ img_r,
+                                                            const int64_t *__restrict__ img_rev, double rc,
+                                                            int64_t *__restrict__ work) {
+    const int64_t nimg = img_start[natoms];
+    int64_t *cursor = work + 2 * SO3_RBINS + 1;
+    int64_t *order = work + 3 * SO3_RBINS + 1;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nimg; e += (int64_t)gridDim.x * blockDim.x)
+        if (img_rev[e] >= e) {
+            const int b = so3_rbin(img_r, e, rc);
+            const int64_t pos = (int64_t)atomicAdd(reinterpret_cast<unsigned long long *>(cursor + b), 1ull);
+            order[pos] = e;
+        }
+}
+
+// Radial integrals of the images in |r| order (see above): one image per
+// lane, flat over the sorted images; writes both rows of each pair.
+template <int NMAX, int LC>
+__global__ __launch_bounds__(256) void so3_radial_sorted_kernel(So3Args a, const int64_t *__restrict__ work,
+                                                                const double *__restrict__ img_r,
+                                                                const int64_t *__restrict__ img_rev,
+                                                                double *__restrict__ rad) {
+    const int64_t n = work[2 * SO3_RBINS];
+    const int64_t *order = work + 3 * SO3_RBINS + 1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t e = order[t];
+    so3_radial_image<NMAX, LC>(a, e, img_rev[e], img_r, rad);
+}
+
 // ------------------------------------------------------------ accumulate
 // ctot[i][n][l][m] = sum_e c_e  (MODE 0)  or its tangent
 // sum_e sum_k dr_e,k dc_e/dr_k (MODE 1).  One workgroup (64 threads) per
@@ -628,6 +707,321 @@ __global__ __launch_bounds__(64) void so3_accum_kernel(So3Args a, int64_t natoms
         __syncthreads();
         cd *oc = reinterpret_cast<cd *>(out) + (size_t)c * csz;
         for (int k = lane; k < csz; k += 64) oc[k] = acc[k];
+    }
+}
+
+// ctot[i][n][l][m] = sum_e c_e,nlm = sum_e w norm_l ex fc I_nl Y_lm (SO3.py:
+// 205-226) with the radial integrals I_nl read from the radial store.  One
+// wavefront per centre (persistent), one lane per (l, m), (lmax + 1)^2 <= 64:
+// lane (l, m) keeps its NMAX sums in registers over all of the centre's
+// images, so no reduction is needed.  The per-image scalars are computed
+// once per 64 images (one lane each) and staged in LDS, where every lane
+// reads the same address (broadcast).  Y_lm of the lane's (l, |m|): the
+// normalised associated Legendre recurrence (the same coefficient tables as
+// ylm_row, held in registers for the lane's fixed (l, m)) times
+// e^{i|m|phi} by repeated multiplication; Y_l^-m = (-1)^m conj(Y_l^m).
+// Entries with |m| > l are written as zero.
+constexpr int SO3_LM_NST = 8;  // staged per image: s, t, cos phi, sin phi, wf (w ex fc), 0: skip
+template <int NMAX>
+__global__ __launch_bounds__(64) void so3_ctot_lm_kernel(So3Args a, int64_t natoms, const int64_t *__restrict__ img_start,
+                                                          const int64_t *__restrict__ img_j,
+                                                          const double *__restrict__ img_r,
+                                                          const int64_t *__restrict__ atom_z,
+                                                          const double *__restrict__ rad, double *__restrict__ out) {
+    __shared__ double stg[SO3_LM_NST][64];
+    const int L = a.lmax + 1;
+    const int M2 = 2 * a.lmax + 1;
+    const int lane = threadIdx.x;
+    // the lane's (l, m): lane = l^2 + l + m
+    int l = 0;
+    while ((l + 1) * (l + 1) <= lane) l++;
+    const int m = lane - l * l - l;
+    const bool act = lane < L * L;
+    const int am = m < 0 ? -m : m;
+    // Legendre coefficients of the lane's (l, |m|), once (lmax <= 7)
+    constexpr int LK = 8;
+    double pmmf[LK], lega_[LK], legb_[LK];
+    double p1 = 0.0;
+    const double lnorm = act ? a.lnorm[l] : 0.0;
+#pragma unroll
+    for (int k = 0; k < LK; k++) {
+        pmmf[k] = (k >= 1 && k <= am) ? g_so3_pmm[k] : 1.0;
+        lega_[k] = (k >= am + 2 && k <= l) ? g_so3_lega[k * SO3_LEGD + am] : 0.0;
+        legb_[k] = (k >= am + 2 && k <= l) ? g_so3_legb[k * SO3_LEGD + am] : 0.0;
+    }
+    if (act && l > am) p1 = g_so3_p1[am];
+    const double sgn = (m < 0 && (am & 1)) ? -1.0 : 1.0;
+    const double inv4pi = 0.25 / M_PI;
+    for (int64_t c = blockIdx.x; c < natoms; c += gridDim.x) {
+        double accr[NMAX], acci[NMAX];
+#pragma unroll
+        for (int n = 0; n < NMAX; n++) {
+            accr[n] = 0.0;
+            acci[n] = 0.0;
+        }
+        const int64_t s0 = img_start[c], s1 = img_start[c + 1];
+        for (int64_t b64 = s0; b64 < s1; b64 += 64) {
+            const int nb = (int)((s1 - b64) < 64 ? (s1 - b64) : 64);
+            __syncthreads();  // the previous chunk's stage has been read
+            {
+                const int64_t e = b64 + lane;
+                const bool ok = lane < nb;
+                So3Img mi;
+                load_image(a, ok, e, c, img_j, img_r, atom_z, mi);
+                const double rho = sqrt(mi.x * mi.x + mi.y * mi.y);
+                stg[0][lane] = rho / mi.R;                    // sin theta
+                stg[1][lane] = mi.z / mi.R;                   // cos theta
+                stg[2][lane] = rho > 0.0 ? mi.x / rho : 1.0;  // cos phi
+                stg[3][lane] = rho > 0.0 ? mi.y / rho : 0.0;  // sin phi
+                stg[4][lane] = ok ? mi.w * mi.ex * mi.fc : 0.0;
+            }
+            __syncthreads();
+            if (act) {
+                for (int b = 0; b < nb; b++) {
+                    const double st = stg[0][b], ct = stg[1][b], cp = stg[2][b], sp = stg[3][b];
+                    const double wf = stg[4][b] * lnorm;
+                    // Pbar_l^|m|(cos theta)
+                    double pmm = sqrt(inv4pi);
+                    cd em = cmake(1.0, 0.0);
+#pragma unroll
+                    for (int k = 1; k < LK; k++)
+                        if (k <= am) {
+                            pmm = pmmf[k] * st * pmm;
+                            em = cmul(em, cmake(cp, sp));
+                        }
+                    double pl = pmm;
+                    if (l > am) {
+                        double pm2 = pmm, pm1 = p1 * ct * pmm;
+#pragma unroll
+                        for (int k = 2; k < LK; k++)
+                            if (k >= am + 2 && k <= l) {
+                                const double pk = lega_[k] * (ct * pm1 - legb_[k] * pm2);
+                                pm2 = pm1;
+                                pm1 = pk;
+                            }
+                        pl = pm1;
+                    }
+                    // Y_l^m w ex fc norm_l, then times I_nl
+                    const double yr = pl * em.re * wf * sgn, yi = (m < 0 ? -1.0 : 1.0) * pl * em.im * wf * sgn;
+                    const double *ri = rad + ((size_t)(b64 + b) * L + l) * 2 * NMAX;
+#pragma unroll
+                    for (int n = 0; n < NMAX; n++) {
+                        const double in = ri[n];
+                        accr[n] = fma(in, yr, accr[n]);
+                        acci[n] = fma(in, yi, acci[n]);
+                    }
+                }
+            }
+        }
+        cd *oc = reinterpret_cast<cd *>(out) + (size_t)c * NMAX * L * M2;
+        if (act) {
+#pragma unroll
+            for (int n = 0; n < NMAX; n++) oc[(n * L + l) * M2 + a.lmax + m] = cmake(accr[n], acci[n]);
+        }
+        // the entries with |m| > l are zero
+        for (int k = lane; k < NMAX * L * M2; k += 64) {
+            const int ll = (k / M2) % L, mm = k % M2 - a.lmax;
+            if (mm > ll || mm < -ll) oc[k] = cmake(0.0, 0.0);
+        }
+    }
+}
+
+// Adjoint dE/dr per image, one lane per (l, m) with l <= lmax + 1
+// ((lmax + 2)^2 <= 64): flat over images, a wavefront takes 64 consecutive
+// images (their scalars staged in LDS, one lane each) and runs them one at
+// a time.  Per image every lane evaluates its Y_lm (as so3_ctot_lm_kernel)
+// into a 64-entry LDS table, from which the lanes with l <= lmax read the
+// Y_{l+-1, m'} of the covariant gradient (SO3.py:649-674; its coefficients
+// depend on (l, m) only and are held in registers).  With
+// alpha = sum_n I_nl A_nlm and beta = sum_n dI_nl A_nlm (A of the image's
+// centre, reloaded when the centre changes),
+//   dE/dr_k = sum_lm  cu rhat_k Re(conj(alpha) Y) + cv Re(conj(alpha) dY/dr_k)
+//                   + cv rhat_k Re(conj(beta) Y),
+// cu = w norm_l ex (dfc - 2 alpha R fc), cv = w norm_l ex fc: the same sums
+// as so3_image_kernel<SO3_ADJOINT> in another order, reduced over the
+// lanes by a butterfly.
+constexpr int SO3_ADJ_NST = 10;  // staged: s, t, cos phi, sin phi, R, w ex fc, w ex (dfc - 2 a R fc), rhat
+template <int NMAX>
+__global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t nimg, const int64_t *__restrict__ img_atom,
+                                                            const int64_t *__restrict__ img_j,
+                                                            const double *__restrict__ img_r,
+                                                            const int64_t *__restrict__ atom_z,
+                                                            const double *__restrict__ A,
+                                                            const double *__restrict__ rad, double *__restrict__ dedr,
+                                                            const int64_t *__restrict__ d_count) {
+    __shared__ double stg[SO3_ADJ_NST][64];
+    __shared__ int64_t s_at[64];
+    __shared__ cd ysh[64];
+    if (d_count) {
+        const int64_t c = *d_count;
+        nimg = c < nimg ? c : nimg;
+    }
+    const int L = a.lmax + 1;
+    const int M2 = 2 * a.lmax + 1;
+    const int lane = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * 64;
+    if (base >= nimg) return;  // a launch over an image capacity
+    const int nb = (int)((nimg - base) < 64 ? (nimg - base) : 64);
+    int l = 0;
+    while ((l + 1) * (l + 1) <= lane) l++;
+    const int m = lane - l * l - l;
+    const bool ylane = lane < (L + 1) * (L + 1);  // rows 0..lmax+1
+    const bool act = lane < L * L;                // rows 0..lmax
+    const int am = m < 0 ? -m : m;
+    constexpr int LK = 8;  // rows l <= 7
+    double pmmf[LK], lega_[LK], legb_[LK];
+    double p1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < LK; k++) {
+        pmmf[k] = (ylane && k >= 1 && k <= am) ? g_so3_pmm[k] : 1.0;
+        lega_[k] = (ylane && k >= am + 2 && k <= l) ? g_so3_lega[k * SO3_LEGD + am] : 0.0;
+        legb_[k] = (ylane && k >= am + 2 && k <= l) ? g_so3_legb[k * SO3_LEGD + am] : 0.0;
+    }
+    if (ylane && l > am) p1 = g_so3_p1[am];
+    const double msg = (m < 0 && (am & 1)) ? -1.0 : 1.0, imsg = m < 0 ? -1.0 : 1.0;
+    // covariant gradient coefficients of the lane's (l, m) and the lanes of
+    // the Y values they multiply (-1: term absent)
+    const double dl = l, dm = m;
+    double gc[6] = {0, 0, 0, 0, 0, 0};
+    int gi[6] = {-1, -1, -1, -1, -1, -1};
+    auto lidx = [&](int ll, int mm) { return ll * ll + ll + mm; };
+    if (act && l > 0) {
+        gc[0] = -sqrt(((dl + 1) * (dl + 1) - dm * dm) / (2 * dl + 1) / (2 * dl + 3)) * dl;  // x0: Y(l+1, m)
+        gi[0] = lidx(l + 1, m);
+        if (am <= l - 1) {
+            gc[1] = sqrt((dl * dl - dm * dm) / (2 * dl - 1) / (2 * dl + 1)) * (dl + 1);      // x0: Y(l-1, m)
+            gi[1] = lidx(l - 1, m);
+        }
+        gc[2] = -sqrt((dl + dm + 1) * (dl + dm + 2) / 2 / (2 * dl + 1) / (2 * dl + 3)) * dl;  // xp: Y(l+1, m+1)
+        gi[2] = lidx(l + 1, m + 1);
+        if (abs(m + 1) <= l - 1) {
+            gc[3] = -sqrt((dl - dm - 1) * (dl - dm) / 2 / (2 * dl - 1) / (2 * dl + 1)) * (dl + 1);  // xp: Y(l-1, m+1)
+            gi[3] = lidx(l - 1, m + 1);
+        }
+        gc[4] = -sqrt((dl - dm + 1) * (dl - dm + 2) / 2 / (2 * dl + 1) / (2 * dl + 3)) * dl;  // xm: Y(l+1, m-1)
+        gi[4] = lidx(l + 1, m - 1);
+        if (abs(m - 1) <= l - 1) {
+            gc[5] = -sqrt((dl + dm - 1) * (dl + dm) / 2 / (2 * dl - 1) / (2 * dl + 1)) * (dl + 1);  // xm: Y(l-1, m-1)
+            gi[5] = lidx(l - 1, m - 1);
+        }
+    }
+    const double lnorm = act ? a.lnorm[l] : 0.0;
+    const double inv4pi = 0.25 / M_PI, r2 = 1.0 / sqrt(2.0);
+    {
+        const int64_t e = base + lane;
+        const bool ok = lane < nb;
+        So3Img mi;
+        load_image(a, ok, e, ok ? img_atom[e] : 0, img_j, img_r, atom_z, mi);
+        const double rho = sqrt(mi.x * mi.x + mi.y * mi.y);
+        stg[0][lane] = rho / mi.R;
+        stg[1][lane] = mi.z / mi.R;
+        stg[2][lane] = rho > 0.0 ? mi.x / rho : 1.0;
+        stg[3][lane] = rho > 0.0 ? mi.y / rho : 0.0;
+        stg[4][lane] = mi.R;
+        stg[5][lane] = ok ? mi.w * mi.ex * mi.fc : 0.0;
+        stg[6][lane] = ok ? mi.w * mi.ex * (mi.dfc - 2.0 * a.alpha * mi.R * mi.fc) : 0.0;
+        stg[7][lane] = mi.rh[0];
+        stg[8][lane] = mi.rh[1];
+        stg[9][lane] = mi.rh[2];
+        s_at[lane] = ok ? mi.i : -1;
+    }
+    __syncthreads();
+    int64_t cur = -1;
+    double Ar[NMAX], Ai[NMAX];
+#pragma unroll
+    for (int n = 0; n < NMAX; n++) {
+        Ar[n] = 0.0;
+        Ai[n] = 0.0;
+    }
+    for (int b = 0; b < nb; b++) {
+        const int64_t i = s_at[b];
+        if (i != cur) {  // uniform: the image's centre changed
+            cur = i;
+            if (act) {
+                const cd *ai = reinterpret_cast<const cd *>(A) + (size_t)i * NMAX * L * M2;
+#pragma unroll
+                for (int n = 0; n < NMAX; n++) {
+                    const cd v = ai[(n * L + l) * M2 + a.lmax + m];
+                    Ar[n] = v.re;
+                    Ai[n] = v.im;
+                }
+            }
+        }
+        const double st = stg[0][b], ct = stg[1][b], cp = stg[2][b], sp = stg[3][b];
+        // Y_lm of every lane (rows 0..lmax+1)
+        double pmm = sqrt(inv4pi);
+        cd em = cmake(1.0, 0.0);
+#pragma unroll
+        for (int k = 1; k < LK; k++)
+            if (k <= am) {
+                pmm = pmmf[k] * st * pmm;
+                em = cmul(em, cmake(cp, sp));
+            }
+        double pl = pmm;
+        if (l > am) {
+            double pm2 = pmm, pm1 = p1 * ct * pmm;
+#pragma unroll
+            for (int k = 2; k < LK; k++)
+                if (k >= am + 2 && k <= l) {
+                    const double pk = lega_[k] * (ct * pm1 - legb_[k] * pm2);
+                    pm2 = pm1;
+                    pm1 = pk;
+                }
+            pl = pm1;
+        }
+        const cd y = cmake(msg * pl * em.re, imsg * msg * pl * em.im);
+        ysh[lane] = y;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+        if (act) {
+            const double R = stg[4][b], iR = 1.0 / R;
+            const double cv = stg[5][b] * lnorm, cu = stg[6][b] * lnorm;
+            const double rh0 = stg[7][b], rh1 = stg[8][b], rh2 = stg[9][b];
+            cd yv[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) yv[q] = gi[q] >= 0 ? ysh[gi[q]] : cmake(0.0, 0.0);
+            const cd x0 = cmake((gc[0] * yv[0].re + gc[1] * yv[1].re) * iR, (gc[0] * yv[0].im + gc[1] * yv[1].im) * iR);
+            const cd xp = cmake((gc[2] * yv[2].re + gc[3] * yv[3].re) * iR, (gc[2] * yv[2].im + gc[3] * yv[3].im) * iR);
+            const cd xm = cmake((gc[4] * yv[4].re + gc[5] * yv[5].re) * iR, (gc[4] * yv[4].im + gc[5] * yv[5].im) * iR);
+            // grad Y: (xm - xp)/sqrt2, i (xm + xp)/sqrt2, x0
+            const cd g0 = cmake((xm.re - xp.re) * r2, (xm.im - xp.im) * r2);
+            const cd g1 = cmake(-(xm.im + xp.im) * r2, (xm.re + xp.re) * r2);
+            const double *ri = rad + ((size_t)(base + b) * L + l) * 2 * NMAX;
+            double ar = 0.0, ai2 = 0.0, br = 0.0, bi = 0.0;
+#pragma unroll
+            for (int n = 0; n < NMAX; n++) {
+                const double In = ri[n], dIn = ri[NMAX + n];
+                ar = fma(In, Ar[n], ar);
+                ai2 = fma(In, Ai[n], ai2);
+                br = fma(dIn, Ar[n], br);
+                bi = fma(dIn, Ai[n], bi);
+            }
+            // Re(conj(alpha) z) = ar z.re + ai z.im
+            const double ray = ar * y.re + ai2 * y.im;
+            const double rby = br * y.re + bi * y.im;
+            const double s = cu * ray + cv * rby;
+            c0 = fma(s, rh0, cv * (ar * g0.re + ai2 * g0.im));
+            c1 = fma(s, rh1, cv * (ar * g1.re + ai2 * g1.im));
+            c2 = fma(s, rh2, cv * (ar * x0.re + ai2 * x0.im));
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            c0 += __shfl_xor(c0, o);
+            c1 += __shfl_xor(c1, o);
+            c2 += __shfl_xor(c2, o);
+        }
+        if (lane == 0) {
+            const int64_t e = base + b;
+            dedr[3 * e] = c0;
+            dedr[3 * e + 1] = c1;
+            dedr[3 * e + 2] = c2;
+        }
+        // the next image rewrites the Y table
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -933,7 +1327,8 @@ static int launch_set_lds(K kern, size_t lds) {
 static int so3_ctot_impl(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start,
                          const int64_t *d_img_j, const double *d_img_r, const int64_t *d_img_rev,
                          const int64_t *d_atom_z, const double *d_G, const double *d_rq, const double *d_lnorm,
-                         const double *d_dr, double *d_out, double *d_radial, void *stream) {
+                         const double *d_dr, double *d_out, double *d_radial, void *stream, int64_t *d_work = nullptr,
+                         int64_t cap = 0) {
     int rc = so3_check(p);
     if (rc) return rc;
     if (natoms <= 0) return 0;
@@ -957,22 +1352,41 @@ static int so3_ctot_impl(const pxff_so3_params *p, int64_t natoms, const int64_t
                 // with pairs, one resident round of wavefronts (2 per SIMD at
                 // 256 registers), each queueing several chunks of 64 images so
                 // that its batches of 64 qualifying images are full
-                const int grid = d_img_rev ? cus() * 2 : cus() * 16;
-                SO3_DISPATCH_NMAX(p->nmax, if constexpr (NM <= 6) {
-                    so3_radial_all_kernel<NM, 7><<<grid, 256, 0, s>>>(a, natoms, d_img_start, d_img_r, d_img_rev,
-                                                                     d_radial);
-                });
+                if (d_img_rev && d_work && cap > 0) {
+                    // pair images sorted into |r| bins, then one image per lane
+                    const int gb = std::max(1, std::min(div_up(cap, 256), cus() * 8));
+                    so3_rbin_zero_kernel<<<1, 256, 0, s>>>(d_work);
+                    so3_rbin_count_kernel<<<gb, 256, 0, s>>>(natoms, d_img_start, d_img_r, d_img_rev, p->rcut, d_work);
+                    so3_rbin_scan_kernel<<<1, SO3_RBINS, 0, s>>>(d_work);
+                    so3_rbin_fill_kernel<<<gb, 256, 0, s>>>(natoms, d_img_start, d_img_r, d_img_rev, p->rcut, d_work);
+                    SO3_DISPATCH_NMAX(p->nmax, if constexpr (NM <= 6) {
+                        so3_radial_sorted_kernel<NM, 7><<<div_up(cap, 256), 256, 0, s>>>(a, d_work, d_img_r,
+                                                                                        d_img_rev, d_radial);
+                    });
+                } else {
+                    const int grid = d_img_rev ? cus() * 2 : cus() * 16;
+                    SO3_DISPATCH_NMAX(p->nmax, if constexpr (NM <= 6) {
+                        so3_radial_all_kernel<NM, 7><<<grid, 256, 0, s>>>(a, natoms, d_img_start, d_img_r,
+                                                                         d_img_rev, d_radial);
+                    });
+                }
             } else {
                 SO3_DISPATCH_NMAX(p->nmax, (so3_radial_kernel<NM><<<cus() * 16, 256, 0, s>>>(
                                                a, natoms, d_img_start, d_img_r, d_radial)));
             }
             PXFF_LAUNCH_CHECK();
         }
-        SO3_DISPATCH_NMAX(p->nmax, {
-            auto k = so3_accum_kernel<NM, SO3_CTOT>;
-            if ((rc = launch_set_lds(k, lds))) return rc;
-            k<<<grid, 64, lds, s>>>(a, natoms, d_img_start, d_img_j, d_img_r, d_atom_z, nullptr, d_out, d_radial);
-        });
+        if (d_radial && p->lmax <= 7) {
+            // one lane per (l, m), the n-sums in registers
+            SO3_DISPATCH_NMAX(p->nmax, (so3_ctot_lm_kernel<NM><<<grid, 64, 0, s>>>(
+                                           a, natoms, d_img_start, d_img_j, d_img_r, d_atom_z, d_radial, d_out)));
+        } else {
+            SO3_DISPATCH_NMAX(p->nmax, {
+                auto k = so3_accum_kernel<NM, SO3_CTOT>;
+                if ((rc = launch_set_lds(k, lds))) return rc;
+                k<<<grid, 64, lds, s>>>(a, natoms, d_img_start, d_img_j, d_img_r, d_atom_z, nullptr, d_out, d_radial);
+            });
+        }
     }
     PXFF_LAUNCH_CHECK();
     return 0;
@@ -989,10 +1403,12 @@ extern "C" int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int
 extern "C" int pxff_so3_ctot_pairs(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start,
                                    const int64_t *d_img_j, const double *d_img_r, const int64_t *d_img_rev,
                                    const int64_t *d_atom_z, const double *d_G, const double *d_rq,
-                                   const double *d_lnorm, double *d_out, double *d_radial, void *stream) {
+                                   const double *d_lnorm, double *d_out, double *d_radial, int64_t *d_work,
+                                   int64_t cap, void *stream) {
     PXFF_REQUIRE(d_img_rev != nullptr && d_radial != nullptr, "SO3: pairs need the reverse images and the radial store");
+    PXFF_REQUIRE(cap >= 0 && (d_work == nullptr || cap > 0), "SO3: the |r| ordering needs the image capacity");
     return so3_ctot_impl(p, natoms, d_img_start, d_img_j, d_img_r, d_img_rev, d_atom_z, d_G, d_rq, d_lnorm, nullptr,
-                         d_out, d_radial, stream);
+                         d_out, d_radial, stream, d_work, cap);
 }
 
 extern "C" int pxff_so3_power(const pxff_so3_params *p, int64_t natoms, const double *d_ctot, const double *d_dctot,
@@ -1032,7 +1448,12 @@ static int so3_image_launch(const pxff_so3_params *p, int mode, int64_t nimg, co
     PXFF_REQUIRE(lds <= 160 * 1024, "SO3: LDS need exceeds 160 KB");
     const int NG = 64 / (p->lmax + 1);
     const int grid = div_up(nimg, NG);
-    if (mode == SO3_ADJOINT) {
+    if (mode == SO3_ADJOINT && d_radial && p->lmax <= 6) {
+        // one lane per (l, m), 64 images per wavefront
+        SO3_DISPATCH_NMAX(p->nmax, (so3_deidrj_lm_kernel<NM><<<div_up(nimg, 64), 64, 0, s>>>(
+                                       a, nimg, d_img_atom, d_img_j, d_img_r, d_atom_z, d_cin, d_radial, d_out,
+                                       d_count)));
+    } else if (mode == SO3_ADJOINT) {
         SO3_DISPATCH_NMAX(p->nmax, {
             auto k = so3_image_kernel<NM, SO3_ADJOINT>;
             if ((rc = launch_set_lds(k, lds))) return rc;

@@ -585,6 +585,187 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void so
     }
 }
 
+// U_tot (and its tangent) with the partial sums in registers.  One
+// wavefront per centre, G = 2J/2+1 lanes per image as in so4_accum_kernel,
+// but lane (g, mb) adds row mb of every level of all its images into
+// registers (one slot per (l, m), 45 complex at 2J=8), so a centre's images
+// cost no LDS traffic; once per centre the NG groups' sums are reduced in
+// LDS in a fixed order (groups g < NP store, groups g + NP add, then the NP
+// arrays are summed in order: deterministic).  2J <= 8 (at 2J=10 the
+// register partials would exceed the register file; so4_accum_kernel).
+template <int TWOJ, bool TANGENT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) void so4_utot_kernel(
+    int64_t natoms, const int64_t *__restrict__ img_start, const int64_t *__restrict__ img_j,
+    const double *__restrict__ img_r, const double *__restrict__ atom_w, const double *__restrict__ site_w,
+    const double *__restrict__ dr, const int64_t *__restrict__ work_order, const double *rootpq_g,
+    const double *lnorm_g, double rcut, double rfac0, int cutoff_id, double fcut0, double *__restrict__ out) {
+    constexpr int G = TWOJ / 2 + 1;
+    constexpr int NG = 64 / G;
+    constexpr int NH = nhalf_of(TWOJ);
+    constexpr int NSLOT = (TWOJ + 1) * (TWOJ + 2) / 2;  // (l, m <= l) of every level
+    constexpr int NP = (NG + 1) / 2;                    // LDS partial arrays
+    __shared__ cd part[NP][NH];
+    __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
+    __shared__ double lnorm[TWOJ + 1];
+    // staged per image: ca, cb, fc w (0: no image); with TANGENT ca, cb,
+    // w fc p(eta), w fc q(eta), w dfc (rhat . dr), valid
+    __shared__ double stg[TANGENT ? 9 : 5][64];
+    const int lane = threadIdx.x;
+    const int g = lane / G, mb = lane % G;
+    for (int k = lane; k < (TWOJ + 1) * (TWOJ + 1); k += 64) rpq[k] = rootpq_g[k];
+    for (int k = lane; k <= TWOJ; k += 64) lnorm[k] = lnorm_g[k];
+    if (blockIdx.x == 0 && (natoms & 63)) {
+        // padding lanes of the last tile (read by the Z kernel) are zero
+        cd *last = reinterpret_cast<cd *>(out) + (size_t)(natoms >> 6) * NH * 64;
+        for (int k = lane; k < NH * 64; k += 64)
+            if ((k & 63) >= (natoms & 63)) last[k] = cmake(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int64_t c = blockIdx.x; c < natoms; c += gridDim.x) {
+        const int64_t i = work_order ? work_order[c] : c;
+        cd acc[NSLOT];
+#pragma unroll
+        for (int q = 0; q < NSLOT; q++) acc[q] = cmake(0.0, 0.0);
+        const int64_t s = img_start[i], e = img_start[i + 1];
+        for (int64_t b64 = s; b64 < e; b64 += 64) {
+            const int64_t e64 = b64 + 64 < e ? b64 + 64 : e;
+            {
+                const int64_t img = b64 + lane;
+                bool ok = img < e64;
+                double x = 0, y = 0, z = 1.0, r = 1.0, w = 0.0;
+                if (ok) {
+                    x = img_r[3 * img];
+                    y = img_r[3 * img + 1];
+                    z = img_r[3 * img + 2];
+                    r = sqrt(x * x + y * y + z * z);
+                    w = atom_w[img_j[img]];
+                    if (r < 1e-8) ok = false;  // SNAP.py:1487 (padding rows)
+                }
+                if (!ok) {
+                    x = 0;
+                    y = 0;
+                    z = 1.0;
+                    r = 1.0;
+                }
+                double fc, dfc;
+                cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
+                CK ck;
+                cayley_klein(x, y, z, r, rfac0 * M_PI * r / rcut, TANGENT, ck);
+                stg[0][lane] = ck.ca.re;
+                stg[1][lane] = ck.ca.im;
+                stg[2][lane] = ck.cb.re;
+                stg[3][lane] = ck.cb.im;
+                if constexpr (!TANGENT) {
+                    stg[4][lane] = ok ? fc * w : 0.0;  // a zero weight adds nothing either way
+                } else {
+                    double p[3];
+                    cd q[3];
+                    su2_generators(ck, p, q);
+                    double d0 = 0, d1 = 0, d2 = 0;
+                    if (ok) {
+                        d0 = dr[3 * img];
+                        d1 = dr[3 * img + 1];
+                        d2 = dr[3 * img + 2];
+                    }
+                    const double wf = ok ? w * fc : 0.0;
+                    stg[4][lane] = wf * (p[0] * d0 + p[1] * d1 + p[2] * d2);
+                    stg[5][lane] = wf * (q[0].re * d0 + q[1].re * d1 + q[2].re * d2);
+                    stg[6][lane] = wf * (q[0].im * d0 + q[1].im * d1 + q[2].im * d2);
+                    // D . dr = fc (dU . dr) + dfc (rhat . dr) U
+                    stg[7][lane] = ok ? w * dfc * (x * d0 + y * d1 + z * d2) / r : 0.0;
+                    stg[8][lane] = ok ? 1.0 : 0.0;
+                }
+            }
+            __syncthreads();
+            for (int64_t base = b64; base < e64; base += NG) {
+                const int64_t img = base + g;
+                const int si = (g < NG && img < e64) ? (int)(img - b64) : 0;
+                const bool valid = (g < NG) && (img < e64) && stg[TANGENT ? 8 : 4][si] != 0.0;
+                CK ck;
+                ck.ca = cmake(stg[0][si], stg[1][si]);
+                ck.cb = cmake(stg[2][si], stg[3][si]);
+                if constexpr (!TANGENT) {
+                    const double sc = stg[4][si];
+                    sweep_rows<TWOJ, false>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
+                        if (!(valid && half)) return;
+                        const int q0 = l * (l + 1) / 2;
+#pragma unroll
+                        for (int m = 0; m <= TWOJ; m++) {
+                            if (m > l) continue;
+                            acc[q0 + m].re = fma(sc, u[m].re, acc[q0 + m].re);
+                            acc[q0 + m].im = fma(sc, u[m].im, acc[q0 + m].im);
+                        }
+                    });
+                } else {
+                    // dU . dr_e = U M(eta_e): (U M)_m = i p (l-2m) U_m + c_m-1 q U_m-1 - c_m conj(q) U_m+1
+                    const double P = stg[4][si], Qr = stg[5][si], Qi = stg[6][si], Gu = stg[7][si];
+                    sweep_rows<TWOJ, false>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
+                        if (!(valid && half)) return;
+                        const int q0 = l * (l + 1) / 2;
+#pragma unroll
+                        for (int m = 0; m <= TWOJ; m++) {
+                            if (m > l) continue;
+                            const double pm = P * (double)(l - 2 * m);
+                            double re = fma(-pm, u[m].im, Gu * u[m].re);
+                            double im = fma(pm, u[m].re, Gu * u[m].im);
+                            if (m >= 1) {
+                                const double cc = sqrt((double)(m * (l - m + 1)));  // c_(m-1)
+                                re = fma(cc, Qr * u[m - 1].re - Qi * u[m - 1].im, re);
+                                im = fma(cc, Qr * u[m - 1].im + Qi * u[m - 1].re, im);
+                            }
+                            if (m < l) {
+                                const double cc = sqrt((double)((m + 1) * (l - m)));  // c_m
+                                re = fma(-cc, Qr * u[m + 1].re + Qi * u[m + 1].im, re);
+                                im = fma(-cc, Qr * u[m + 1].im - Qi * u[m + 1].re, im);
+                            }
+                            acc[q0 + m].re += re;
+                            acc[q0 + m].im += im;
+                        }
+                    });
+                }
+            }
+            __syncthreads();  // the next 64 images reuse the stage
+        }
+        // reduce the NG groups' register sums: g < NP store, g >= NP add
+#pragma unroll
+        for (int pass = 0; pass < 2; pass++) {
+            const bool mine = g < NG && (pass == 0 ? g < NP : g >= NP);
+            const int pidx = g < NG ? (pass == 0 ? g : g - NP) : 0;
+            if (mine) {
+#pragma unroll
+                for (int l = 0; l <= TWOJ; l++) {
+                    if (2 * mb > l) continue;
+                    const int h0 = hoff_of(l) + mb * (l + 1), q0 = l * (l + 1) / 2;
+#pragma unroll
+                    for (int m = 0; m <= l; m++) {
+                        if (pass == 0)
+                            part[pidx][h0 + m] = acc[q0 + m];
+                        else
+                            part[pidx][h0 + m] = cadd(part[pidx][h0 + m], acc[q0 + m]);
+                    }
+                }
+            }
+            __syncthreads();  // (odd NG: array NP - 1 has no second group)
+        }
+        for (int h = lane; h < NH; h += 64) {
+            cd a = cmake(0.0, 0.0);
+            for (int q = 0; q < NP; q++) a = cadd(a, part[q][h]);
+            int l = 0, hm = 0, hma = 0;
+#pragma unroll
+            for (int L = 0; L <= TWOJ; L++)
+                if (h >= hoff_of(L) && h < hoff_of(L + 1)) {
+                    l = L;
+                    hm = (h - hoff_of(L)) / (L + 1);
+                    hma = (h - hoff_of(L)) % (L + 1);
+                }
+            if (!TANGENT && hm == hma) a.re += fcut0 * site_w[i];  // addself_uarraytot
+            a = cscale(a, lnorm[l]);                                 // ulisttot *= u_norm
+            reinterpret_cast<cd *>(out)[((size_t)(i >> 6) * NH + h) * 64 + (i & 63)] = a;
+        }
+        __syncthreads();  // the next centre rewrites part
+    }
+}
+
 // B (compute_bi, SNAP.py:1101-1141) from the tile-major U_tot and Z: one
 // atom per lane (64-atom tiles, so the U_tot loads are coalesced), the
 // components b spread over the wavefronts of a tile (grid.y); loop bounds
@@ -702,11 +883,12 @@ __global__ __launch_bounds__(256) void so4_yi_kernel(int64_t natoms, int ntasks,
 
 // -------------------------------------------------------- adjoint dE/dr
 constexpr int DEIDRJ_NST = 16;  // staged doubles per image of the dE/dr kernel
-// Flat over images: a wavefront serves NB = NR x NG images (any centres).
+// Flat over images: a wavefront serves NB = nr x NG images (any centres).
 // It first stages, one lane per image, the Cayley-Klein parameters and the
 // per-image factors of the generator contraction in LDS (the G lanes of an
-// image would otherwise each compute them), then sweeps them in NR rounds
-// of NG images:
+// image would otherwise each compute them), then sweeps them in nr rounds
+// of NG images (nr is chosen by the launch so that the grid holds enough
+// wavefronts to hide latency):
 //   dE/dr_k = w fc (p_k A1 + Re q_k A2 + Im q_k A3) + w dfc rhat_k A_u.
 template <int TWOJ>
 __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int64_t *__restrict__ img_atom,
@@ -716,11 +898,10 @@ __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int
                                                            const double *__restrict__ y, const double *rootpq_g,
                                                            double rcut, double rfac0, int cutoff_id,
                                                            double *__restrict__ dedr,
-                                                           const int64_t *__restrict__ d_count) {
+                                                           const int64_t *__restrict__ d_count, int nr) {
     constexpr int G = TWOJ / 2 + 1;
     constexpr int NG = 64 / G;
-    constexpr int NR = 64 / NG;
-    constexpr int NB = NG * NR;
+    const int NB = NG * nr;  // images per wavefront (nr rounds, nr <= 64 / NG)
     constexpr int NST = DEIDRJ_NST;
     constexpr int NH = nhalf_of(TWOJ);
     __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
@@ -793,7 +974,7 @@ __global__ __launch_bounds__(256) void so4_deidrj_kernel(int64_t nimg, const int
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int rd = 0; rd < NR; rd++) {
+    for (int rd = 0; rd < nr; rd++) {
         if (base + rd * NG >= nimg) break;  // uniform
         const int slot = rd * NG + (g < NG ? g : 0);
         const int64_t i = s_at[slot];
@@ -1184,9 +1365,9 @@ using namespace pxff;
 
 namespace pxff {
 // Process-wide launch parameters set by the autotuner (pxff_tune_set).
-static int g_tune[PXFF_TUNE_COUNT] = {16, 16, 4, 0};
+static int g_tune[PXFF_TUNE_COUNT] = {16, 16, 4, 4};
 static const int g_tune_lo[PXFF_TUNE_COUNT] = {1, 1, 1, 0};
-static const int g_tune_hi[PXFF_TUNE_COUNT] = {256, 256, 4, 3};
+static const int g_tune_hi[PXFF_TUNE_COUNT] = {256, 256, 4, 4};
 int tune_value(int knob) { return g_tune[knob]; }
 
 int so4_check_params(const pxff_so4_params *p) {
@@ -1242,11 +1423,20 @@ extern "C" int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64
                                     natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_site_w, nullptr,       \
                                     d_work_order, T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, fcut0, \
                                     d_utot)))
-    switch (tune_value(PXFF_TUNE_ACCUM_LDSADD)) {
-    case 1: PXFF_ACCUM_UI(1); break;
-    case 2: PXFF_ACCUM_UI(2); break;
-    case 3: PXFF_ACCUM_UI(3); break;
-    default: PXFF_ACCUM_UI(0); break;
+    const int mode = tune_value(PXFF_TUNE_ACCUM_LDSADD);
+    if (mode == 4 && p->twol <= 8) {
+        PXFF_DISPATCH_TWOJ(p->twol, if constexpr (TJ <= 8) {
+            so4_utot_kernel<TJ, false><<<grid, 64, 0, s>>>(natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_site_w,
+                                                          nullptr, d_work_order, T->rootpq, T->lnorm, p->rcut,
+                                                          p->rfac0, p->cutoff_id, fcut0, d_utot);
+        });
+    } else {
+        switch (mode) {
+        case 1: PXFF_ACCUM_UI(1); break;
+        case 2: PXFF_ACCUM_UI(2); break;
+        case 3: PXFF_ACCUM_UI(3); break;
+        default: PXFF_ACCUM_UI(0); break;
+        }
     }
 #undef PXFF_ACCUM_UI
     PXFF_LAUNCH_CHECK();
@@ -1300,13 +1490,16 @@ static int so4_deidrj_launch(const pxff_so4_params *p, int64_t nimg, const int64
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
     const int ng = 64 / (p->twol / 2 + 1);
-    const int nb = ng * (64 / ng);  // images per wavefront (staged batch)
+    // rounds per wavefront: at most 64 / ng (one staged lane per image), and
+    // few enough that the grid has about 16 wavefronts per CU
+    const int nr = (int)std::max<int64_t>(1, std::min<int64_t>(64 / ng, nimg / ((int64_t)num_cus() * 16 * ng)));
+    const int nb = ng * nr;  // images per wavefront (staged batch)
     const int wpb = g_tune[PXFF_TUNE_IMAGE_WAVES_PER_BLOCK];
     const int grid = div_up(div_up(nimg, nb), wpb);
     const size_t lds = sizeof(double) * (size_t)wpb * (DEIDRJ_NST + 5) * 64;  // per-wavefront stage
     PXFF_DISPATCH_TWOJ(p->twol, (so4_deidrj_kernel<TJ><<<grid, 64 * wpb, lds, s>>>(nimg, d_img_atom, d_img_j, d_img_r, d_atom_w,
                                                                            d_y, T->rootpq, p->rcut, p->rfac0,
-                                                                           p->cutoff_id, d_dedr, d_count)));
+                                                                           p->cutoff_id, d_dedr, d_count, nr)));
     PXFF_LAUNCH_CHECK();
     return 0;
 }
@@ -1369,11 +1562,20 @@ static int so4_tangent_launch(const pxff_so4_params *p, const So4Dev *T, int64_t
     PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, true, MODE><<<grid, 64, 0, s>>>(         \
                                     natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr, d_dr, nullptr, \
                                     T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, 0.0, d_dutot)))
-    switch (tune_value(PXFF_TUNE_ACCUM_LDSADD)) {
-    case 1: PXFF_ACCUM_TAN(1); break;
-    case 2: PXFF_ACCUM_TAN(2); break;
-    case 3: PXFF_ACCUM_TAN(3); break;
-    default: PXFF_ACCUM_TAN(0); break;
+    const int mode = tune_value(PXFF_TUNE_ACCUM_LDSADD);
+    if (mode == 4 && p->twol <= 8) {
+        PXFF_DISPATCH_TWOJ(p->twol, if constexpr (TJ <= 8) {
+            so4_utot_kernel<TJ, true><<<grid, 64, 0, s>>>(natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr,
+                                                         d_dr, nullptr, T->rootpq, T->lnorm, p->rcut, p->rfac0,
+                                                         p->cutoff_id, 0.0, d_dutot);
+        });
+    } else {
+        switch (mode) {
+        case 1: PXFF_ACCUM_TAN(1); break;
+        case 2: PXFF_ACCUM_TAN(2); break;
+        case 3: PXFF_ACCUM_TAN(3); break;
+        default: PXFF_ACCUM_TAN(0); break;
+        }
     }
 #undef PXFF_ACCUM_TAN
     PXFF_LAUNCH_CHECK();

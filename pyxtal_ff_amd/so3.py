@@ -16,6 +16,7 @@ import torch
 from . import _native as nat
 
 CUTOFF_IDS = {"cosine": 1, "tanh": 2, "poly1": 3, "poly2": 4, "poly3": 5, "poly4": 6, "exp": 7, "unity": 8}
+RBINS = 512  # |r| bins of the radial kernel's scheduling (pxff_so3_ctot_pairs)
 
 
 def W(nmax):
@@ -91,18 +92,25 @@ def ctot(nl, cfg, atom_z, img_r=None, dr=None, radial=False):
     out = _f64(nl.natoms * cfg.csz * 2, nl.pos.device)
     r = nl.img_r if img_r is None else img_r.contiguous()
     d = None if dr is None else dr.contiguous()
+    # the plain expansion always goes through the radial store (the
+    # per-(l, m) accumulation reads it); it is returned only when asked for
+    keep = radial
+    radial = radial or d is None
     rad = _f64(nl.img_capacity * (cfg.lmax + 1) * 2 * cfg.nmax, nl.pos.device) if radial else None
     if radial and d is None and r.data_ptr() == nl.img_r.data_ptr():
         # the list's own separations: an image and its reverse share |r|,
         # so the radial integrals are evaluated once per pair
+        # workspace of the |r| ordering of the pairs (scheduling only)
+        cap = nl.img_capacity
+        work = torch.empty(3 * RBINS + 1 + cap, dtype=torch.int64, device=nl.pos.device)
         nat.check(nat.lib().pxff_so3_ctot_pairs(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r),
                                                 p(nl.img_rev), p(atom_z), p(cfg.G), p(cfg.rq), p(cfg.lnorm),
-                                                p(out), p(rad), nat.stream()), "pxff_so3_ctot_pairs")
-        return out, rad
+                                                p(out), p(rad), p(work), cap, nat.stream()), "pxff_so3_ctot_pairs")
+        return (out, rad) if keep else out
     nat.check(nat.lib().pxff_so3_ctot(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r), p(atom_z),
                                       p(cfg.G), p(cfg.rq), p(cfg.lnorm), p(d), p(out), p(rad), nat.stream()),
               "pxff_so3_ctot")
-    return (out, rad) if radial else out
+    return (out, rad) if keep else out
 
 
 def power(nl, cfg, ct, dct=None):

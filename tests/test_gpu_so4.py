@@ -438,28 +438,34 @@ def test_diamond4096_full_parity(oracle_mod):
 
 
 def test_accumulation_modes_agree_and_reproduce():
-    """The U_tot summation modes (per-group LDS partials, LDS atomic adds
-    issued by one wavefront, partials shared by two or three groups) agree;
-    each mode is bitwise reproducible run to run (forces too)."""
+    """The U_tot summation modes (register sums reduced once per centre,
+    per-group LDS partials, LDS atomic adds issued by one wavefront,
+    partials shared by two or three groups) agree, for U_tot and for its
+    tangent; each mode is bitwise reproducible run to run (forces too)."""
     from pyxtal_ff_amd import autotune
     s = workloads.diamond_si(reps=3, sigma=0.08, seed=4)
     nl, cfg, aw, sw, model, species, state = _engine(s, weights={"Si": 1.0})
+    g = torch.Generator(device="cpu").manual_seed(3)
+    dr = torch.randn((nl.nimg, 3), generator=g, dtype=torch.float64).cuda()
     old = autotune.get_knob(autotune.ACCUM_LDSADD)
     try:
         res = {}
-        for mode in (0, 1, 2, 3):
+        for mode in (0, 1, 2, 3, 4):
             autotune.set_knob(autotune.ACCUM_LDSADD, mode)
             runs = []
             for _ in range(2):
-                b, _, _ = dso4.forward(nl, cfg, aw, sw)
+                b, z, _ = dso4.forward(nl, cfg, aw, sw)
                 f = potential.energy_forces_adjoint(nl, state, model, species)["forces"]
-                runs.append((b.cpu().numpy(), f.cpu().numpy()))
+                t = dso4.tangent(nl, cfg, aw, z, dr)
+                runs.append((b.cpu().numpy(), f.cpu().numpy(), t.cpu().numpy()))
             _print("mode", mode, "rerun max |dB|", np.abs(runs[0][0] - runs[1][0]).max())
-            assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1]), mode
+            for k in range(3):
+                assert np.array_equal(runs[0][k], runs[1][k]), (mode, k)
             res[mode] = runs[0]
-        for mode in (1, 2, 3):
+        for mode in (1, 2, 3, 4):
             assert rel_err(res[mode][0], res[0][0]) < 1e-13
             assert rel_err(res[mode][1], res[0][1]) < 1e-12
+            assert rel_err(res[mode][2], res[0][2]) < 1e-12
     finally:
         autotune.set_knob(autotune.ACCUM_LDSADD, old)
 
