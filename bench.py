@@ -118,7 +118,7 @@ PLAIN_KERNELS = ("dbdr", "rows", "dxdr_contract")
 KERNEL_SYMBOL = {"ui": "so4_accum_kernel", "z": "so4_z_tile_kernel", "bi": "so4_b_kernel",
                  "zb": "so4_z_tile_kernel+so4_brow_sum_kernel",
                  "network": "mlp_energy_grad_kernel", "yi": "so4_yi_kernel",
-                 "deidrj": "so4_deidrj_kernel", "gather": "so4_gather_kernel", "dbdr": "so4_dbdr_kernel",
+                 "deidrj": "so4_deidrj_kernel", "gather": "so4_gather_kernel", "dbdr": "so4_dbdr_centre_kernel",
                  "rows": "so4_rows_kernel", "dxdr_contract": "dxdr_contract_kernel"}
 
 

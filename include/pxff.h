@@ -249,6 +249,15 @@ int pxff_so4_z_atom_major(const pxff_so4_params *p, int64_t natoms, const double
 int pxff_so4_dbdr(const pxff_so4_params *p, int64_t nimg, const int64_t *d_img_atom, const int64_t *d_img_j,
                   const double *d_img_r, const double *d_atom_w, const double *d_z, double *d_dbdr, void *stream);
 
+/* Plain dB/dr per image [nimg*nb*3] from the tile-major Z of pxff_so4_zb
+ * (no atom-major copy): the same values as pxff_so4_dbdr, one workgroup per
+ * centre over its images img_start[c] .. img_start[c+1]-1 (the image count
+ * is read on the device).  Replaces the dblist part of compute_dbidrj /
+ * get_bispectrum_components (SNAP.py:1143-1257, 1428-1548). */
+int pxff_so4_dbdr_centres(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start,
+                          const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w, const double *d_z,
+                          double *d_dbdr, void *stream);
+
 /* Plain path, step 2: reference layout rows.  dxdr [nrows*nb*3] (rows (i,j)
  * = sum over images, row (i,i) = -sum of the others), and, if d_rdxdr is not
  * NULL, per-row -bstress/V [nrows*nb*9] with bstress[nsite] += Ri (x) dB,
@@ -332,10 +341,11 @@ int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img
  * e and its reverse d_img_rev[e] (pxff_nlist_rev) have bitwise the same |r|:
  * the radial integrals are evaluated once per image pair and written to
  * both rows (same values as pxff_so3_ctot).  With d_work (int64 workspace
- * of 3*512 + 1 + cap entries, cap = the image capacity of the arrays) the
- * pairs are first ordered by |r| (a counting sort into 512 bins) so that
- * the images a wavefront evaluates together take the same branches; the
- * values do not depend on it.  d_work = NULL: the unsorted order. */
+ * of 256*256 + 1 + cap entries, cap = the image capacity of the arrays) the
+ * pairs are first ordered by |r| (a counting sort into 256 bins, over 256
+ * chunks of the images) so that the images a wavefront evaluates together
+ * take the same branches; the values do not depend on it.  d_work = NULL:
+ * the unsorted order. */
 int pxff_so3_ctot_pairs(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start,
                         const int64_t *d_img_j, const double *d_img_r, const int64_t *d_img_rev,
                         const int64_t *d_atom_z, const double *d_G, const double *d_rq, const double *d_lnorm,

@@ -5,7 +5,7 @@
 // activation each, descriptors scaled by the element's drange,
 // neuralnetwork.py:938-1029) evaluated forward, then backward to dE/dx.
 // Three kernels, by what fits in LDS: 16-atom tiles on the FP64 matrix
-// cores (mlp_mfma_kernel, scratch and one element's weights within 64 KB);
+// cores (mlp_mfma_kernel, scratch and one element's weights within 160 KB);
 // else one wavefront per atom with lane n holding neuron n of a layer
 // (widths <= 64), the weights of every element in LDS (mlp_lds_kernel) or
 // streamed from L2 (mlp_energy_grad_kernel).
@@ -471,7 +471,7 @@ extern "C" int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const in
     }
     const size_t scratch = sizeof(double) * ((size_t)nin + 2 * (size_t)nlm * hw);
     // matrix cores: 16 atoms per workgroup, their scratch and one element's
-    // zero-padded weights and biases within 64 KB
+    // zero-padded weights and biases within the 160 KB of a CU
     size_t wmax = 0;
     for (int e = 0; e < nelem; e++) {
         const int *lay = h_layout + e * MLP_STRIDE;
@@ -484,7 +484,15 @@ extern "C" int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const in
     }
     const size_t ldsm =
         sizeof(double) * (MLP_TILE * ((size_t)mlp_row(nin) + 2 * (size_t)nlm * mlp_row(hw)) + wmax);
-    if (ldsm <= 65536) {
+    // above 64 KB (long descriptors, e.g. SO3 6/6: 147 inputs) one workgroup
+    // per CU; the kernel's static LDS (a few hundred bytes) comes on top
+    if (ldsm <= 160 * 1024 - 1024) {
+        static int mfma_dyn_max = 65536;  // dynamic LDS limit set so far
+        if ((int)ldsm > mfma_dyn_max) {
+            PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)mlp_mfma_kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsm));
+            mfma_dyn_max = (int)ldsm;
+        }
         mlp_mfma_kernel<<<(unsigned)div_up(natoms, MLP_TILE), 256, ldsm, s>>>(
             natoms, nin, nelem, d_species, d_layout, d_params, d_dmin, d_scale, d_x, d_e, d_beta, nlm, hw);
         PXFF_LAUNCH_CHECK();

@@ -11,10 +11,13 @@ namespace pxff {
 
 void set_error(const std::string &msg);
 
+// A failed call is reported once: the runtime's last-error slot is cleared
+// so that a later launch check does not report it again.
 #define PXFF_HIP_CHECK(expr)                                                                  \
     do {                                                                                      \
         hipError_t _e = (expr);                                                               \
         if (_e != hipSuccess) {                                                               \
+            (void)hipGetLastError();                                                          \
             ::pxff::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));            \
             return PXFF_EHIP;                                                                 \
         }                                                                                     \

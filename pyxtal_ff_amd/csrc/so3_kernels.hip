@@ -355,8 +355,9 @@ __global__ __launch_bounds__(256) void so3_radial_kernel(So3Args a, int64_t nato
 // and the closed forms of i_0, i_1 with the upward recurrence from XUP on,
 // where it is stable (XUP = 10 for orders <= 6, 20 above).  Checked against
 // scipy.special.spherical_in over x in [1e-3, 100]: <= 1e-14 relative.
+// ix = 1/x, which the caller computes once and reuses.
 template <int LC>
-__device__ inline void sph_bessel_i_all(double x, double *f) {
+__device__ inline void sph_bessel_i_all(double x, double ix, double *f) {
     constexpr double XUP = LC <= 7 ? 10.0 : 20.0;
     if (x < 0.1) {
         const double h = 0.5 * x * x;
@@ -376,10 +377,20 @@ __device__ inline void sph_bessel_i_all(double x, double *f) {
         return;
     }
     if (x < XUP) {
-        const double ix = 1.0 / x;
         const int N = LC + 12 + (int)x;
         double fkp1 = 0.0, fk = 1.0, sum = 0.0;
-        for (int k = N; k >= LC; k--) {
+        int k = N;
+        // two steps per trip (the same operations in the same order as one
+        // step per trip, without the register rotation)
+        for (; k - 1 >= LC; k -= 2) {
+            sum += (2 * k + 1) * fk;
+            const double fkm1 = fkp1 + (2 * k + 1) * ix * fk;
+            sum += (2 * k - 1) * fkm1;
+            const double fkm2 = fk + (2 * k - 1) * ix * fkm1;
+            fkp1 = fkm1;
+            fk = fkm2;
+        }
+        if (k >= LC) {
             sum += (2 * k + 1) * fk;
             const double fkm1 = fkp1 + (2 * k + 1) * ix * fk;
             fkp1 = fk;
@@ -402,7 +413,6 @@ __device__ inline void sph_bessel_i_all(double x, double *f) {
     }
     const double e = exp(x), ei = 1.0 / e;
     const double sh = 0.5 * (e - ei), ch = 0.5 * (e + ei);
-    const double ix = 1.0 / x;
     f[0] = sh * ix;
     if (LC > 1) f[1] = (x * ch - sh) * ix * ix;
 #pragma unroll
@@ -434,20 +444,25 @@ __device__ inline void so3_radial_image(const So3Args &a, int64_t e, int64_t re,
         }
     for (int q = 0; q < a.nq; q++) {
         const double rq = a.rq[q];
+        // the node's quadrature weights, loaded once for every order
+        double g[NMAX];
+#pragma unroll
+        for (int n = 0; n < NMAX; n++) g[n] = a.G[n * a.nq + q];
         const double x = ta * R * rq;
+        const double ix = 1.0 / x;
         double f[LC];
-        sph_bessel_i_all<LC>(x, f);
+        sph_bessel_i_all<LC>(x, ix, f);
+        const double tq = ta * rq;
 #pragma unroll
         for (int l = 0; l < LC; l++) {
             if (l >= L) continue;  // uniform
             // i_0' = i_1; i_l' = i_{l-1} - (l+1)/x i_l
-            const double dil = (l == 0) ? f[1] : f[l - 1] - (l + 1) / x * f[l];
-            const double fd = ta * rq * dil;
+            const double dil = (l == 0) ? f[1] : f[l - 1] - (l + 1) * ix * f[l];
+            const double fd = tq * dil;
 #pragma unroll
             for (int n = 0; n < NMAX; n++) {
-                const double g = a.G[n * a.nq + q];
-                I[l][n] += g * f[l];
-                dI[l][n] += g * fd;
+                I[l][n] += g[n] * f[l];
+                dI[l][n] += g[n] * fd;
             }
         }
     }
@@ -520,9 +535,16 @@ __global__ __launch_bounds__(256) void so3_radial_all_kernel(So3Args a, int64_t 
 // closed form), which otherwise diverge and cost the wavefront all three.
 // Only the order in which images are processed changes: every image's
 // integrals are the same computation as in so3_radial_all_kernel.
-// Workspace (int64): count [NB], start [NB + 1] (start[NB] = number of
-// sorted images), cursor [NB], order [image capacity].
-constexpr int SO3_RBINS = 512;
+// The sort is the two-level one of a radix-sort pass, with no global
+// atomics: SO3_RBLK workgroups each histogram a contiguous chunk of the
+// images in LDS and store it bin-major, one workgroup scans the
+// SO3_RBINS x SO3_RBLK counts into offsets, and each workgroup places its
+// chunk from its own offsets (LDS atomics: the order inside a bin of a
+// chunk is arbitrary, which the values do not see).
+// Workspace (int64): offsets [SO3_RBINS][SO3_RBLK], the number of sorted
+// images, the order [image capacity].
+constexpr int SO3_RBINS = 256, SO3_RBLK = 256;
+constexpr int64_t SO3_RWORK = (int64_t)SO3_RBINS * SO3_RBLK + 1;
 __device__ inline int so3_rbin(const double *__restrict__ img_r, int64_t e, double rc) {
     const double x = img_r[3 * e], y = img_r[3 * e + 1], z = img_r[3 * e + 2];
     const double R = sqrt(x * x + y * y + z * z);
@@ -530,50 +552,90 @@ __device__ inline int so3_rbin(const double *__restrict__ img_r, int64_t e, doub
     return b < 0 ? 0 : (b >= SO3_RBINS ? SO3_RBINS - 1 : b);
 }
 
-__global__ void so3_rbin_zero_kernel(int64_t *__restrict__ work) {
-    for (int k = threadIdx.x; k < SO3_RBINS; k += blockDim.x) work[k] = 0;
+// the chunk [lo, hi) of workgroup blockIdx.x
+__device__ inline void so3_rchunk(int64_t nimg, int64_t &lo, int64_t &hi) {
+    const int64_t ch = (nimg + SO3_RBLK - 1) / SO3_RBLK;
+    lo = (int64_t)blockIdx.x * ch;
+    hi = lo + ch < nimg ? lo + ch : nimg;
 }
 
-__global__ __launch_bounds__(256) void so3_rbin_count_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
-                                                             const double *__restrict__ img_r,
-                                                             const int64_t *__restrict__ img_rev, double rc,
-                                                             int64_t *__restrict__ work) {
-    const int64_t nimg = img_start[natoms];
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nimg; e += (int64_t)gridDim.x * blockDim.x)
-        if (img_rev[e] >= e)
-            atomicAdd(reinterpret_cast<unsigned long long *>(work + so3_rbin(img_r, e, rc)), 1ull);
-}
-
-__global__ __launch_bounds__(SO3_RBINS) void so3_rbin_scan_kernel(int64_t *__restrict__ work) {
-    __shared__ int64_t sc[SO3_RBINS];
-    const int t = threadIdx.x;
-    sc[t] = work[t];
+__global__ __launch_bounds__(256) void so3_rbin_hist_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
+                                                            const double *__restrict__ img_r,
+                                                            const int64_t *__restrict__ img_rev, double rc,
+                                                            int64_t *__restrict__ work) {
+    __shared__ unsigned int h[SO3_RBINS];
+    for (int k = threadIdx.x; k < SO3_RBINS; k += blockDim.x) h[k] = 0;
     __syncthreads();
+    int64_t lo, hi;
+    so3_rchunk(img_start[natoms], lo, hi);
+    for (int64_t e = lo + threadIdx.x; e < hi; e += blockDim.x)
+        if (img_rev[e] >= e) atomicAdd(&h[so3_rbin(img_r, e, rc)], 1u);
+    __syncthreads();
+    for (int k = threadIdx.x; k < SO3_RBINS; k += blockDim.x) work[(int64_t)k * SO3_RBLK + blockIdx.x] = h[k];
+}
+
+// exclusive scan of the SO3_RBINS * SO3_RBLK counts in place (bin-major:
+// bin b's images from workgroup g follow those of bins < b and of
+// workgroups < g in bin b).  16 wavefronts; a wavefront scans a bin's
+// SO3_RBLK counts with 4 per lane (coalesced) and a lane prefix sum.
+constexpr int SO3_RSCAN_T = 1024;
+__global__ __launch_bounds__(SO3_RSCAN_T) void so3_rbin_scan_kernel(int64_t *__restrict__ work) {
+    static_assert(SO3_RBLK == 256 && SO3_RBINS <= SO3_RSCAN_T, "scan layout");
+    constexpr int PER = SO3_RBLK / 64, NW = SO3_RSCAN_T / 64;
+    __shared__ int64_t tot[SO3_RBINS];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int b = wv; b < SO3_RBINS; b += NW) {
+        const int64_t *w = work + (int64_t)b * SO3_RBLK + lane * PER;
+        int64_t v = 0;
+#pragma unroll
+        for (int k = 0; k < PER; k++) v += w[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) tot[b] = v;
+    }
+    __syncthreads();
+    // inclusive scan of the bin totals
     for (int o = 1; o < SO3_RBINS; o <<= 1) {
-        const int64_t v = t >= o ? sc[t - o] : 0;
+        const int64_t v = (t < SO3_RBINS && t >= o) ? tot[t - o] : 0;
         __syncthreads();
-        sc[t] += v;
+        if (t < SO3_RBINS) tot[t] += v;
         __syncthreads();
     }
-    const int64_t excl = sc[t] - work[t];
-    work[SO3_RBINS + t] = excl;                  // start
-    work[2 * SO3_RBINS + 1 + t] = excl;          // cursor
-    if (t == SO3_RBINS - 1) work[2 * SO3_RBINS] = sc[t];  // start[NB]: sorted images
+    for (int b = wv; b < SO3_RBINS; b += NW) {
+        int64_t *w = work + (int64_t)b * SO3_RBLK + lane * PER;
+        int64_t c[PER], loc = 0;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            c[k] = w[k];
+            loc += c[k];
+        }
+        int64_t inc = loc;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int64_t y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        int64_t run = (b > 0 ? tot[b - 1] : 0) + inc - loc;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            w[k] = run;
+            run += c[k];
+        }
+    }
+    if (t == 0) work[SO3_RWORK - 1] = tot[SO3_RBINS - 1];  // sorted images
 }
 
 __global__ __launch_bounds__(256) void so3_rbin_fill_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
                                                             const double *__restrict__ img_r,
                                                             const int64_t *__restrict__ img_rev, double rc,
                                                             int64_t *__restrict__ work) {
-    const int64_t nimg = img_start[natoms];
-    int64_t *cursor = work + 2 * SO3_RBINS + 1;
-    int64_t *order = work + 3 * SO3_RBINS + 1;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nimg; e += (int64_t)gridDim.x * blockDim.x)
-        if (img_rev[e] >= e) {
-            const int b = so3_rbin(img_r, e, rc);
-            const int64_t pos = (int64_t)atomicAdd(reinterpret_cast<unsigned long long *>(cursor + b), 1ull);
-            order[pos] = e;
-        }
+    __shared__ unsigned long long cur[SO3_RBINS];
+    for (int k = threadIdx.x; k < SO3_RBINS; k += blockDim.x)
+        cur[k] = (unsigned long long)work[(int64_t)k * SO3_RBLK + blockIdx.x];
+    __syncthreads();
+    int64_t *order = work + SO3_RWORK;
+    int64_t lo, hi;
+    so3_rchunk(img_start[natoms], lo, hi);
+    for (int64_t e = lo + threadIdx.x; e < hi; e += blockDim.x)
+        if (img_rev[e] >= e) order[atomicAdd(&cur[so3_rbin(img_r, e, rc)], 1ull)] = e;
 }
 
 // Radial integrals of the images in |r| order (see above): one image per
@@ -583,8 +645,8 @@ __global__ __launch_bounds__(256) void so3_radial_sorted_kernel(So3Args a, const
                                                                 const double *__restrict__ img_r,
                                                                 const int64_t *__restrict__ img_rev,
                                                                 double *__restrict__ rad) {
-    const int64_t n = work[2 * SO3_RBINS];
-    const int64_t *order = work + 3 * SO3_RBINS + 1;
+    const int64_t n = work[SO3_RWORK - 1];
+    const int64_t *order = work + SO3_RWORK;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int64_t e = order[t];
@@ -826,6 +888,28 @@ __global__ __launch_bounds__(64) void so3_ctot_lm_kernel(So3Args a, int64_t nato
     }
 }
 
+// A double moved across lanes by one DPP control (two 32-bit moves);
+// lanes of rows outside ROWMASK read 0.
+template <int CTRL, int ROWMASK = 0xf>
+__device__ inline double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wavefront, complete in lane 63: pairs and
+// quads (quad_perm), half rows and rows (row_half_mirror, row_mirror), then
+// row 0 into row 1 and row 2 into row 3 (row_bcast:15), rows 0-1 into rows
+// 2-3 (row_bcast:31).  Vector moves only, no LDS round trips.
+__device__ inline void wave_sum3_lane63(double &a, double &b, double &c) {
+    a += dpp_f64<0xb1>(a); b += dpp_f64<0xb1>(b); c += dpp_f64<0xb1>(c);
+    a += dpp_f64<0x4e>(a); b += dpp_f64<0x4e>(b); c += dpp_f64<0x4e>(c);
+    a += dpp_f64<0x141>(a); b += dpp_f64<0x141>(b); c += dpp_f64<0x141>(c);
+    a += dpp_f64<0x140>(a); b += dpp_f64<0x140>(b); c += dpp_f64<0x140>(c);
+    a += dpp_f64<0x142, 0xa>(a); b += dpp_f64<0x142, 0xa>(b); c += dpp_f64<0x142, 0xa>(c);
+    a += dpp_f64<0x143, 0xc>(a); b += dpp_f64<0x143, 0xc>(b); c += dpp_f64<0x143, 0xc>(c);
+}
+
 // Adjoint dE/dr per image, one lane per (l, m) with l <= lmax + 1
 // ((lmax + 2)^2 <= 64): flat over images, a wavefront takes 64 consecutive
 // images (their scalars staged in LDS, one lane each) and runs them one at
@@ -839,7 +923,7 @@ __global__ __launch_bounds__(64) void so3_ctot_lm_kernel(So3Args a, int64_t nato
 //                   + cv rhat_k Re(conj(beta) Y),
 // cu = w norm_l ex (dfc - 2 alpha R fc), cv = w norm_l ex fc: the same sums
 // as so3_image_kernel<SO3_ADJOINT> in another order, reduced over the
-// lanes by a butterfly.
+// lanes with DPP moves (wave_sum3_lane63).
 constexpr int SO3_ADJ_NST = 10;  // staged: s, t, cos phi, sin phi, R, w ex fc, w ex (dfc - 2 a R fc), rhat
 template <int NMAX>
 __global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t nimg, const int64_t *__restrict__ img_atom,
@@ -1006,13 +1090,8 @@ __global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t ni
             c1 = fma(s, rh1, cv * (ar * g1.re + ai2 * g1.im));
             c2 = fma(s, rh2, cv * (ar * x0.re + ai2 * x0.im));
         }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            c0 += __shfl_xor(c0, o);
-            c1 += __shfl_xor(c1, o);
-            c2 += __shfl_xor(c2, o);
-        }
-        if (lane == 0) {
+        wave_sum3_lane63(c0, c1, c2);
+        if (lane == 63) {
             const int64_t e = base + b;
             dedr[3 * e] = c0;
             dedr[3 * e + 1] = c1;
@@ -1354,11 +1433,11 @@ static int so3_ctot_impl(const pxff_so3_params *p, int64_t natoms, const int64_t
                 // that its batches of 64 qualifying images are full
                 if (d_img_rev && d_work && cap > 0) {
                     // pair images sorted into |r| bins, then one image per lane
-                    const int gb = std::max(1, std::min(div_up(cap, 256), cus() * 8));
-                    so3_rbin_zero_kernel<<<1, 256, 0, s>>>(d_work);
-                    so3_rbin_count_kernel<<<gb, 256, 0, s>>>(natoms, d_img_start, d_img_r, d_img_rev, p->rcut, d_work);
-                    so3_rbin_scan_kernel<<<1, SO3_RBINS, 0, s>>>(d_work);
-                    so3_rbin_fill_kernel<<<gb, 256, 0, s>>>(natoms, d_img_start, d_img_r, d_img_rev, p->rcut, d_work);
+                    so3_rbin_hist_kernel<<<SO3_RBLK, 256, 0, s>>>(natoms, d_img_start, d_img_r, d_img_rev, p->rcut,
+                                                                  d_work);
+                    so3_rbin_scan_kernel<<<1, SO3_RSCAN_T, 0, s>>>(d_work);
+                    so3_rbin_fill_kernel<<<SO3_RBLK, 256, 0, s>>>(natoms, d_img_start, d_img_r, d_img_rev, p->rcut,
+                                                                  d_work);
                     SO3_DISPATCH_NMAX(p->nmax, if constexpr (NM <= 6) {
                         so3_radial_sorted_kernel<NM, 7><<<div_up(cap, 256), 256, 0, s>>>(a, d_work, d_img_r,
                                                                                         d_img_rev, d_radial);

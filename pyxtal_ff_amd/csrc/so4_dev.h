@@ -25,6 +25,7 @@ struct So4Dev {
     int *zgroups;
     int nbrow;        // B row partials per atom
     int *b_brow0;     // [nb] first B row partial of b
+    int zlevel_max;   // largest level of Z blocks: sum of its blocks' half-row entries
     int nytask;       // Y tasks: (level, run of half elements) per wavefront
     int *ytask;       // [nytask * YTASK_STRIDE]
 };
@@ -44,6 +45,21 @@ __host__ __device__ constexpr int nhalf_of(int twoj) {
     int s = 0;
     for (int l = 0; l <= twoj; l++) s += (l / 2 + 1) * (l + 1);
     return s;
+}
+// Half-row entries of the largest level of Z blocks: sum over the blocks
+// (l1, l2, l) of level l (l1 >= l2, |l1 - l2| <= l <= l1 + l2, l1 + l2 - l
+// even) of (l/2 + 1)(l + 1), maximised over l.
+__host__ __device__ constexpr int zlevel_max_of(int twoj) {
+    int best = 0;
+    for (int l = 0; l <= twoj; l++) {
+        int nbl = 0;
+        for (int l1 = 0; l1 <= twoj; l1++)
+            for (int l2 = 0; l2 <= l1; l2++)
+                if (l >= l1 - l2 && l <= l1 + l2 && ((l1 + l2 - l) & 1) == 0) nbl++;
+        const int n = nbl * (l / 2 + 1) * (l + 1);
+        if (n > best) best = n;
+    }
+    return best;
 }
 __host__ __device__ constexpr int nb_of(int twoj) {
     int s = 0;

@@ -104,6 +104,10 @@ int get_so4_tables(int twol, int norm, const So4Dev **out) {
     if ((rc = upload(t.idxu_block, &d.idxu_block))) return rc;
     d.nbrow = t.nbrow;
     if ((rc = upload(t.b_brow0, &d.b_brow0))) return rc;
+    d.zlevel_max = 0;
+    for (int l = 0; l + 1 < (int)t.blk_level_start.size(); l++)
+        d.zlevel_max = std::max(d.zlevel_max,
+                                (t.blk_level_start[l + 1] - t.blk_level_start[l]) * (l / 2 + 1) * (l + 1));
     d.nzgroups = t.nzgroups;
     if ((rc = upload(t.zgroups, &d.zgroups))) return rc;
     // inverse of the block targets: the (block, factor) pairs of each b
@@ -1126,6 +1130,143 @@ __global__ __launch_bounds__(64) void so4_dbdr_kernel(
     }
 }
 
+// --------------------------------------- plain dB/dr, one centre per workgroup
+// The same dB/dr per image as so4_dbdr_kernel, organised around the centre:
+// a workgroup (one wavefront) takes one centre and runs its images NG at a
+// time.  With the su(2) generators of the image (see su2_generators), the
+// derivative of a block product needs U alone:
+//   Re <dU_tot/dr_k, Z_blk> = fc (p_k A1 + Re q_k A2 + Im q_k A3) + dfc rhat_k Au,
+// A the generator sums of the image's U rows against the block's Z rows
+// (generator_dots; the middle row of an even level weighted 1 / 0.5 / 0
+// for ma < , = , > l/2).  Level by level the groups put their U half rows
+// into LDS and the wavefront stages the centre's Z blocks of the level
+// there (weights applied), read once from the tile-major Z; then each lane
+// sums one (image, block) pair over all the block's rows and adds
+// 2 norm_l w f (.) to the block's (b, factor) targets (three terms of
+// compute_dbidrj, SNAP.py:1143-1257; within a level each (b, k) receives
+// at most one block).  Workgroups are mapped to centres so that the eight
+// centres whose Z entries share a 128-byte line run on one XCD (the Z reads
+// of the tile-major layout then hit that XCD's L2).
+template <int TWOJ>
+__global__ __launch_bounds__(64) void so4_dbdr_centre_kernel(
+    int64_t natoms, const int64_t *__restrict__ img_start, const int64_t *__restrict__ img_j,
+    const double *__restrict__ img_r, const double *__restrict__ atom_w, const double *__restrict__ zt, int nz,
+    int nb, const double *rootpq_g, const double *__restrict__ lnorm, const int *__restrict__ blk_level_start,
+    const int *__restrict__ blk_zoff, const int *__restrict__ blk_tb, const double *__restrict__ blk_tf, double rcut,
+    double rfac0, int cutoff_id, double *__restrict__ dbdr) {
+    constexpr int G = TWOJ / 2 + 1;
+    constexpr int NG = 64 / G;
+    constexpr int NL = (TWOJ / 2 + 1) * (TWOJ + 1);
+    constexpr int NBMAX = nb_of(TWOJ);
+    constexpr int ZLM = zlevel_max_of(TWOJ);
+    constexpr int NSC = 15;  // per image: fc, dfc, w, rhat[3], p[3], Re q[3], Im q[3]
+    __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
+    __shared__ cd Ul[NG][NL];
+    __shared__ cd Zl[ZLM];
+    __shared__ double acc[NG][NBMAX * 3];
+    __shared__ double sc[NG][NSC];
+    __shared__ int s_ok[NG];
+    // centre of this workgroup: swap the 3-bit fields (b / 8) % 8 and b % 8
+    // of the block index (the XCD is b % 8)
+    const int64_t bid = blockIdx.x;
+    const int64_t c = (bid & ~(int64_t)63) | ((bid & 7) << 3) | ((bid >> 3) & 7);
+    if (c >= natoms) return;
+    const int lane = threadIdx.x;
+    const int g = lane / G, mb = lane % G;
+    for (int k = lane; k < (TWOJ + 1) * (TWOJ + 1); k += 64) rpq[k] = rootpq_g[k];
+    const int64_t s0 = img_start[c], s1 = img_start[c + 1];
+    const cd *zc = reinterpret_cast<const cd *>(zt) + (size_t)(c >> 6) * nz * 64 + (c & 63);
+    for (int64_t e0 = s0; e0 < s1; e0 += NG) {
+        for (int k = lane; k < NG * NBMAX * 3; k += 64) (&acc[0][0])[k] = 0.0;
+        const int64_t e = e0 + g;
+        bool valid = g < NG && e < s1;
+        double x = 0, yy = 0, zz = 1.0, r = 1.0, w = 0.0;
+        if (valid) {
+            x = img_r[3 * e];
+            yy = img_r[3 * e + 1];
+            zz = img_r[3 * e + 2];
+            r = sqrt(x * x + yy * yy + zz * zz);
+            w = atom_w[img_j[e]];
+            if (r < 1e-8) {  // coincident atoms contribute nothing (SNAP.py:1487)
+                valid = false;
+                x = 0;
+                yy = 0;
+                zz = 1.0;
+                r = 1.0;
+            }
+        }
+        double fc, dfc;
+        cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
+        CK ck;
+        cayley_klein(x, yy, zz, r, rfac0 * M_PI * r / rcut, true, ck);
+        if (g < NG && mb == 0) {
+            double p[3];
+            cd q[3];
+            su2_generators(ck, p, q);
+            double *S = sc[g];
+            S[0] = fc;
+            S[1] = dfc;
+            S[2] = w;
+            S[3] = x / r;
+            S[4] = yy / r;
+            S[5] = zz / r;
+            for (int k = 0; k < 3; k++) {
+                S[6 + k] = p[k];
+                S[9 + k] = q[k].re;
+                S[12 + k] = q[k].im;
+            }
+            s_ok[g] = valid ? 1 : 0;
+        }
+        __syncthreads();
+        sweep_rows<TWOJ, false>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
+            (void)du;
+            if (g < NG && half) {
+#pragma unroll
+                for (int m = 0; m <= TWOJ; m++)
+                    if (m <= l) Ul[g][mb * (l + 1) + m] = u[m];
+            }
+            const int b0 = blk_level_start[l], nbl = blk_level_start[l + 1] - b0;
+            const int nloc = (l / 2 + 1) * (l + 1);
+            for (int it = lane; it < nbl * nloc; it += 64) {
+                const int bk = it / nloc, loc = it - bk * nloc;
+                const int row = loc / (l + 1), ma = loc - row * (l + 1);
+                const double wt = 2 * row < l ? 1.0 : (ma < row ? 1.0 : (ma == row ? 0.5 : 0.0));
+                const cd zv = zc[(size_t)(blk_zoff[b0 + bk] + loc) * 64];
+                Zl[it] = cmake(wt * zv.re, wt * zv.im);
+            }
+            __syncthreads();
+            const double nrm2 = 2.0 * lnorm[l];
+            const int nrow = l / 2 + 1;
+            for (int it = lane; it < NG * nbl; it += 64) {
+                const int bk = it / NG, gg = it - bk * NG;
+                if (!s_ok[gg]) continue;
+                double au = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+                for (int row = 0; row < nrow; row++)
+                    generator_dots<TWOJ>(l, &Ul[gg][row * (l + 1)], &Zl[bk * nloc + row * (l + 1)], au, a1, a2, a3);
+                const double *S = sc[gg];
+                const double f0 = S[0] * nrm2 * S[2], f1 = S[1] * nrm2 * S[2];
+                const int blk = b0 + bk;
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const double pk = f0 * (S[6 + k] * a1 + S[9 + k] * a2 + S[12 + k] * a3) + f1 * S[3 + k] * au;
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        const int b = blk_tb[3 * blk + q];
+                        if (b >= 0) acc[gg][3 * b + k] += blk_tf[3 * blk + q] * pk;
+                    }
+                }
+            }
+            __syncthreads();
+        });
+        for (int gg = 0; gg < NG; gg++) {
+            const int64_t im = e0 + gg;
+            if (im >= s1) break;
+            for (int k = lane; k < nb * 3; k += 64) dbdr[(size_t)im * nb * 3 + k] = acc[gg][k];
+        }
+        __syncthreads();
+    }
+}
+
 // -------------------------------------------------------------- row scatter
 // Reference layout rows (SNAP.py:1458-1468).  One workgroup per centre;
 // threads over the (b, k) / (b, a, c) components, looping over the centre's
@@ -1673,6 +1814,29 @@ extern "C" int pxff_so4_dbdr(const pxff_so4_params *p, int64_t nimg, const int64
     const int grid = div_up(nimg, ng);
     PXFF_DISPATCH_TWOJ(p->twol, (so4_dbdr_kernel<TJ><<<grid, 64, 0, s>>>(
                                     nimg, d_img_atom, d_img_j, d_img_r, d_atom_w, d_z, T->nz, T->nb, T->rootpq,
+                                    T->lnorm, T->blk_level_start, T->blk_zoff, T->blk_tb, T->blk_tf, p->rcut,
+                                    p->rfac0, p->cutoff_id, d_dbdr)));
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_so4_dbdr_centres(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start,
+                                     const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
+                                     const double *d_z, double *d_dbdr, void *stream) {
+    int rc = so4_check_params(p);
+    if (rc) return rc;
+    if (natoms <= 0) return 0;
+    const So4Dev *T = nullptr;
+    if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
+    PXFF_REQUIRE(T->zlevel_max <= zlevel_max_of(p->twol) && T->nb <= nb_of(p->twol),
+                 "Z level staging exceeds its LDS bound");
+    PXFF_REQUIRE(natoms <= ((int64_t)1 << 31) - 64, "too many atoms for one launch");
+    hipStream_t s = (hipStream_t)stream;
+    // one workgroup per centre, rounded up to whole 64-centre tiles (the
+    // centre map permutes within a tile; centres >= natoms exit)
+    const unsigned grid = (unsigned)((natoms + 63) / 64 * 64);
+    PXFF_DISPATCH_TWOJ(p->twol, (so4_dbdr_centre_kernel<TJ><<<grid, 64, 0, s>>>(
+                                    natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_z, T->nz, T->nb, T->rootpq,
                                     T->lnorm, T->blk_level_start, T->blk_zoff, T->blk_tb, T->blk_tf, p->rcut,
                                     p->rfac0, p->cutoff_id, d_dbdr)));
     PXFF_LAUNCH_CHECK();

@@ -16,7 +16,7 @@ import torch
 from . import _native as nat
 
 CUTOFF_IDS = {"cosine": 1, "tanh": 2, "poly1": 3, "poly2": 4, "poly3": 5, "poly4": 6, "exp": 7, "unity": 8}
-RBINS = 512  # |r| bins of the radial kernel's scheduling (pxff_so3_ctot_pairs)
+RWORK = 256 * 256 + 1  # |r| bins x chunks + count: the fixed part of pxff_so3_ctot_pairs' workspace
 
 
 def W(nmax):
@@ -102,7 +102,7 @@ def ctot(nl, cfg, atom_z, img_r=None, dr=None, radial=False):
         # so the radial integrals are evaluated once per pair
         # workspace of the |r| ordering of the pairs (scheduling only)
         cap = nl.img_capacity
-        work = torch.empty(3 * RBINS + 1 + cap, dtype=torch.int64, device=nl.pos.device)
+        work = torch.empty(RWORK + cap, dtype=torch.int64, device=nl.pos.device)
         nat.check(nat.lib().pxff_so3_ctot_pairs(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r),
                                                 p(nl.img_rev), p(atom_z), p(cfg.G), p(cfg.rq), p(cfg.lnorm),
                                                 p(out), p(rad), p(work), cap, nat.stream()), "pxff_so3_ctot_pairs")

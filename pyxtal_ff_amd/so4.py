@@ -203,8 +203,19 @@ def z_atom_major(nl, cfg, z):
 
 
 def dbdr(nl, cfg, atom_w, z, img_r=None):
-    """Materialised dB/dr per image [nimg, nb, 3] (z: the tile-major Z; the
-    kernel reads an atom-major copy)."""
+    """Materialised dB/dr per image [nimg, nb, 3] from the tile-major Z, one
+    workgroup per centre (pxff_so4_dbdr_centres)."""
+    p = nat.ptr
+    out = _f64(nl.nimg * cfg.nb * 3, nl.pos.device)
+    r = nl.img_r if img_r is None else img_r.contiguous()
+    nat.check(nat.lib().pxff_so4_dbdr_centres(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r), p(atom_w),
+                                              p(z), p(out), nat.stream()), "pxff_so4_dbdr_centres")
+    return out
+
+
+def dbdr_per_image(nl, cfg, atom_w, z, img_r=None):
+    """The same dB/dr from the per-image kernel over an atom-major copy of Z
+    (pxff_so4_z_atom_major + pxff_so4_dbdr)."""
     p = nat.ptr
     out = _f64(nl.nimg * cfg.nb * 3, nl.pos.device)
     r = nl.img_r if img_r is None else img_r.contiguous()

@@ -1,7 +1,9 @@
 """The fused network kernel (pxff_mlp_energy_grad: per-element networks
 forward and backward to dE/dx) against torch autograd on the same
 parameters: three elements mixed over 300 atoms, narrow and deep / wide
-networks (up to 8 layers of 64), and atoms without a network."""
+networks (up to 8 layers of 64), a 147-long descriptor (SO3 nmax = lmax =
+6: the matrix-core kernel with more than 64 KB of LDS), and atoms without a
+network."""
 import numpy as np
 import pytest
 import torch
@@ -20,10 +22,11 @@ def _torch_reference(model, x, symbols):
     return e.detach(), g
 
 
-@pytest.mark.parametrize("hidden,act", [((30, 30), "Tanh"), ((24, 16, 8), "Softplus"), ((64,) * 7, "ELU")])
-def test_fused_network_matches_autograd(hidden, act):
+@pytest.mark.parametrize("hidden,act,nin", [((30, 30), "Tanh", 55), ((24, 16, 8), "Softplus", 55),
+                                            ((64,) * 7, "ELU", 55), ((30, 30), "Tanh", 147)])
+def test_fused_network_matches_autograd(hidden, act, nin):
     rng = np.random.default_rng(5)
-    n, nin = 300, 55
+    n = 300
     symbols = list(rng.choice(["O", "Si", "C"], size=n, p=[0.5, 0.4, 0.1]))
     x = torch.as_tensor(rng.random((n, nin)), dtype=torch.float64, device="cuda")
     dr = {e: np.stack([np.full(nin, -0.1), np.full(nin, 1.2)], 1) for e in ("O", "Si", "C")}
