@@ -1140,13 +1140,27 @@ __global__ __launch_bounds__(64) void so4_dbdr_kernel(
 // (generator_dots; the middle row of an even level weighted 1 / 0.5 / 0
 // for ma < , = , > l/2).  Level by level the groups put their U half rows
 // into LDS and the wavefront stages the centre's Z blocks of the level
-// there (weights applied), read once from the tile-major Z; then each lane
-// sums one (image, block) pair over all the block's rows and adds
+// there (weights applied), read from the tile-major Z; then each lane sums
+// one (image, block) pair over all the block's rows and adds
 // 2 norm_l w f (.) to the block's (b, factor) targets (three terms of
-// compute_dbidrj, SNAP.py:1143-1257; within a level each (b, k) receives
-// at most one block).  Workgroups are mapped to centres so that the eight
-// centres whose Z entries share a 128-byte line run on one XCD (the Z reads
-// of the tile-major layout then hit that XCD's L2).
+// compute_dbidrj, SNAP.py:1143-1257) directly in the output rows: within a
+// level each (image, b, k) receives at most one block, and the barriers
+// between levels order the additions of different levels.  LDS holds only
+// the level's U rows, its Z blocks and per-image factors; NG is chosen so
+// that it stays within 20 KB where possible (two workgroups per SIMD).
+// Workgroups are mapped to centres so that the eight centres whose Z
+// entries share a 128-byte line run on one XCD (the Z reads of the
+// tile-major layout then hit that XCD's L2).
+template <int TWOJ>
+__host__ __device__ constexpr int dbdr_centre_ng() {
+    constexpr int G = TWOJ / 2 + 1;
+    constexpr int NL = (TWOJ / 2 + 1) * (TWOJ + 1);
+    constexpr int fixed = zlevel_max_of(TWOJ) * 16 + (TWOJ + 1) * (TWOJ + 1) * 8;
+    constexpr int per = NL * 16 + 12 * 8;
+    constexpr int fit = (20480 - fixed) / per;
+    return fit >= 64 / G ? 64 / G : (fit >= 4 ? fit : 64 / G);
+}
+
 template <int TWOJ>
 __global__ __launch_bounds__(64) void so4_dbdr_centre_kernel(
     int64_t natoms, const int64_t *__restrict__ img_start, const int64_t *__restrict__ img_j,
@@ -1155,17 +1169,15 @@ __global__ __launch_bounds__(64) void so4_dbdr_centre_kernel(
     const int *__restrict__ blk_zoff, const int *__restrict__ blk_tb, const double *__restrict__ blk_tf, double rcut,
     double rfac0, int cutoff_id, double *__restrict__ dbdr) {
     constexpr int G = TWOJ / 2 + 1;
-    constexpr int NG = 64 / G;
+    constexpr int NG = dbdr_centre_ng<TWOJ>();
     constexpr int NL = (TWOJ / 2 + 1) * (TWOJ + 1);
-    constexpr int NBMAX = nb_of(TWOJ);
     constexpr int ZLM = zlevel_max_of(TWOJ);
-    constexpr int NSC = 15;  // per image: fc, dfc, w, rhat[3], p[3], Re q[3], Im q[3]
+    // per image: w fc p_k, w fc Re q_k, w fc Im q_k, w dfc rhat_k (k = 0..2)
+    constexpr int NSC = 12;
     __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
     __shared__ cd Ul[NG][NL];
     __shared__ cd Zl[ZLM];
-    __shared__ double acc[NG][NBMAX * 3];
     __shared__ double sc[NG][NSC];
-    __shared__ int s_ok[NG];
     // centre of this workgroup: swap the 3-bit fields (b / 8) % 8 and b % 8
     // of the block index (the XCD is b % 8)
     const int64_t bid = blockIdx.x;
@@ -1177,9 +1189,11 @@ __global__ __launch_bounds__(64) void so4_dbdr_centre_kernel(
     const int64_t s0 = img_start[c], s1 = img_start[c + 1];
     const cd *zc = reinterpret_cast<const cd *>(zt) + (size_t)(c >> 6) * nz * 64 + (c & 63);
     for (int64_t e0 = s0; e0 < s1; e0 += NG) {
-        for (int k = lane; k < NG * NBMAX * 3; k += 64) (&acc[0][0])[k] = 0.0;
+        const int ne = (int)(s1 - e0 < NG ? s1 - e0 : NG);
+        // the chunk's output rows start at zero
+        for (int k = lane; k < ne * nb * 3; k += 64) dbdr[(size_t)e0 * nb * 3 + k] = 0.0;
         const int64_t e = e0 + g;
-        bool valid = g < NG && e < s1;
+        bool valid = g < ne;
         double x = 0, yy = 0, zz = 1.0, r = 1.0, w = 0.0;
         if (valid) {
             x = img_r[3 * e];
@@ -1193,6 +1207,7 @@ __global__ __launch_bounds__(64) void so4_dbdr_centre_kernel(
                 yy = 0;
                 zz = 1.0;
                 r = 1.0;
+                w = 0.0;
             }
         }
         double fc, dfc;
@@ -1204,18 +1219,19 @@ __global__ __launch_bounds__(64) void so4_dbdr_centre_kernel(
             cd q[3];
             su2_generators(ck, p, q);
             double *S = sc[g];
-            S[0] = fc;
-            S[1] = dfc;
-            S[2] = w;
-            S[3] = x / r;
-            S[4] = yy / r;
-            S[5] = zz / r;
-            for (int k = 0; k < 3; k++) {
-                S[6 + k] = p[k];
-                S[9 + k] = q[k].re;
-                S[12 + k] = q[k].im;
-            }
-            s_ok[g] = valid ? 1 : 0;
+            const double wf = w * fc, wd = w * dfc / r;
+            S[0] = wf * p[0];
+            S[1] = wf * p[1];
+            S[2] = wf * p[2];
+            S[3] = wf * q[0].re;
+            S[4] = wf * q[1].re;
+            S[5] = wf * q[2].re;
+            S[6] = wf * q[0].im;
+            S[7] = wf * q[1].im;
+            S[8] = wf * q[2].im;
+            S[9] = wd * x;
+            S[10] = wd * yy;
+            S[11] = wd * zz;
         }
         __syncthreads();
         sweep_rows<TWOJ, false>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
@@ -1234,36 +1250,36 @@ __global__ __launch_bounds__(64) void so4_dbdr_centre_kernel(
                 const cd zv = zc[(size_t)(blk_zoff[b0 + bk] + loc) * 64];
                 Zl[it] = cmake(wt * zv.re, wt * zv.im);
             }
+            // also orders the previous level's output additions before this
+            // level's
             __syncthreads();
             const double nrm2 = 2.0 * lnorm[l];
             const int nrow = l / 2 + 1;
-            for (int it = lane; it < NG * nbl; it += 64) {
-                const int bk = it / NG, gg = it - bk * NG;
-                if (!s_ok[gg]) continue;
+            for (int it = lane; it < ne * nbl; it += 64) {
+                const int bk = it / ne, gg = it - bk * ne;
                 double au = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
                 for (int row = 0; row < nrow; row++)
                     generator_dots<TWOJ>(l, &Ul[gg][row * (l + 1)], &Zl[bk * nloc + row * (l + 1)], au, a1, a2, a3);
                 const double *S = sc[gg];
-                const double f0 = S[0] * nrm2 * S[2], f1 = S[1] * nrm2 * S[2];
                 const int blk = b0 + bk;
+                double *o = dbdr + (size_t)(e0 + gg) * nb * 3;
+                int tb[3];
+                double tf[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    tb[q] = blk_tb[3 * blk + q];
+                    tf[q] = blk_tf[3 * blk + q] * nrm2;
+                }
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
-                    const double pk = f0 * (S[6 + k] * a1 + S[9 + k] * a2 + S[12 + k] * a3) + f1 * S[3 + k] * au;
+                    const double pk = S[k] * a1 + S[3 + k] * a2 + S[6 + k] * a3 + S[9 + k] * au;
 #pragma unroll
-                    for (int q = 0; q < 3; q++) {
-                        const int b = blk_tb[3 * blk + q];
-                        if (b >= 0) acc[gg][3 * b + k] += blk_tf[3 * blk + q] * pk;
-                    }
+                    for (int q = 0; q < 3; q++)
+                        if (tb[q] >= 0) o[3 * tb[q] + k] += tf[q] * pk;
                 }
             }
             __syncthreads();
         });
-        for (int gg = 0; gg < NG; gg++) {
-            const int64_t im = e0 + gg;
-            if (im >= s1) break;
-            for (int k = lane; k < nb * 3; k += 64) dbdr[(size_t)im * nb * 3 + k] = acc[gg][k];
-        }
-        __syncthreads();
     }
 }
 
