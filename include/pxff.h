@@ -341,8 +341,8 @@ int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img
  * e and its reverse d_img_rev[e] (pxff_nlist_rev) have bitwise the same |r|:
  * the radial integrals are evaluated once per image pair and written to
  * both rows (same values as pxff_so3_ctot).  With d_work (int64 workspace
- * of 256*256 + 1 + cap entries, cap = the image capacity of the arrays) the
- * pairs are first ordered by |r| (a counting sort into 256 bins, over 256
+ * of 256*64 + 1 + cap entries, cap = the image capacity of the arrays) the
+ * pairs are first ordered by |r| (a counting sort into 256 bins, over 64
  * chunks of the images) so that the images a wavefront evaluates together
  * take the same branches; the values do not depend on it.  d_work = NULL:
  * the unsorted order. */

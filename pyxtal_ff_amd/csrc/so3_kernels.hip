@@ -543,7 +543,7 @@ __global__ __launch_bounds__(256) void so3_radial_all_kernel(So3Args a, int64_t 
 // chunk is arbitrary, which the values do not see).
 // Workspace (int64): offsets [SO3_RBINS][SO3_RBLK], the number of sorted
 // images, the order [image capacity].
-constexpr int SO3_RBINS = 256, SO3_RBLK = 256;
+constexpr int SO3_RBINS = 256, SO3_RBLK = 64, SO3_RBT = 1024;  // bins, chunks, threads per chunk
 constexpr int64_t SO3_RWORK = (int64_t)SO3_RBINS * SO3_RBLK + 1;
 __device__ inline int so3_rbin(const double *__restrict__ img_r, int64_t e, double rc) {
     const double x = img_r[3 * e], y = img_r[3 * e + 1], z = img_r[3 * e + 2];
@@ -559,7 +559,7 @@ __device__ inline void so3_rchunk(int64_t nimg, int64_t &lo, int64_t &hi) {
     hi = lo + ch < nimg ? lo + ch : nimg;
 }
 
-__global__ __launch_bounds__(256) void so3_rbin_hist_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
+__global__ __launch_bounds__(SO3_RBT) void so3_rbin_hist_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
                                                             const double *__restrict__ img_r,
                                                             const int64_t *__restrict__ img_rev, double rc,
                                                             int64_t *__restrict__ work) {
@@ -576,54 +576,46 @@ __global__ __launch_bounds__(256) void so3_rbin_hist_kernel(int64_t natoms, cons
 
 // exclusive scan of the SO3_RBINS * SO3_RBLK counts in place (bin-major:
 // bin b's images from workgroup g follow those of bins < b and of
-// workgroups < g in bin b).  16 wavefronts; a wavefront scans a bin's
-// SO3_RBLK counts with 4 per lane (coalesced) and a lane prefix sum.
+// workgroups < g in bin b).  One workgroup: thread t sums the 16
+// consecutive counts it owns (one 128-byte line), one workgroup scan of the
+// 1024 thread sums, then thread t rewrites its counts as offsets.
 constexpr int SO3_RSCAN_T = 1024;
 __global__ __launch_bounds__(SO3_RSCAN_T) void so3_rbin_scan_kernel(int64_t *__restrict__ work) {
-    static_assert(SO3_RBLK == 256 && SO3_RBINS <= SO3_RSCAN_T, "scan layout");
-    constexpr int PER = SO3_RBLK / 64, NW = SO3_RSCAN_T / 64;
-    __shared__ int64_t tot[SO3_RBINS];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    for (int b = wv; b < SO3_RBINS; b += NW) {
-        const int64_t *w = work + (int64_t)b * SO3_RBLK + lane * PER;
-        int64_t v = 0;
+    constexpr int TOT = SO3_RBINS * SO3_RBLK;
+    constexpr int PER = TOT / SO3_RSCAN_T;
+    static_assert(TOT % SO3_RSCAN_T == 0, "scan layout");
+    __shared__ int64_t sc[SO3_RSCAN_T / 64];
+    const int t = threadIdx.x;
+    int64_t *w = work + (int64_t)t * PER;
+    int64_t loc = 0;
 #pragma unroll
-        for (int k = 0; k < PER; k++) v += w[k];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        if (lane == 0) tot[b] = v;
+    for (int k = 0; k < PER; k++) loc += w[k];
+    // inclusive scan: within each wavefront by shuffles, then over the 16
+    // wavefront totals
+    const int lane = t & 63, wv = t >> 6;
+    int64_t inc = loc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
     }
+    if (lane == 63) sc[wv] = inc;
     __syncthreads();
-    // inclusive scan of the bin totals
-    for (int o = 1; o < SO3_RBINS; o <<= 1) {
-        const int64_t v = (t < SO3_RBINS && t >= o) ? tot[t - o] : 0;
-        __syncthreads();
-        if (t < SO3_RBINS) tot[t] += v;
-        __syncthreads();
-    }
-    for (int b = wv; b < SO3_RBINS; b += NW) {
-        int64_t *w = work + (int64_t)b * SO3_RBLK + lane * PER;
-        int64_t c[PER], loc = 0;
+    int64_t before = 0;
+    for (int k = 0; k < wv; k++) before += sc[k];
+    const int64_t total = before + (wv == SO3_RSCAN_T / 64 - 1 ? inc : 0);
+    int64_t run = before + inc - loc;
+    // second pass over the (cached) counts
 #pragma unroll
-        for (int k = 0; k < PER; k++) {
-            c[k] = w[k];
-            loc += c[k];
-        }
-        int64_t inc = loc;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int64_t y = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += y;
-        }
-        int64_t run = (b > 0 ? tot[b - 1] : 0) + inc - loc;
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            w[k] = run;
-            run += c[k];
-        }
+    for (int k = 0; k < PER; k++) {
+        const int64_t v = w[k];
+        w[k] = run;
+        run += v;
     }
-    if (t == 0) work[SO3_RWORK - 1] = tot[SO3_RBINS - 1];  // sorted images
+    if (t == SO3_RSCAN_T - 1) work[SO3_RWORK - 1] = total;  // sorted images
 }
 
-__global__ __launch_bounds__(256) void so3_rbin_fill_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
+__global__ __launch_bounds__(SO3_RBT) void so3_rbin_fill_kernel(int64_t natoms, const int64_t *__restrict__ img_start,
                                                             const double *__restrict__ img_r,
                                                             const int64_t *__restrict__ img_rev, double rc,
                                                             int64_t *__restrict__ work) {
@@ -772,6 +764,39 @@ __global__ __launch_bounds__(64) void so3_accum_kernel(So3Args a, int64_t natoms
     }
 }
 
+// Per-image tables of the spherical harmonics, built by one lane per image
+// (so3_ylm_tables): Pbar_l^m(cos theta) for m <= l < LT (the normalised
+// associated Legendre recurrence of ylm_row, in the same order) and
+// e^{i m phi} for m < LT (repeated multiplication).  Y_l^m = Pbar_l^m
+// e^{i m phi} and Y_l^-m = (-1)^m conj(Y_l^m): a lane of a fixed (l, m)
+// then needs two LDS reads per image instead of its own recurrences (the
+// adjoint kernel below; in the expansion kernel, whose per-image work is
+// small, the lanes' own recurrences are faster than building tables).
+template <int LT>
+__device__ inline void so3_ylm_tables(int lmx, double st, double ct, double cp, double sp, double *P, cd *E) {
+    const double inv4pi = 0.25 / M_PI;
+    double pmm = sqrt(inv4pi);
+    cd em = cmake(1.0, 0.0);
+    for (int m = 0; m <= lmx; m++) {
+        if (m > 0) {
+            pmm = g_so3_pmm[m] * st * pmm;
+            em = cmul(em, cmake(cp, sp));
+        }
+        E[m] = em;
+        P[m * (m + 1) / 2 + m] = pmm;
+        if (m < lmx) {
+            double pm2 = pmm, pm1 = g_so3_p1[m] * ct * pmm;
+            P[(m + 1) * (m + 2) / 2 + m] = pm1;
+            for (int k = m + 2; k <= lmx; k++) {
+                const double pk = g_so3_lega[k * SO3_LEGD + m] * (ct * pm1 - g_so3_legb[k * SO3_LEGD + m] * pm2);
+                pm2 = pm1;
+                pm1 = pk;
+                P[k * (k + 1) / 2 + m] = pk;
+            }
+        }
+    }
+}
+
 // ctot[i][n][l][m] = sum_e c_e,nlm = sum_e w norm_l ex fc I_nl Y_lm (SO3.py:
 // 205-226) with the radial integrals I_nl read from the radial store.  One
 // wavefront per centre (persistent), one lane per (l, m), (lmax + 1)^2 <= 64:
@@ -913,8 +938,9 @@ __device__ inline void wave_sum3_lane63(double &a, double &b, double &c) {
 // Adjoint dE/dr per image, one lane per (l, m) with l <= lmax + 1
 // ((lmax + 2)^2 <= 64): flat over images, a wavefront takes 64 consecutive
 // images (their scalars staged in LDS, one lane each) and runs them one at
-// a time.  Per image every lane evaluates its Y_lm (as so3_ctot_lm_kernel)
-// into a 64-entry LDS table, from which the lanes with l <= lmax read the
+// a time.  Per 16 images, one lane per image builds its Y tables
+// (so3_ylm_tables, rows <= lmax + 1); per image every lane forms its Y_lm
+// from them into a 64-entry LDS table, from which the lanes with l <= lmax read the
 // Y_{l+-1, m'} of the covariant gradient (SO3.py:649-674; its coefficients
 // depend on (l, m) only and are held in registers).  With
 // alpha = sum_n I_nl A_nlm and beta = sum_n dI_nl A_nlm (A of the image's
@@ -925,6 +951,9 @@ __device__ inline void wave_sum3_lane63(double &a, double &b, double &c) {
 // as so3_image_kernel<SO3_ADJOINT> in another order, reduced over the
 // lanes with DPP moves (wave_sum3_lane63).
 constexpr int SO3_ADJ_NST = 10;  // staged: s, t, cos phi, sin phi, R, w ex fc, w ex (dfc - 2 a R fc), rhat
+constexpr int SO3_ADJ_LT = 8;                                   // Y rows l <= 7
+constexpr int SO3_ADJ_NP = SO3_ADJ_LT * (SO3_ADJ_LT + 1) / 2;  // (l, m >= 0)
+constexpr int SO3_ADJ_TB = 16;                                  // images per table batch
 template <int NMAX>
 __global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t nimg, const int64_t *__restrict__ img_atom,
                                                             const int64_t *__restrict__ img_j,
@@ -936,6 +965,8 @@ __global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t ni
     __shared__ double stg[SO3_ADJ_NST][64];
     __shared__ int64_t s_at[64];
     __shared__ cd ysh[64];
+    __shared__ double Pt[SO3_ADJ_TB][SO3_ADJ_NP + 1];
+    __shared__ cd Et[SO3_ADJ_TB][SO3_ADJ_LT];
     if (d_count) {
         const int64_t c = *d_count;
         nimg = c < nimg ? c : nimg;
@@ -952,17 +983,8 @@ __global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t ni
     const bool ylane = lane < (L + 1) * (L + 1);  // rows 0..lmax+1
     const bool act = lane < L * L;                // rows 0..lmax
     const int am = m < 0 ? -m : m;
-    constexpr int LK = 8;  // rows l <= 7
-    double pmmf[LK], lega_[LK], legb_[LK];
-    double p1 = 0.0;
-#pragma unroll
-    for (int k = 0; k < LK; k++) {
-        pmmf[k] = (ylane && k >= 1 && k <= am) ? g_so3_pmm[k] : 1.0;
-        lega_[k] = (ylane && k >= am + 2 && k <= l) ? g_so3_lega[k * SO3_LEGD + am] : 0.0;
-        legb_[k] = (ylane && k >= am + 2 && k <= l) ? g_so3_legb[k * SO3_LEGD + am] : 0.0;
-    }
-    if (ylane && l > am) p1 = g_so3_p1[am];
-    const double msg = (m < 0 && (am & 1)) ? -1.0 : 1.0, imsg = m < 0 ? -1.0 : 1.0;
+    const int pidx = ylane ? l * (l + 1) / 2 + am : 0;
+    const double msg = (m < 0 && (am & 1)) ? -1.0 : 1.0, imsg = m < 0 ? -msg : msg;
     // covariant gradient coefficients of the lane's (l, m) and the lanes of
     // the Y values they multiply (-1: term absent)
     const double dl = l, dm = m;
@@ -990,7 +1012,7 @@ __global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t ni
         }
     }
     const double lnorm = act ? a.lnorm[l] : 0.0;
-    const double inv4pi = 0.25 / M_PI, r2 = 1.0 / sqrt(2.0);
+    const double r2 = 1.0 / sqrt(2.0);
     {
         const int64_t e = base + lane;
         const bool ok = lane < nb;
@@ -1031,29 +1053,19 @@ __global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t ni
                 }
             }
         }
-        const double st = stg[0][b], ct = stg[1][b], cp = stg[2][b], sp = stg[3][b];
-        // Y_lm of every lane (rows 0..lmax+1)
-        double pmm = sqrt(inv4pi);
-        cd em = cmake(1.0, 0.0);
-#pragma unroll
-        for (int k = 1; k < LK; k++)
-            if (k <= am) {
-                pmm = pmmf[k] * st * pmm;
-                em = cmul(em, cmake(cp, sp));
-            }
-        double pl = pmm;
-        if (l > am) {
-            double pm2 = pmm, pm1 = p1 * ct * pmm;
-#pragma unroll
-            for (int k = 2; k < LK; k++)
-                if (k >= am + 2 && k <= l) {
-                    const double pk = lega_[k] * (ct * pm1 - legb_[k] * pm2);
-                    pm2 = pm1;
-                    pm1 = pk;
-                }
-            pl = pm1;
+        if ((b % SO3_ADJ_TB) == 0) {  // uniform: the next 16 images' tables
+            __syncthreads();          // the previous tables have been read
+            const int bb = b + lane;
+            if (lane < SO3_ADJ_TB && bb < nb)
+                so3_ylm_tables<SO3_ADJ_LT>(a.lmax + 1, stg[0][bb], stg[1][bb], stg[2][bb], stg[3][bb], Pt[lane],
+                                           Et[lane]);
+            __syncthreads();
         }
-        const cd y = cmake(msg * pl * em.re, imsg * msg * pl * em.im);
+        // Y_lm of every lane (rows 0..lmax+1)
+        const int bt = b % SO3_ADJ_TB;
+        const double pl = ylane ? Pt[bt][pidx] : 0.0;
+        const cd em = Et[bt][ylane ? am : 0];
+        const cd y = cmake(msg * pl * em.re, imsg * pl * em.im);
         ysh[lane] = y;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1433,10 +1445,10 @@ static int so3_ctot_impl(const pxff_so3_params *p, int64_t natoms, const int64_t
                 // that its batches of 64 qualifying images are full
                 if (d_img_rev && d_work && cap > 0) {
                     // pair images sorted into |r| bins, then one image per lane
-                    so3_rbin_hist_kernel<<<SO3_RBLK, 256, 0, s>>>(natoms, d_img_start, d_img_r, d_img_rev, p->rcut,
+                    so3_rbin_hist_kernel<<<SO3_RBLK, SO3_RBT, 0, s>>>(natoms, d_img_start, d_img_r, d_img_rev, p->rcut,
                                                                   d_work);
                     so3_rbin_scan_kernel<<<1, SO3_RSCAN_T, 0, s>>>(d_work);
-                    so3_rbin_fill_kernel<<<SO3_RBLK, 256, 0, s>>>(natoms, d_img_start, d_img_r, d_img_rev, p->rcut,
+                    so3_rbin_fill_kernel<<<SO3_RBLK, SO3_RBT, 0, s>>>(natoms, d_img_start, d_img_r, d_img_rev, p->rcut,
                                                                   d_work);
                     SO3_DISPATCH_NMAX(p->nmax, if constexpr (NM <= 6) {
                         so3_radial_sorted_kernel<NM, 7><<<div_up(cap, 256), 256, 0, s>>>(a, d_work, d_img_r,

@@ -16,7 +16,7 @@ import torch
 from . import _native as nat
 
 CUTOFF_IDS = {"cosine": 1, "tanh": 2, "poly1": 3, "poly2": 4, "poly3": 5, "poly4": 6, "exp": 7, "unity": 8}
-RWORK = 256 * 256 + 1  # |r| bins x chunks + count: the fixed part of pxff_so3_ctot_pairs' workspace
+RWORK = 256 * 64 + 1  # |r| bins x chunks + count: the fixed part of pxff_so3_ctot_pairs' workspace
 
 
 def W(nmax):
