@@ -118,7 +118,8 @@ PLAIN_KERNELS = ("dbdr", "rows", "dxdr_contract")
 KERNEL_SYMBOL = {"ui": "so4_accum_kernel", "z": "so4_z_tile_kernel", "bi": "so4_b_kernel",
                  "zb": "so4_z_tile_kernel+so4_brow_sum_kernel",
                  "network": "mlp_energy_grad_kernel", "yi": "so4_yi_kernel",
-                 "deidrj": "so4_deidrj_kernel", "gather": "so4_gather_kernel", "dbdr": "so4_dbdr_centre_kernel",
+                 "deidrj": "so4_deidrj_kernel", "gather": "so4_gather_kernel",
+                 "dbdr": "so4_dbdr_centre_kernel", "dbdr_atom_major_z": "so4_z_atom_major_kernel+so4_dbdr_centre_kernel",
                  "rows": "so4_rows_kernel", "dxdr_contract": "dxdr_contract_kernel"}
 
 
@@ -152,7 +153,10 @@ def isolated_kernel_ms(dso4, potential, model, species_atom, nl, cfg, w, reps=5)
     out["deidrj"] = t(lambda: dso4.deidrj(nl, cfg, w, y))
     out["gather"] = t(lambda: dso4.gather(nl, dedr))
     db = dso4.dbdr(nl, cfg, w, z)
+    # the plain path's dB/dr (per-centre kernel over the tile-major Z), and
+    # the alternative over an atom-major copy of Z (copy included)
     out["dbdr"] = t(lambda: dso4.dbdr(nl, cfg, w, z))
+    out["dbdr_atom_major_z"] = t(lambda: dso4.dbdr(nl, cfg, w, z, atom_major=True))
     dx, _ = dso4.rows(nl, cfg.nb, db, stress=False)
     out["rows"] = t(lambda: dso4.rows(nl, cfg.nb, db, stress=False))
     out["dxdr_contract"] = t(lambda: dso4.dxdr_contract(nl, cfg.nb, beta, dx))
@@ -344,9 +348,22 @@ def so3_flops(nmax, lmax):
     return dict(radial=2 * 2 * nmax * nq * (lmax + 1), ctot=8 * nlm, image_adjoint=3 * (8 + 4) * nlm)
 
 
-SO3_KERNELS = {"expansion": "so3_radial_all_kernel+so3_accum_kernel", "power": "so3_power_kernel",
-               "network": "mlp_energy_grad_kernel", "adjoint_a": "so3_adj_a_kernel",
-               "deidrj": "so3_image_kernel", "gather": "so4_gather_kernel"}
+SO3_KERNELS = {"expansion": "so3_rbin_hist_kernel+so3_rbin_scan_kernel+so3_rbin_fill_kernel+"
+                            "so3_radial_sorted_kernel+so3_ctot_lm_kernel",
+               "power": "so3_power_kernel", "network": "mlp_mfma_kernel", "adjoint_a": "so3_adj_a_kernel",
+               "deidrj": "so3_deidrj_lm_kernel", "forces_energy": "so4_gather_kernel"}
+
+
+def committed_traffic_sum(symbols):
+    """Sum of committed_traffic over '+'-joined kernel names (None if any is
+    missing)."""
+    tot = 0
+    for k in symbols.split("+"):
+        v = committed_traffic(k)
+        if v is None:
+            return None
+        tot += v
+    return tot
 
 
 def run_so3_sio2(steps=5):
@@ -435,6 +452,7 @@ def run_so3_sio2(steps=5):
             "force_rel_diff_autograd_vs_fused": ferr, "kernel_ms_isolated": kt,
             "roofline": {"bound": "valu", "kernel": SO3_KERNELS[dom], "achieved": ach / 1e12, "peak": PEAK_FP64 / 1e12,
                          "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "flops_per_launch": alg[dom],
+                         "traffic": committed_traffic_sum(SO3_KERNELS[dom]),
                          "kernel_ms_isolated": kt[dom],
                          "note": "algorithmic flops from the reference loop bounds (bench.so3_flops); the radial "
                                  "kernel evaluates each image pair once (an image and its reverse share |r|), so "

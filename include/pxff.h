@@ -249,14 +249,16 @@ int pxff_so4_z_atom_major(const pxff_so4_params *p, int64_t natoms, const double
 int pxff_so4_dbdr(const pxff_so4_params *p, int64_t nimg, const int64_t *d_img_atom, const int64_t *d_img_j,
                   const double *d_img_r, const double *d_atom_w, const double *d_z, double *d_dbdr, void *stream);
 
-/* Plain dB/dr per image [nimg*nb*3] from the tile-major Z of pxff_so4_zb
- * (no atom-major copy): the same values as pxff_so4_dbdr, one workgroup per
- * centre over its images img_start[c] .. img_start[c+1]-1 (the image count
- * is read on the device).  Replaces the dblist part of compute_dbidrj /
+/* Plain dB/dr per image [nimg*nb*3]: the same values as pxff_so4_dbdr, one
+ * workgroup per centre over its images img_start[c] .. img_start[c+1]-1 (the
+ * image count is read on the device).  d_z: the tile-major Z of
+ * pxff_so4_zb (z_atom_major = 0) or its atom-major copy from
+ * pxff_so4_z_atom_major (z_atom_major = 1; contiguous block rows, far fewer
+ * cache lines per centre).  Replaces the dblist part of compute_dbidrj /
  * get_bispectrum_components (SNAP.py:1143-1257, 1428-1548). */
 int pxff_so4_dbdr_centres(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start,
                           const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w, const double *d_z,
-                          double *d_dbdr, void *stream);
+                          int z_atom_major, double *d_dbdr, void *stream);
 
 /* Plain path, step 2: reference layout rows.  dxdr [nrows*nb*3] (rows (i,j)
  * = sum over images, row (i,i) = -sum of the others), and, if d_rdxdr is not

@@ -1167,7 +1167,7 @@ __global__ __launch_bounds__(64) void so4_dbdr_centre_kernel(
     const double *__restrict__ img_r, const double *__restrict__ atom_w, const double *__restrict__ zt, int nz,
     int nb, const double *rootpq_g, const double *__restrict__ lnorm, const int *__restrict__ blk_level_start,
     const int *__restrict__ blk_zoff, const int *__restrict__ blk_tb, const double *__restrict__ blk_tf, double rcut,
-    double rfac0, int cutoff_id, double *__restrict__ dbdr) {
+    double rfac0, int cutoff_id, int z_atom_major, double *__restrict__ dbdr) {
     constexpr int G = TWOJ / 2 + 1;
     constexpr int NG = dbdr_centre_ng<TWOJ>();
     constexpr int NL = (TWOJ / 2 + 1) * (TWOJ + 1);
@@ -1187,7 +1187,11 @@ __global__ __launch_bounds__(64) void so4_dbdr_centre_kernel(
     const int g = lane / G, mb = lane % G;
     for (int k = lane; k < (TWOJ + 1) * (TWOJ + 1); k += 64) rpq[k] = rootpq_g[k];
     const int64_t s0 = img_start[c], s1 = img_start[c + 1];
-    const cd *zc = reinterpret_cast<const cd *>(zt) + (size_t)(c >> 6) * nz * 64 + (c & 63);
+    // the centre's Z entries: stride 64 in the tile-major layout, 1 in the
+    // atom-major copy
+    const cd *zc = reinterpret_cast<const cd *>(zt) + (z_atom_major ? (size_t)c * nz
+                                                                      : (size_t)(c >> 6) * nz * 64 + (c & 63));
+    const int zs = z_atom_major ? 1 : 64;
     for (int64_t e0 = s0; e0 < s1; e0 += NG) {
         const int ne = (int)(s1 - e0 < NG ? s1 - e0 : NG);
         // the chunk's output rows start at zero
@@ -1247,7 +1251,7 @@ __global__ __launch_bounds__(64) void so4_dbdr_centre_kernel(
                 const int bk = it / nloc, loc = it - bk * nloc;
                 const int row = loc / (l + 1), ma = loc - row * (l + 1);
                 const double wt = 2 * row < l ? 1.0 : (ma < row ? 1.0 : (ma == row ? 0.5 : 0.0));
-                const cd zv = zc[(size_t)(blk_zoff[b0 + bk] + loc) * 64];
+                const cd zv = zc[(size_t)(blk_zoff[b0 + bk] + loc) * zs];
                 Zl[it] = cmake(wt * zv.re, wt * zv.im);
             }
             // also orders the previous level's output additions before this
@@ -1838,7 +1842,7 @@ extern "C" int pxff_so4_dbdr(const pxff_so4_params *p, int64_t nimg, const int64
 
 extern "C" int pxff_so4_dbdr_centres(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start,
                                      const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
-                                     const double *d_z, double *d_dbdr, void *stream) {
+                                     const double *d_z, int z_atom_major, double *d_dbdr, void *stream) {
     int rc = so4_check_params(p);
     if (rc) return rc;
     if (natoms <= 0) return 0;
@@ -1854,7 +1858,7 @@ extern "C" int pxff_so4_dbdr_centres(const pxff_so4_params *p, int64_t natoms, c
     PXFF_DISPATCH_TWOJ(p->twol, (so4_dbdr_centre_kernel<TJ><<<grid, 64, 0, s>>>(
                                     natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_z, T->nz, T->nb, T->rootpq,
                                     T->lnorm, T->blk_level_start, T->blk_zoff, T->blk_tb, T->blk_tf, p->rcut,
-                                    p->rfac0, p->cutoff_id, d_dbdr)));
+                                    p->rfac0, p->cutoff_id, z_atom_major ? 1 : 0, d_dbdr)));
     PXFF_LAUNCH_CHECK();
     return 0;
 }

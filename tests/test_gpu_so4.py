@@ -297,13 +297,15 @@ def test_dbdr_centres_equals_per_image(lmax):
     w = {"Si": 1.0, "O": 0.8}
     nl, cfg, aw, sw, model, species, state = _engine(s, lmax=lmax, weights=w)
     _, z, _ = dso4.forward(nl, cfg, aw, sw)
-    a = dso4.dbdr(nl, cfg, aw, z)
     b = dso4.dbdr_per_image(nl, cfg, aw, z)
     n = nl.nimg * cfg.nb * 3
-    _print("dbdr centres 2J", 2 * lmax, "images", nl.nimg, "rel", rel_err(a[:n].cpu().numpy(), b[:n].cpu().numpy()))
-    assert rel_err(a[:n].cpu().numpy(), b[:n].cpu().numpy()) < 1e-12
-    floor = 1e-9 * float(b[:n].abs().max())
-    assert max_rel_elem(a[:n].cpu().numpy(), b[:n].cpu().numpy(), floor) < 1e-9
+    for atom_major in (True, False):
+        a = dso4.dbdr(nl, cfg, aw, z, atom_major=atom_major)
+        _print("dbdr centres 2J", 2 * lmax, "atom-major Z", atom_major, "images", nl.nimg, "rel",
+               rel_err(a[:n].cpu().numpy(), b[:n].cpu().numpy()))
+        assert rel_err(a[:n].cpu().numpy(), b[:n].cpu().numpy()) < 1e-12
+        floor = 1e-9 * float(b[:n].abs().max())
+        assert max_rel_elem(a[:n].cpu().numpy(), b[:n].cpu().numpy(), floor) < 1e-9
 
 
 def test_adjoint_forces_match_plain_dxdr():
