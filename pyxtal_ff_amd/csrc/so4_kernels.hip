@@ -1454,11 +1454,15 @@ __global__ void rdxdr_reduce_kernel(int64_t natoms, int nb, const int64_t *__res
 // ----------------------------------------------------------- tangent of B
 // dB_b = sum over the (block k, factor f) pairs targeting b of f p_k with
 // p_k = 2 Re sum_loc hw(loc) conj(dU^l[loc]) Z_k[loc] and dU = dU_tot (u_norm
-// already applied by the accumulation kernel).  One workgroup per 64-atom
-// tile, one atom per lane: dU and Z are both tile-major, so every load is a
-// coalesced 1 KB row; the wavefronts split the blocks (p_k into LDS), then
-// the components b.
-__global__ __launch_bounds__(256) void so4_tangent_b_kernel(int64_t natoms, int nhalf, int nz, int nb, int nblk,
+// already applied by the accumulation kernel).  One workgroup of 16
+// wavefronts per 64-atom tile, one atom per lane: dU and Z are both
+// tile-major, so every load is a coalesced 1 KB row; the wavefronts split
+// the blocks (p_k into LDS; four partial sums per block keep four loads and
+// FMA chains in flight), then the components b.  The sixteen wavefronts of
+// a tile keep enough of Z in flight that the kernel runs at the rate Z
+// streams from memory.
+constexpr int TANB_THREADS = 1024;
+__global__ __launch_bounds__(TANB_THREADS) void so4_tangent_b_kernel(int64_t natoms, int nhalf, int nz, int nb, int nblk,
                                                              int ldim, const double *__restrict__ dutile,
                                                              const double *__restrict__ z,
                                                              const double *__restrict__ h_weight,
@@ -1473,21 +1477,28 @@ __global__ __launch_bounds__(256) void so4_tangent_b_kernel(int64_t natoms, int 
     const int64_t tile = blockIdx.x;
     const cd *dut = reinterpret_cast<const cd *>(dutile) + (size_t)tile * nhalf * 64 + lane;
     const cd *zt = reinterpret_cast<const cd *>(z) + (size_t)tile * nz * 64 + lane;
-    for (int k = wv; k < nblk; k += 4) {
+    const int nw = blockDim.x >> 6;
+    for (int k = wv; k < nblk; k += nw) {
         int l = 0;
         while (l + 1 < ldim && k >= blk_level_start[l + 1]) l++;
         const int nloc = (l / 2 + 1) * (l + 1);
         const int z0 = blk_zoff[k], h0 = hoff[l];
-        double p = 0.0;
-#pragma unroll 8
-        for (int t = 0; t < nloc; t++)
-            p += h_weight[h0 + t] * cdotre(dut[(size_t)(h0 + t) * 64], zt[(size_t)(z0 + t) * 64]);
-        pk[k * 64 + lane] = 2.0 * p;
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+        int t = 0;
+        for (; t + 4 <= nloc; t += 4) {
+            p0 += h_weight[h0 + t] * cdotre(dut[(size_t)(h0 + t) * 64], zt[(size_t)(z0 + t) * 64]);
+            p1 += h_weight[h0 + t + 1] * cdotre(dut[(size_t)(h0 + t + 1) * 64], zt[(size_t)(z0 + t + 1) * 64]);
+            p2 += h_weight[h0 + t + 2] * cdotre(dut[(size_t)(h0 + t + 2) * 64], zt[(size_t)(z0 + t + 2) * 64]);
+            p3 += h_weight[h0 + t + 3] * cdotre(dut[(size_t)(h0 + t + 3) * 64], zt[(size_t)(z0 + t + 3) * 64]);
+        }
+        for (; t < nloc; t++)
+            p0 += h_weight[h0 + t] * cdotre(dut[(size_t)(h0 + t) * 64], zt[(size_t)(z0 + t) * 64]);
+        pk[k * 64 + lane] = 2.0 * ((p0 + p1) + (p2 + p3));
     }
     __syncthreads();
     const int64_t i = tile * 64 + lane;
     if (i >= natoms) return;
-    for (int b = wv; b < nb; b += 4) {
+    for (int b = wv; b < nb; b += nw) {
         double acc = 0.0;
 #pragma unroll
         for (int t = 0; t < 3; t++) {
@@ -1723,7 +1734,10 @@ static int so4_tangent_launch(const pxff_so4_params *p, const So4Dev *T, int64_t
     PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, true, MODE><<<grid, 64, 0, s>>>(         \
                                     natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr, d_dr, nullptr, \
                                     T->rootpq, T->lnorm, p->rcut, p->rfac0, p->cutoff_id, 0.0, d_dutot)))
-    const int mode = tune_value(PXFF_TUNE_ACCUM_LDSADD);
+    // the tangent sums in registers whenever they fit (2J <= 8): 0.53 ms
+    // against 0.75 ms for the LDS modes on the 33.7k-atom training batch,
+    // whatever mode the forward's tuning chose
+    const int mode = p->twol <= 8 ? 4 : tune_value(PXFF_TUNE_ACCUM_LDSADD);
     if (mode == 4 && p->twol <= 8) {
         PXFF_DISPATCH_TWOJ(p->twol, if constexpr (TJ <= 8) {
             so4_utot_kernel<TJ, true><<<grid, 64, 0, s>>>(natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr,
@@ -1744,7 +1758,7 @@ static int so4_tangent_launch(const pxff_so4_params *p, const So4Dev *T, int64_t
     if (lds > 65536)
         PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)so4_tangent_b_kernel,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    so4_tangent_b_kernel<<<(unsigned)div_up(natoms, 64), 256, lds, s>>>(
+    so4_tangent_b_kernel<<<(unsigned)div_up(natoms, 64), TANB_THREADS, lds, s>>>(
         natoms, T->nhalf, T->nz, T->nb, T->nblk, p->twol + 1, d_dutot, d_z, T->h_weight, T->hoff,
         T->blk_level_start, T->blk_zoff, T->b_src, T->b_srcf, d_db);
     PXFF_LAUNCH_CHECK();

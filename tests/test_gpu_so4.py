@@ -395,11 +395,13 @@ def test_adjoint_energy_central_fd():
         assert abs(fd - f[m, k]) < 1e-7 * max(1.0, abs(f[m, k]))
 
 
-def test_double_backward_tangent_matches_fd():
+@pytest.mark.parametrize("lmax", [4, 5])
+def test_double_backward_tangent_matches_fd(lmax):
     """The backward's backward (J . dr) equals the central difference of B
-    along a random per-image displacement."""
+    along a random per-image displacement (2J = 8: register sums; 2J = 10:
+    the tuned LDS summation mode)."""
     s = workloads.diamond_si(reps=2, sigma=0.08, seed=3)
-    nl, cfg, aw, sw, model, species, state = _engine(s)
+    nl, cfg, aw, sw, model, species, state = _engine(s, lmax=lmax)
     r0 = nl.img_r[: 3 * nl.nimg].view(-1, 3).clone()
     g = torch.randn(r0.shape, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(0))
     db = dso4.tangent(nl, cfg, aw, dso4.forward(nl, cfg, aw, sw)[1], g, img_r=r0)
@@ -464,8 +466,9 @@ def test_diamond4096_full_parity(oracle_mod):
 def test_accumulation_modes_agree_and_reproduce():
     """The U_tot summation modes (register sums reduced once per centre,
     per-group LDS partials, LDS atomic adds issued by one wavefront,
-    partials shared by two or three groups) agree, for U_tot and for its
-    tangent; each mode is bitwise reproducible run to run (forces too)."""
+    partials shared by two or three groups) agree, for U_tot and the forces;
+    each mode is bitwise reproducible run to run (the tangent too, which
+    sums in registers at 2J <= 8 whatever the mode)."""
     from pyxtal_ff_amd import autotune
     s = workloads.diamond_si(reps=3, sigma=0.08, seed=4)
     nl, cfg, aw, sw, model, species, state = _engine(s, weights={"Si": 1.0})
