@@ -307,9 +307,12 @@ def run_batched(steps=3):
 
 def run_training(steps=10):
     """256 structures of the batched set; per-element 55-30-30-1 tanh MLP;
-    labels from a seeded teacher on the materialised dxdr; loss
-    1 E + 0.1 F + 0.01 S per atom; one step = network + adjoint forces +
-    their backward + Adam, eager and as a captured HIP graph."""
+    labels from a seeded teacher on the materialised dxdr; the reference's
+    calculate_loss (force coefficient 0.1, stress coefficient 0.01); one step
+    = network + forces and stress + their backward + Adam, eager and as a
+    captured HIP graph, with the forces through the adjoint (its tangent in
+    the backward) and through the dxdr / rdxdr materialised once for the
+    fixed structures (the reference's way of training)."""
     import torch
     from pyxtal_ff_amd import batch as bt, potential, training, workloads
     structs = workloads.random_sio_batch(256, seed=0)
@@ -320,9 +323,13 @@ def run_training(steps=10):
     teacher = potential.NNPotential(el, d.nfeat, (30, 30), seed=100).cuda()
     labels = training.teacher_labels(d, teacher, sp, plain=True)
     out = {"structures": b.nstruct, "atoms": b.natoms}
-    for mode in ("eager", "graph"):
+    for mode in ("eager", "graph", "materialized_eager", "materialized_graph"):
         student = potential.NNPotential(el, d.nfeat, (30, 30), seed=5).cuda()
-        tr = training.ForceTrainer(d, student, sp, labels, graph=(mode == "graph"))
+        tr = training.ForceTrainer(d, student, sp, labels, graph=mode.endswith("graph"),
+                                   materialized=mode.startswith("materialized"))
+        if mode.startswith("materialized"):
+            out["dxdr_rows"] = int(tr.dx.shape[0])
+            out["dxdr_bytes"] = int(tr.dx.numel() * 8)
         l0 = float(tr.step())
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -331,6 +338,7 @@ def run_training(steps=10):
         torch.cuda.synchronize()
         out[mode + "_ms_per_step"] = (time.perf_counter() - t0) / steps * 1e3
         out[mode + "_loss_first_last"] = [l0, float(tr.step())]
+        del tr
     return out
 
 

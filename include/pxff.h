@@ -276,6 +276,14 @@ int pxff_dxdr_contract(int64_t natoms, int nb, const int64_t *d_row_start, const
                        const int64_t *d_row_rev, const double *d_beta, const double *d_dxdr, double *d_grad,
                        void *stream);
 
+/* Transpose of pxff_dxdr_contract, for training on a materialised dxdr:
+ * d_gbeta [natoms*nb], gbeta[i,b] = sum over centre i's rows (i,m) of
+ * dxdr[row,b,:] . g[m,:] for d_g [natoms*3] -- the derivative of the force
+ * contraction with respect to beta = dE/dx that the reference's loss
+ * backpropagates (neuralnetwork.py:484-497, 499-545). */
+int pxff_dxdr_contract_t(int64_t natoms, int nb, const int64_t *d_row_start, const int64_t *d_seq, const double *d_g,
+                         const double *d_dxdr, double *d_gbeta, void *stream);
+
 /* Per-atom rdxdr reduction (utilities/__init__.py:226-235): out [natoms*nb*6]
  * Voigt (xx, yy, zz, xy, xz, yz) of the sum of the centre's rows. */
 int pxff_rdxdr_reduce(int64_t natoms, int nb, const int64_t *d_row_start, const double *d_rdxdr, double *d_out,

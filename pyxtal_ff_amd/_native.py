@@ -64,6 +64,7 @@ _SIGS = {
     "pxff_so4_dbdr": ([P, I64, P, P, P, P, P, P, P], I32),
     "pxff_so4_dbdr_centres": ([P, I64, P, P, P, P, P, I32, P, P], I32),
     "pxff_so4_rows": ([I64, ctypes.c_int, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "pxff_dxdr_contract_t": ([I64, I32, P, P, P, P, P, P], I32),
     "pxff_dxdr_contract": ([I64, ctypes.c_int, P, P, P, P, P, P, P], I32),
     "pxff_rdxdr_reduce": ([I64, ctypes.c_int, P, P, P, P], I32),
     "pxff_so3_ctot": ([P, I64, P, P, P, P, P, P, P, P, P, P, P], I32),

@@ -1376,6 +1376,43 @@ __global__ __launch_bounds__(64) void dxdr_contract_kernel(int64_t natoms, int n
     }
 }
 
+// Transpose of the plain force contraction: for a gradient g [N, 3] with
+// respect to dE/dR, grad_beta[i, b] = sum over rows (i, m) of
+// dxdr[row, b, :] . g[m, :] -- the backward of dxdr_contract_kernel (the
+// reference's loss differentiates its dxdr contraction this way,
+// neuralnetwork.py:484-497).  One wavefront per centre over its own rows
+// (no reverse index), lanes over b, four rows in flight.
+__global__ __launch_bounds__(64) void dxdr_contract_t_kernel(int64_t natoms, int nb,
+                                                              const int64_t *__restrict__ row_start,
+                                                              const int64_t *__restrict__ seq,
+                                                              const double *__restrict__ g,
+                                                              const double *__restrict__ dxdr,
+                                                              double *__restrict__ gbeta) {
+    const int64_t i = blockIdx.x;
+    if (i >= natoms) return;
+    const int lane = threadIdx.x;
+    const int64_t r0 = row_start[i], r1 = row_start[i + 1];
+    for (int b = lane; b < nb; b += 64) {
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        int64_t rw = r0;
+        for (; rw + 4 <= r1; rw += 4) {
+            const double *d = dxdr + (size_t)rw * nb * 3 + 3 * b;
+            const int64_t m0 = seq[2 * rw + 1], m1 = seq[2 * rw + 3], m2 = seq[2 * rw + 5], m3 = seq[2 * rw + 7];
+            const size_t st = (size_t)nb * 3;
+            a0 += d[0] * g[3 * m0] + d[1] * g[3 * m0 + 1] + d[2] * g[3 * m0 + 2];
+            a1 += d[st] * g[3 * m1] + d[st + 1] * g[3 * m1 + 1] + d[st + 2] * g[3 * m1 + 2];
+            a2 += d[2 * st] * g[3 * m2] + d[2 * st + 1] * g[3 * m2 + 1] + d[2 * st + 2] * g[3 * m2 + 2];
+            a3 += d[3 * st] * g[3 * m3] + d[3 * st + 1] * g[3 * m3 + 1] + d[3 * st + 2] * g[3 * m3 + 2];
+        }
+        for (; rw < r1; rw++) {
+            const double *d = dxdr + (size_t)rw * nb * 3 + 3 * b;
+            const int64_t m = seq[2 * rw + 1];
+            a0 += d[0] * g[3 * m] + d[1] * g[3 * m + 1] + d[2] * g[3 * m + 2];
+        }
+        gbeta[(size_t)i * nb + b] = (a0 + a1) + (a2 + a3);
+    }
+}
+
 // Adjoint path: dE/dR_i = sum_{e of i} (dE/dr_rev(e) - dE/dr_e); virial of
 // centre i = sum_e r_e (x) dE/dr_e.  One wavefront per atom (4 per
 // workgroup), lanes over the centre's images, then a butterfly reduction
@@ -1886,6 +1923,17 @@ extern "C" int pxff_so4_rows(int64_t natoms, int nb, const int64_t *d_row_start,
     hipStream_t s = (hipStream_t)stream;
     so4_rows_kernel<<<(int)natoms, 256, 0, s>>>(natoms, nb, d_row_start, d_row_img, d_seq, d_nsite, d_pos, d_img_r,
                                                  d_dbdr, d_inv_vol, d_dxdr, d_rdxdr);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_dxdr_contract_t(int64_t natoms, int nb, const int64_t *d_row_start, const int64_t *d_seq,
+                                    const double *d_g, const double *d_dxdr, double *d_gbeta, void *stream) {
+    if (natoms <= 0) return 0;
+    PXFF_REQUIRE(nb > 0, "nb must be > 0");
+    PXFF_REQUIRE(natoms <= 0x7fffffff, "too many atoms for one launch");
+    dxdr_contract_t_kernel<<<(unsigned)natoms, 64, 0, (hipStream_t)stream>>>(natoms, nb, d_row_start, d_seq, d_g,
+                                                                            d_dxdr, d_gbeta);
     PXFF_LAUNCH_CHECK();
     return 0;
 }

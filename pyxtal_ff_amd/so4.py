@@ -255,6 +255,17 @@ def dxdr_contract(nl, nb, beta, dxdr):
     return grad[: nl.natoms * 3].view(nl.natoms, 3)
 
 
+def dxdr_contract_t(nl, nb, g, dxdr):
+    """grad_beta [N, nb] = sum over centre i's rows (i, m) of dxdr[(i, m)] . g[m]
+    (the transpose of dxdr_contract)."""
+    p = nat.ptr
+    out = _f64(nl.natoms * nb, nl.pos.device)
+    g = g.contiguous()
+    nat.check(nat.lib().pxff_dxdr_contract_t(nl.natoms, nb, p(nl.row_start), p(nl.seq), p(g), p(dxdr), p(out),
+                                             nat.stream()), "pxff_dxdr_contract_t")
+    return out[: nl.natoms * nb].view(nl.natoms, nb)
+
+
 def rdxdr_reduce(nl, nb, rdxdr):
     """Per-atom Voigt rdxdr [N, nb, 6] (utilities/__init__.py:226-235)."""
     p = nat.ptr

@@ -1,5 +1,5 @@
-"""Force-matched training through the adjoint, with the step captured in a
-HIP graph.
+"""Force-matched training through the adjoint or on a materialised dxdr,
+with the step captured in a HIP graph.
 
 The loss is the reference's (``pyxtal_ff/models/neuralnetwork.py:466-545``,
 ``calculate_loss``), see ``reference_loss``.  The force term is
@@ -9,6 +9,14 @@ are fixed during training, so the neighbour list, Z (or the SO3 ctot) and
 the descriptors are computed once; each step runs the network, the adjoint
 sweep (dE/dr per image, recomputed every step), its tangent, and the
 optimiser.
+
+With ``materialized=True`` the trainer instead does what the reference's
+training does with its database records: dxdr (reference row layout) and
+the per-atom rdxdr are computed once for the fixed structures and kept in
+HBM, forces are the contraction of dE/dx with dxdr (``pxff_dxdr_contract``)
+and its derivative with respect to dE/dx is the transposed contraction
+(``pxff_dxdr_contract_t``); each step then streams dxdr twice instead of
+re-running the adjoint sweep and its tangent.
 
 Per-structure sums (energies, force errors, virials) are segment sums over
 the contiguous atoms / images of each structure (``pxff_segment_sum``:
@@ -92,6 +100,23 @@ def teacher_labels(desc, teacher, species, plain=True):
     return o["energy"].detach(), o["forces"].detach(), (stress_voigt(o["virial"], b) * EV2GPA).detach()
 
 
+class _DxdrForce(torch.autograd.Function):
+    """dE/dR [N, 3] = sum over rows (i, m) of beta_i . dxdr[(i, m)] (linear
+    in beta; its backward is the transposed contraction)."""
+
+    @staticmethod
+    def forward(ctx, beta, trainer):
+        from . import so4 as dso4
+        ctx.trainer = trainer
+        return dso4.dxdr_contract(trainer.batch.nl, trainer.nf, beta, trainer.dx)
+
+    @staticmethod
+    def backward(ctx, g):
+        from . import so4 as dso4
+        t = ctx.trainer
+        return dso4.dxdr_contract_t(t.batch.nl, t.nf, g, t.dx), None
+
+
 def weight_penalty(model, alpha):
     """alpha * sum of the squares of every parameter whose name contains
     'weight' (neuralnetwork.py:533-540), or None without alpha."""
@@ -109,10 +134,11 @@ class ForceTrainer:
     2 x atoms, stress in GPa over 2 x stress components, the per-structure
     weights ``sf`` and the alpha ||w||^2 penalty.  Energies come from the
     network on fixed descriptors, forces and the virial stress from the
-    adjoint.  labels = (E [S] eV, F [N, 3] eV/A, S [S, 6] GPa)."""
+    adjoint, or (``materialized``) from the dxdr and per-atom rdxdr computed
+    once and kept in HBM.  labels = (E [S] eV, F [N, 3] eV/A, S [S, 6] GPa)."""
 
     def __init__(self, desc, model, species, labels, force_coefficient=0.1, stress_coefficient=0.01, alpha=None,
-                 sf=None, stress_mask=None, lr=1e-3, graph=False, capturable=None):
+                 sf=None, stress_mask=None, lr=1e-3, graph=False, capturable=None, materialized=False):
         self.desc, self.model, self.species = desc, model, species
         b = desc.batch
         self.batch = b
@@ -137,6 +163,21 @@ class ForceTrainer:
             from . import so3 as dso3
             # fixed structures: the radial integrals are computed once too
             _, self.ct, self.rad = dso3.forward(nl, desc.cfg, desc.state.atom_z, radial=True)
+        self.materialized = bool(materialized)
+        if self.materialized:
+            from . import so4 as dso4
+            with torch.no_grad():
+                if hasattr(desc, "atom_w"):
+                    self.nf = desc.cfg.nb
+                    dx, rd = dso4.rows(nl, self.nf, dso4.dbdr(nl, desc.cfg, desc.atom_w, self.z), stress=True)
+                else:
+                    from . import so3 as dso3
+                    self.nf = desc.cfg.ncoef
+                    dx, rd = dso4.rows(nl, self.nf, dso3.dpdr(nl, desc.cfg, desc.state.atom_z, self.ct,
+                                                              radial=self.rad), stress=True)
+                self.dx = dx
+                self.rv = dso4.rdxdr_reduce(nl, self.nf, rd)  # [N, nf, 6], Voigt
+                del rd
         self.n_struct = torch.as_tensor(b.counts, dtype=torch.float64, device=dev)
         self._voigt = torch.as_tensor([3 * i + j for i, j in zip(*VOIGT)], dtype=torch.int64, device=dev)
         # Adam keeps its step count on the device when the step is captured;
@@ -162,6 +203,11 @@ class ForceTrainer:
         e_atom = self.model.atom_energies(x, self.species)
         # per-structure buckets first: no multi-block sum over atoms (graph capture)
         (beta,) = torch.autograd.grad(b.per_structure(e_atom).sum(), [x], create_graph=True)
+        if self.materialized:
+            # neuralnetwork.py:484-497: F = -dE/dx . dxdr, S = -dE/dx . rdxdr (GPa)
+            forces = -_DxdrForce.apply(beta.contiguous(), self)
+            s_atom = -(beta.unsqueeze(2) * self.rv).sum(1)
+            return b.per_structure(e_atom), forces, b.per_structure(s_atom) * EV2GPA
         dedr = self._dedr(beta.contiguous())
         forces = -adjoint._Gather.apply(dedr, nl)
         w = self.r.unsqueeze(2) * dedr.unsqueeze(1)  # r_e (x) dE/dr_e

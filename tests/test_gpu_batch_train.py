@@ -59,7 +59,7 @@ def test_batched_so3_matches_plain():
     assert rel_err(bt.stress_voigt(o["virial"], b).detach().cpu().numpy() * training.EV2GPA, S.cpu().numpy()) < 1e-9
 
 
-def _trainer(structs, graph=False, seed_student=5, capturable=None):
+def _trainer(structs, graph=False, seed_student=5, capturable=None, materialized=False):
     b = bt.StructureBatch(structs, 4.9)
     d = bt.BatchedSNAP(b, W, lmax=4, rcut=4.9)
     el = ["O", "Si"]
@@ -68,7 +68,7 @@ def _trainer(structs, graph=False, seed_student=5, capturable=None):
     labels = training.teacher_labels(d, teacher, sp, plain=True)
     student = potential.NNPotential(el, d.nfeat, (30, 30), seed=seed_student).cuda()
     tr = training.ForceTrainer(d, student, sp, labels, force_coefficient=0.1, stress_coefficient=0.01, lr=1e-3,
-                               graph=graph, capturable=capturable)
+                               graph=graph, capturable=capturable, materialized=materialized)
     return tr, b, d, sp
 
 
@@ -137,3 +137,35 @@ def test_graph_training_replays_match_eager():
     for k in range(4):
         assert abs(lg[k] - le[k + 2]) <= 1e-9 * abs(le[k + 2]), (k, lg, le)
     assert le[-1] < le[0]
+
+
+def test_materialized_training_equals_adjoint():
+    """Training on the materialised dxdr / per-atom rdxdr (the reference's
+    way) gives the adjoint trainer's loss, forces, stress and parameter
+    gradients; its captured step replays its eager steps."""
+    structs = small_batch(4, seed=8, nmax=30)
+    tr_a, _, _, _ = _trainer(structs)
+    tr_m, _, _, _ = _trainer(structs, materialized=True)
+    Ea, Fa, Sa = tr_a.predict()
+    Em, Fm, Sm = tr_m.predict()
+    assert rel_err(Em.detach().cpu().numpy(), Ea.detach().cpu().numpy()) < 1e-13
+    assert rel_err(Fm.detach().cpu().numpy(), Fa.detach().cpu().numpy()) < 1e-10
+    assert rel_err(Sm.detach().cpu().numpy(), Sa.detach().cpu().numpy()) < 1e-10
+    grads = []
+    for tr in (tr_a, tr_m):
+        tr.opt.zero_grad()
+        loss, _ = tr.loss()
+        loss.backward()
+        grads.append([p.grad.clone() for p in tr.model.parameters()])
+    for ga, gm in zip(*grads):
+        assert rel_err(gm.cpu().numpy(), ga.cpu().numpy()) < 1e-9
+    tr_e, _, _, _ = _trainer(structs, graph=False, capturable=True, materialized=True)
+    tr_g, _, _, _ = _trainer(structs, graph=True, materialized=True)
+    le = [float(tr_e.step()) for _ in range(5)]
+    lg = []
+    for _ in range(3):
+        lg.append(float(tr_g.step()))
+        torch.cuda.synchronize()
+        _ = (torch.ones(1000, device="cuda") * 2).sum().item()
+    for k in range(3):
+        assert abs(lg[k] - le[k + 2]) <= 1e-9 * abs(le[k + 2]), (k, lg, le)
