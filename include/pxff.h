@@ -355,11 +355,26 @@ int pxff_so3_ctot(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img
  * pairs are first ordered by |r| (a counting sort into 256 bins, over 64
  * chunks of the images) so that the images a wavefront evaluates together
  * take the same branches; the values do not depend on it.  d_work = NULL:
- * the unsorted order. */
+ * the unsorted order.  d_table (non-NULL, with d_work): the integrals from
+ * the table of pxff_so3_radial_table instead of the quadrature. */
 int pxff_so3_ctot_pairs(const pxff_so3_params *p, int64_t natoms, const int64_t *d_img_start,
                         const int64_t *d_img_j, const double *d_img_r, const int64_t *d_img_rev,
                         const int64_t *d_atom_z, const double *d_G, const double *d_rq, const double *d_lnorm,
-                        double *d_out, double *d_radial, int64_t *d_work, int64_t cap, void *stream);
+                        double *d_out, double *d_radial, int64_t *d_work, int64_t cap, const double *d_table,
+                        void *stream);
+
+/* Tabulated radial integrals for pxff_so3_ctot_pairs (d_table; nmax, lmax
+ * <= 6): I_nl(R) and dI_nl/dR on nk equal intervals of [0, rc] as local
+ * polynomials of degree 11, built on the device from the same quadrature at
+ * each interval's Chebyshev nodes (to about 1e-14 of each function's
+ * largest value; nk from the steepness 2 alpha rc^2, at least 256).  The
+ * size call gives nk and the table length in doubles; the build needs a
+ * scratch of the same length.  A table serves every call with the same
+ * parameters (it depends on nmax, lmax, rcut, alpha, not on the cutoff
+ * function or weights, which the store does not hold). */
+int pxff_so3_radial_table_size(const pxff_so3_params *p, int *nk, int64_t *len);
+int pxff_so3_radial_table(const pxff_so3_params *p, const double *d_G, const double *d_rq, const double *d_lnorm,
+                          double *d_scratch, double *d_table, void *stream);
 
 /* Power spectrum x [natoms*ncoef]; with d_dctot non-NULL the tangent of x. */
 int pxff_so3_power(const pxff_so3_params *p, int64_t natoms, const double *d_ctot, const double *d_dctot,

@@ -68,7 +68,9 @@ _SIGS = {
     "pxff_dxdr_contract": ([I64, ctypes.c_int, P, P, P, P, P, P, P], I32),
     "pxff_rdxdr_reduce": ([I64, ctypes.c_int, P, P, P, P], I32),
     "pxff_so3_ctot": ([P, I64, P, P, P, P, P, P, P, P, P, P, P], I32),
-    "pxff_so3_ctot_pairs": ([P, I64, P, P, P, P, P, P, P, P, P, P, P, I64, P], I32),
+    "pxff_so3_ctot_pairs": ([P, I64, P, P, P, P, P, P, P, P, P, P, P, I64, P, P], I32),
+    "pxff_so3_radial_table_size": ([P, P, P], I32),
+    "pxff_so3_radial_table": ([P, P, P, P, P, P, P], I32),
     "pxff_so3_power": ([P, I64, P, P, P, P], I32),
     "pxff_so3_adjoint_a": ([P, I64, P, P, P, P], I32),
     "pxff_so3_deidrj": ([P, I64, P, P, P, P, P, P, P, P, P, P, P], I32),

@@ -427,14 +427,13 @@ __device__ inline void sph_bessel_i_all(double x, double ix, double *f) {
 // bitwise the same |r|), only images with rev(e) >= e are evaluated and
 // their integrals are written to both rows: each wavefront queues the
 // images it owns that qualify in LDS and runs them 64 at a time.
+// I_nl(R), dI_nl/dR for l < LC, n < NMAX by the quadrature (one evaluation
+// of i_0..i_{LC-1} per node).
 template <int NMAX, int LC>
-__device__ inline void so3_radial_image(const So3Args &a, int64_t e, int64_t re, const double *__restrict__ img_r,
-                                        double *__restrict__ rad) {
+__device__ inline void so3_radial_quadrature(const So3Args &a, double R, double (&I)[LC][NMAX],
+                                             double (&dI)[LC][NMAX]) {
     const int L = a.lmax + 1;
     const double ta = 2.0 * a.alpha;
-    const double x0 = img_r[3 * e], y0 = img_r[3 * e + 1], z0 = img_r[3 * e + 2];
-    const double R = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
-    double I[LC][NMAX], dI[LC][NMAX];
 #pragma unroll
     for (int l = 0; l < LC; l++)
 #pragma unroll
@@ -466,6 +465,16 @@ __device__ inline void so3_radial_image(const So3Args &a, int64_t e, int64_t re,
             }
         }
     }
+}
+
+template <int NMAX, int LC>
+__device__ inline void so3_radial_image(const So3Args &a, int64_t e, int64_t re, const double *__restrict__ img_r,
+                                        double *__restrict__ rad) {
+    const int L = a.lmax + 1;
+    const double x0 = img_r[3 * e], y0 = img_r[3 * e + 1], z0 = img_r[3 * e + 2];
+    const double R = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+    double I[LC][NMAX], dI[LC][NMAX];
+    so3_radial_quadrature<NMAX, LC>(a, R, I, dI);
 #pragma unroll
     for (int w = 0; w < 2; w++) {
         if (w == 1 && re == e) break;
@@ -630,19 +639,227 @@ __global__ __launch_bounds__(SO3_RBT) void so3_rbin_fill_kernel(int64_t natoms, 
         if (img_rev[e] >= e) order[atomicAdd(&cur[so3_rbin(img_r, e, rc)], 1ull)] = e;
 }
 
-// Radial integrals of the images in |r| order (see above): one image per
-// lane, flat over the sorted images; writes both rows of each pair.
+// ------------------------------------------------ tabulated radial integrals
+// For a fixed parameter set I_nl(R) and dI_nl/dR are smooth functions of
+// |r| alone on [0, rc].  They are tabulated once, piecewise on NK equal
+// intervals, as polynomials of degree SO3_TDEG in the local variable
+// t = 2 (R/h - k) - 1 in [-1, 1]: the quadrature above is evaluated at the
+// SO3_TDEG + 1 Chebyshev nodes of every interval (so3_rtab_nodes_kernel),
+// and a fixed linear map (the Chebyshev interpolant at those nodes, written
+// in powers of t; so3_rtab_weights) turns the values into coefficients
+// (so3_rtab_coef_kernel).  With NK chosen from the steepness 2 alpha rc^2
+// (>= 256 intervals; 2.5 per unit of it, rounded up to a power of two) the
+// interpolant matches the quadrature to about 1e-14 of each function's
+// largest value (checked against the quadrature with SciPy's spherical_in).
+// Table layout: [NK][(lmax+1) * 2 * nmax][SO3_TDEG + 1], the functions in
+// the radial store's order (l, {I, dI}, n).
+constexpr int SO3_TDEG = 11;
+constexpr int SO3_TNP = SO3_TDEG + 1;
+
+struct So3TabW {
+    double w[SO3_TNP][SO3_TNP];  // coefficient p of t^p = sum_j w[p][j] f(t_j)
+};
+
+// t_j = cos(pi (j + 1/2) / NP) and the map from node values to monomial
+// coefficients of the interpolant (Chebyshev coefficients c_i = (2 - d_i0) /
+// NP sum_j T_i(t_j) f_j, then T_i written in powers of t).
+static So3TabW so3_rtab_weights() {
+    So3TabW W{};
+    double tn[SO3_TNP], T[SO3_TNP][SO3_TNP], M[SO3_TNP][SO3_TNP];  // T[i][j] = T_i(t_j); M[i][p]: t^p in T_i
+    for (int j = 0; j < SO3_TNP; j++) tn[j] = std::cos(M_PI * (j + 0.5) / SO3_TNP);
+    for (int j = 0; j < SO3_TNP; j++) {
+        T[0][j] = 1.0;
+        T[1][j] = tn[j];
+        for (int i = 2; i < SO3_TNP; i++) T[i][j] = 2.0 * tn[j] * T[i - 1][j] - T[i - 2][j];
+    }
+    for (int i = 0; i < SO3_TNP; i++)
+        for (int q = 0; q < SO3_TNP; q++) M[i][q] = 0.0;
+    M[0][0] = 1.0;
+    M[1][1] = 1.0;
+    for (int i = 2; i < SO3_TNP; i++)
+        for (int q = 0; q < SO3_TNP; q++) M[i][q] = (q > 0 ? 2.0 * M[i - 1][q - 1] : 0.0) - M[i - 2][q];
+    for (int q = 0; q < SO3_TNP; q++)
+        for (int j = 0; j < SO3_TNP; j++) {
+            double acc = 0.0;
+            for (int i = 0; i < SO3_TNP; i++) acc += M[i][q] * (i == 0 ? 1.0 : 2.0) / SO3_TNP * T[i][j];
+            W.w[q][j] = acc;
+        }
+    return W;
+}
+
+static int so3_rtab_nk(const pxff_so3_params *p) {
+    const double steep = 2.0 * p->alpha * p->rcut * p->rcut;
+    int nk = 256;
+    while (nk < 4096 && nk < 2.5 * steep) nk *= 2;
+    return nk;
+}
+
+// node values: vals[(k * NP + j) * nf + f], one thread per (interval, node)
 template <int NMAX, int LC>
-__global__ __launch_bounds__(256) void so3_radial_sorted_kernel(So3Args a, const int64_t *__restrict__ work,
-                                                                const double *__restrict__ img_r,
-                                                                const int64_t *__restrict__ img_rev,
-                                                                double *__restrict__ rad) {
+__global__ __launch_bounds__(256) void so3_rtab_nodes_kernel(So3Args a, int nk, double h, double *__restrict__ vals) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk * SO3_TNP) return;
+    const int k = t / SO3_TNP, j = t - k * SO3_TNP;
+    const double tj = cos(M_PI * (j + 0.5) / SO3_TNP);
+    const double R = (k + 0.5 + 0.5 * tj) * h;
+    double I[LC][NMAX], dI[LC][NMAX];
+    so3_radial_quadrature<NMAX, LC>(a, R, I, dI);
+    const int L = a.lmax + 1, nf = L * 2 * NMAX;
+    double *o = vals + (size_t)t * nf;
+#pragma unroll
+    for (int l = 0; l < LC; l++) {
+        if (l >= L) continue;
+#pragma unroll
+        for (int n = 0; n < NMAX; n++) {
+            o[(l * 2) * NMAX + n] = I[l][n];
+            o[(l * 2 + 1) * NMAX + n] = dI[l][n];
+        }
+    }
+}
+
+// coefficients: tab[(k * nf + f) * NP + p], one thread per (interval, function)
+__global__ __launch_bounds__(256) void so3_rtab_coef_kernel(So3TabW W, int nk, int nf, const double *__restrict__ vals,
+                                                            double *__restrict__ tab) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk * nf) return;
+    const int k = t / nf, f = t - k * nf;
+    double v[SO3_TNP];
+#pragma unroll
+    for (int j = 0; j < SO3_TNP; j++) v[j] = vals[((size_t)k * SO3_TNP + j) * nf + f];
+#pragma unroll
+    for (int q = 0; q < SO3_TNP; q++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < SO3_TNP; j++) acc = fma(W.w[q][j], v[j], acc);
+        tab[(size_t)t * SO3_TNP + q] = acc;
+    }
+}
+
+// A wave-uniform copy of lane r's 64-bit value (r uniform).
+__device__ inline int64_t so3_readlane64(int64_t x, int r) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, r);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)x >> 32), r);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// The radial store from the table, image pairs in |r| order (as
+// so3_radial_sorted_kernel), one wavefront per 64 pairs.  Lane j of the
+// wavefront owns the functions 2j and 2j + 1 of the row and the wavefront
+// walks its pairs one by one: each pair's row then goes out as one 16-byte
+// store per lane, contiguous over the row, to both images of the pair (the
+// rows are nf <= 84 doubles; a lane per pair writing its own row scatters
+// every store instruction over 64 rows, and those stores bounded the
+// kernel).  The coefficients of the lane's two functions stay in registers
+// while consecutive pairs share an interval, the common case in |r| order.
+template <int NMAX, int LC>
+__global__ __launch_bounds__(64) void so3_radial_tab_kernel(So3Args a, const int64_t *__restrict__ work,
+                                                            const double *__restrict__ img_r,
+                                                            const int64_t *__restrict__ img_rev,
+                                                            const double *__restrict__ tab, int nk, double h,
+                                                            double *__restrict__ rad) {
     const int64_t n = work[SO3_RWORK - 1];
     const int64_t *order = work + SO3_RWORK;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int64_t e = order[t];
-    so3_radial_image<NMAX, LC>(a, e, img_rev[e], img_r, rad);
+    const int64_t base = (int64_t)blockIdx.x * 64;
+    // the grid covers the image capacity: wavefronts past the sorted pairs
+    // leave at once
+    if (base >= n) return;
+    const int lane = threadIdx.x;
+    const int cnt = n - base < 64 ? (int)(n - base) : 64;
+    int64_t e = 0, re = 0, tb = 0;
+    int k = 0;
+    if (lane < cnt) {
+        e = order[base + lane];
+        re = img_rev[e];
+        const double x0 = img_r[3 * e], y0 = img_r[3 * e + 1], z0 = img_r[3 * e + 2];
+        const double u = sqrt(x0 * x0 + y0 * y0 + z0 * z0) / h;
+        k = (int)u;
+        k = k < 0 ? 0 : (k >= nk ? nk - 1 : k);
+        tb = __double_as_longlong(2.0 * (u - k) - 1.0);
+    }
+    const int nf = (a.lmax + 1) * 2 * NMAX;
+    const bool act = 2 * lane < nf;
+    const int f0 = act ? 2 * lane : 0;
+    double c0[SO3_TNP], c1[SO3_TNP];
+    int kc = -1;
+    for (int r = 0; r < cnt; r++) {
+        const int kr = __builtin_amdgcn_readlane(k, r);
+        if (kr != kc) {
+            kc = kr;
+            const double2 *src = reinterpret_cast<const double2 *>(tab + ((size_t)kr * nf + f0) * SO3_TNP);
+#pragma unroll
+            for (int q = 0; q < SO3_TNP / 2; q++) {
+                const double2 v = src[q], w = src[SO3_TNP / 2 + q];
+                c0[2 * q] = v.x;
+                c0[2 * q + 1] = v.y;
+                c1[2 * q] = w.x;
+                c1[2 * q + 1] = w.y;
+            }
+        }
+        const double t = __longlong_as_double(so3_readlane64(tb, r));
+        double v0 = c0[SO3_TDEG], v1 = c1[SO3_TDEG];
+#pragma unroll
+        for (int q = SO3_TDEG - 1; q >= 0; q--) {
+            v0 = fma(v0, t, c0[q]);
+            v1 = fma(v1, t, c1[q]);
+        }
+        const int64_t er = so3_readlane64(e, r), rer = so3_readlane64(re, r);
+        if (act) {
+            const double2 v = make_double2(v0, v1);
+            *reinterpret_cast<double2 *>(rad + (size_t)er * nf + f0) = v;
+            if (rer != er) *reinterpret_cast<double2 *>(rad + (size_t)rer * nf + f0) = v;
+        }
+    }
+}
+
+// The quadrature for the image pairs in |r| order, one lane per pair and
+// one wavefront per 64 pairs.  The rows are written one l at a time through
+// LDS, 2 nmax doubles of every pair's row at once, so that a store
+// instruction covers about 11 row pieces of 16 nmax bytes instead of 64
+// scattered 8-byte pieces.
+template <int NMAX, int LC>
+__global__ __launch_bounds__(64) void so3_radial_sorted_kernel(So3Args a, const int64_t *__restrict__ work,
+                                                               const double *__restrict__ img_r,
+                                                               const int64_t *__restrict__ img_rev,
+                                                               double *__restrict__ rad) {
+    __shared__ double buf[64][2 * NMAX + 1];
+    __shared__ int64_t s_e[64], s_re[64];
+    const int64_t n = work[SO3_RWORK - 1];
+    const int64_t *order = work + SO3_RWORK;
+    const int64_t base = (int64_t)blockIdx.x * 64;
+    if (base >= n) return;
+    const int lane = threadIdx.x;
+    const int cnt = n - base < 64 ? (int)(n - base) : 64;
+    int64_t e = 0, re = 0;
+    double R = 1.0;
+    if (lane < cnt) {
+        e = order[base + lane];
+        re = img_rev[e];
+        const double x0 = img_r[3 * e], y0 = img_r[3 * e + 1], z0 = img_r[3 * e + 2];
+        R = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+    }
+    s_e[lane] = e;
+    s_re[lane] = re;
+    const int L = a.lmax + 1, nf = L * 2 * NMAX;
+    double I[LC][NMAX], dI[LC][NMAX];
+    so3_radial_quadrature<NMAX, LC>(a, R, I, dI);
+#pragma unroll
+    for (int l = 0; l < LC; l++) {
+        if (l >= L) break;
+#pragma unroll
+        for (int nn = 0; nn < NMAX; nn++) {
+            buf[lane][nn] = I[l][nn];
+            buf[lane][NMAX + nn] = dI[l][nn];
+        }
+        __syncthreads();
+        for (int q = lane; q < cnt * NMAX; q += 64) {
+            const int r = q / NMAX, c = q - r * NMAX;
+            const double2 v = make_double2(buf[r][2 * c], buf[r][2 * c + 1]);
+            const int64_t er = s_e[r], rer = s_re[r];
+            *reinterpret_cast<double2 *>(rad + (size_t)er * nf + 2 * l * NMAX + 2 * c) = v;
+            if (rer != er) *reinterpret_cast<double2 *>(rad + (size_t)rer * nf + 2 * l * NMAX + 2 * c) = v;
+        }
+        __syncthreads();
+    }
 }
 
 // ------------------------------------------------------------ accumulate
@@ -1419,7 +1636,7 @@ static int so3_ctot_impl(const pxff_so3_params *p, int64_t natoms, const int64_t
                          const int64_t *d_img_j, const double *d_img_r, const int64_t *d_img_rev,
                          const int64_t *d_atom_z, const double *d_G, const double *d_rq, const double *d_lnorm,
                          const double *d_dr, double *d_out, double *d_radial, void *stream, int64_t *d_work = nullptr,
-                         int64_t cap = 0) {
+                         int64_t cap = 0, const double *d_table = nullptr) {
     int rc = so3_check(p);
     if (rc) return rc;
     if (natoms <= 0) return 0;
@@ -1450,10 +1667,20 @@ static int so3_ctot_impl(const pxff_so3_params *p, int64_t natoms, const int64_t
                     so3_rbin_scan_kernel<<<1, SO3_RSCAN_T, 0, s>>>(d_work);
                     so3_rbin_fill_kernel<<<SO3_RBLK, SO3_RBT, 0, s>>>(natoms, d_img_start, d_img_r, d_img_rev, p->rcut,
                                                                   d_work);
-                    SO3_DISPATCH_NMAX(p->nmax, if constexpr (NM <= 6) {
-                        so3_radial_sorted_kernel<NM, 7><<<div_up(cap, 256), 256, 0, s>>>(a, d_work, d_img_r,
-                                                                                        d_img_rev, d_radial);
-                    });
+                    if (d_table) {
+                        // from the tabulated integrals (pxff_so3_radial_table)
+                        const int nk = so3_rtab_nk(p);
+                        const double h = p->rcut / nk;
+                        SO3_DISPATCH_NMAX(p->nmax, if constexpr (NM <= 6) {
+                            so3_radial_tab_kernel<NM, 7><<<div_up(cap, 64), 64, 0, s>>>(
+                                a, d_work, d_img_r, d_img_rev, d_table, nk, h, d_radial);
+                        });
+                    } else {
+                        SO3_DISPATCH_NMAX(p->nmax, if constexpr (NM <= 6) {
+                            so3_radial_sorted_kernel<NM, 7><<<div_up(cap, 64), 64, 0, s>>>(a, d_work, d_img_r,
+                                                                                          d_img_rev, d_radial);
+                        });
+                    }
                 } else {
                     const int grid = d_img_rev ? cus() * 2 : cus() * 16;
                     SO3_DISPATCH_NMAX(p->nmax, if constexpr (NM <= 6) {
@@ -1495,11 +1722,44 @@ extern "C" int pxff_so3_ctot_pairs(const pxff_so3_params *p, int64_t natoms, con
                                    const int64_t *d_img_j, const double *d_img_r, const int64_t *d_img_rev,
                                    const int64_t *d_atom_z, const double *d_G, const double *d_rq,
                                    const double *d_lnorm, double *d_out, double *d_radial, int64_t *d_work,
-                                   int64_t cap, void *stream) {
+                                   int64_t cap, const double *d_table, void *stream) {
     PXFF_REQUIRE(d_img_rev != nullptr && d_radial != nullptr, "SO3: pairs need the reverse images and the radial store");
     PXFF_REQUIRE(cap >= 0 && (d_work == nullptr || cap > 0), "SO3: the |r| ordering needs the image capacity");
+    PXFF_REQUIRE(d_table == nullptr || (d_work != nullptr && p != nullptr && p->nmax <= 6 && p->lmax <= 6),
+                 "SO3: the radial table needs the |r| ordering workspace and nmax, lmax <= 6");
     return so3_ctot_impl(p, natoms, d_img_start, d_img_j, d_img_r, d_img_rev, d_atom_z, d_G, d_rq, d_lnorm, nullptr,
-                         d_out, d_radial, stream, d_work, cap);
+                         d_out, d_radial, stream, d_work, cap, d_table);
+}
+
+extern "C" int pxff_so3_radial_table_size(const pxff_so3_params *p, int *nk, int64_t *len) {
+    int rc = so3_check(p);
+    if (rc) return rc;
+    PXFF_REQUIRE(nk != nullptr && len != nullptr, "null output");
+    PXFF_REQUIRE(p->nmax <= 6 && p->lmax <= 6, "SO3: the radial table is for nmax, lmax <= 6");
+    *nk = so3_rtab_nk(p);
+    *len = (int64_t)(*nk) * (p->lmax + 1) * 2 * p->nmax * SO3_TNP;
+    return 0;
+}
+
+extern "C" int pxff_so3_radial_table(const pxff_so3_params *p, const double *d_G, const double *d_rq,
+                                     const double *d_lnorm, double *d_scratch, double *d_table, void *stream) {
+    int rc = so3_check(p);
+    if (rc) return rc;
+    PXFF_REQUIRE(p->nmax <= 6 && p->lmax <= 6, "SO3: the radial table is for nmax, lmax <= 6");
+    PXFF_REQUIRE(d_scratch != nullptr && d_table != nullptr, "null table or scratch");
+    So3Args a = make_args(p, d_G, d_rq, d_lnorm);
+    hipStream_t s = (hipStream_t)stream;
+    const int nk = so3_rtab_nk(p);
+    const int nf = (p->lmax + 1) * 2 * p->nmax;
+    const double h = p->rcut / nk;
+    SO3_DISPATCH_NMAX(p->nmax, if constexpr (NM <= 6) {
+        so3_rtab_nodes_kernel<NM, 7><<<div_up(nk * SO3_TNP, 256), 256, 0, s>>>(a, nk, h, d_scratch);
+    });
+    PXFF_LAUNCH_CHECK();
+    static const So3TabW W = so3_rtab_weights();
+    so3_rtab_coef_kernel<<<div_up(nk * nf, 256), 256, 0, s>>>(W, nk, nf, d_scratch, d_table);
+    PXFF_LAUNCH_CHECK();
+    return 0;
 }
 
 extern "C" int pxff_so3_power(const pxff_so3_params *p, int64_t natoms, const double *d_ctot, const double *d_dctot,

@@ -73,21 +73,41 @@ class So3Config:
         self.lnorm = torch.as_tensor(ln, dtype=torch.float64, device=dev)
         self._p = nat.So3Params(self.nmax, self.lmax, self.rcut, self.alpha, self.cutoff_id, self.nq,
                                 int(self.weight_on))
+        self._table = None
 
     @property
     def params(self):
         return ctypes.byref(self._p)
+
+    def radial_table(self):
+        """The tabulated radial integrals (pxff_so3_radial_table) for
+        nmax, lmax <= 6, built on the device at first use; None otherwise."""
+        if self.nmax > 6 or self.lmax > 6:
+            return None
+        if self._table is None:
+            nk, n = ctypes.c_int(), ctypes.c_int64()
+            nat.check(nat.lib().pxff_so3_radial_table_size(self.params, ctypes.byref(nk), ctypes.byref(n)),
+                      "pxff_so3_radial_table_size")
+            tab = torch.empty(n.value, dtype=torch.float64, device=self.G.device)
+            scratch = torch.empty(n.value, dtype=torch.float64, device=self.G.device)
+            p = nat.ptr
+            nat.check(nat.lib().pxff_so3_radial_table(self.params, p(self.G), p(self.rq), p(self.lnorm), p(scratch),
+                                                      p(tab), nat.stream()), "pxff_so3_radial_table")
+            self._table = tab
+        return self._table
 
 
 def _f64(n, dev):
     return torch.empty(max(int(n), 1), dtype=torch.float64, device=dev)
 
 
-def ctot(nl, cfg, atom_z, img_r=None, dr=None, radial=False):
+def ctot(nl, cfg, atom_z, img_r=None, dr=None, radial=False, use_table=True):
     """Per-centre expansion [N, nmax, lmax+1, 2lmax+1] complex (as float64
     pairs); with dr, its tangent along the per-image displacements.  With
     radial=True (plain expansion only) also the per-image radial integrals
-    [nimg, lmax+1, 2, nmax] for the backward: returns (ctot, radial)."""
+    [nimg, lmax+1, 2, nmax] for the backward: returns (ctot, radial).  With
+    the list's own separations and nmax, lmax <= 6 the integrals come from
+    the configuration's table (use_table) or the quadrature."""
     p = nat.ptr
     out = _f64(nl.natoms * cfg.csz * 2, nl.pos.device)
     r = nl.img_r if img_r is None else img_r.contiguous()
@@ -103,9 +123,11 @@ def ctot(nl, cfg, atom_z, img_r=None, dr=None, radial=False):
         # workspace of the |r| ordering of the pairs (scheduling only)
         cap = nl.img_capacity
         work = torch.empty(RWORK + cap, dtype=torch.int64, device=nl.pos.device)
+        tab = cfg.radial_table() if use_table else None
         nat.check(nat.lib().pxff_so3_ctot_pairs(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r),
                                                 p(nl.img_rev), p(atom_z), p(cfg.G), p(cfg.rq), p(cfg.lnorm),
-                                                p(out), p(rad), p(work), cap, nat.stream()), "pxff_so3_ctot_pairs")
+                                                p(out), p(rad), p(work), cap, p(tab) if tab is not None else None,
+                                                nat.stream()), "pxff_so3_ctot_pairs")
         return (out, rad) if keep else out
     nat.check(nat.lib().pxff_so3_ctot(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r), p(atom_z),
                                       p(cfg.G), p(cfg.rq), p(cfg.lnorm), p(d), p(out), p(rad), nat.stream()),

@@ -132,19 +132,42 @@ def test_so3_radial_integrals_vs_scipy_quadrature(nmax, lmax):
     """The stored radial integrals I_nl(R) = sum_q G[n][q] i_l(2 alpha R r_q)
     and dI_nl/dR against the same quadrature with scipy.special.spherical_in
     on the host (amorphous SiO2 images, 2 alpha R r_q from ~0.001 to 100:
-    the series, downward-recurrence and upward-recurrence ranges)."""
+    the series, downward-recurrence and upward-recurrence ranges), both as
+    tabulated (the default for these sizes) and by the device quadrature;
+    the quadrature once per image pair equals it for every image to rounding
+    (the same sums; a multiply-add may be fused in one kernel and not in the
+    other)."""
     from scipy.special import spherical_in
     s = workloads.amorphous_sio2(40, 80, seed=1)
     nl = DeviceNeighborList([s.positions], [s.cell], [s.pbc], 5.0)
     cfg = dso3.So3Config(nmax, lmax, 5.0, 2.0)
     z = torch.as_tensor(s.numbers, dtype=torch.int64, device="cuda")
-    ct, rad = dso3.ctot(nl, cfg, z, radial=True)  # once per image pair (the list's own separations)
     n = nl.nimg
-    # every image evaluated (a copy of the separations): bitwise the same
-    ct2, rad2 = dso3.ctot(nl, cfg, z, img_r=nl.img_r.clone(), radial=True)
     m = n * (lmax + 1) * 2 * nmax
-    assert torch.equal(rad[:m], rad2[:m]) and torch.equal(ct, ct2)
-    rad = rad[: n * (lmax + 1) * 2 * nmax].view(n, lmax + 1, 2, nmax).cpu().numpy()
+    # once per image pair (the list's own separations): table and quadrature
+    ct, rad = dso3.ctot(nl, cfg, z, radial=True)
+    ctq, radq = dso3.ctot(nl, cfg, z, radial=True, use_table=False)
+    # every image evaluated by the quadrature (a copy of the separations)
+    ct2, rad2 = dso3.ctot(nl, cfg, z, img_r=nl.img_r.clone(), radial=True)
+    # pairs' rows: both images of a pair carry the one evaluation
+    rev = nl.img_rev[:n].cpu().numpy()
+    rq = radq[:m].view(n, -1).cpu().numpy()
+    assert np.array_equal(rq, rq[rev])
+    print("pair vs per-image quadrature", rel_err(radq[:m].cpu().numpy(), rad2[:m].cpu().numpy()),
+          rel_err(ctq.cpu().numpy(), ct2.cpu().numpy()), flush=True)
+    assert rel_err(radq[:m].cpu().numpy(), rad2[:m].cpu().numpy()) < 1e-14
+    assert rel_err(ctq.cpu().numpy(), ct2.cpu().numpy()) < 1e-14
+    print("table vs quadrature", rel_err(rad[:m].cpu().numpy(), radq[:m].cpu().numpy()), flush=True)
+    assert rel_err(rad[:m].cpu().numpy(), radq[:m].cpu().numpy()) < 1e-13
+    assert rel_err(ct.cpu().numpy(), ctq.cpu().numpy()) < 1e-13
+    rt = rad[:m].view(n, -1).cpu().numpy()
+    assert np.array_equal(rt, rt[rev])
+    for store in (rad, radq):
+        _check_radial_store(store[:m].view(n, lmax + 1, 2, nmax).cpu().numpy(), nl, n, nmax, lmax)
+
+
+def _check_radial_store(rad, nl, n, nmax, lmax):
+    from scipy.special import spherical_in
     G, Q, _ = dso3.radial_tables(nmax, lmax, 5.0, 2.0)
     r = nl.img_r[: 3 * n].view(n, 3).cpu().numpy()
     pick = np.linspace(0, n - 1, 300).astype(int)
@@ -185,3 +208,23 @@ def test_so3_sio2_scale_adjoint_vs_plain():
     ep, fp = _plain_forces(nl, cfg, z, model, species)
     assert rel_err(fa.cpu().numpy(), fp.cpu().numpy()) < 1e-9
     assert float(torch.linalg.norm(fa.sum(0))) < 1e-9 * float(torch.linalg.norm(fa))
+
+
+@pytest.mark.parametrize("nmax,lmax,rc,alpha", [(6, 6, 6.0, 4.0), (3, 3, 4.0, 0.5), (6, 4, 5.0, 6.0)])
+def test_so3_radial_table_other_parameters(nmax, lmax, rc, alpha):
+    """The tabulated integrals against the device quadrature for steeper and
+    flatter settings (the interval count grows with 2 alpha rc^2): norm-wise
+    1e-13, and every value within 1e-12 of its function's largest value."""
+    s = workloads.amorphous_sio2(40, 80, seed=2)
+    nl = DeviceNeighborList([s.positions], [s.cell], [s.pbc], rc)
+    cfg = dso3.So3Config(nmax, lmax, rc, alpha)
+    z = torch.as_tensor(s.numbers, dtype=torch.int64, device="cuda")
+    m = nl.nimg * (lmax + 1) * 2 * nmax
+    _, rad = dso3.ctot(nl, cfg, z, radial=True)
+    _, radq = dso3.ctot(nl, cfg, z, radial=True, use_table=False)
+    a = rad[:m].view(nl.nimg, -1).cpu().numpy()
+    b = radq[:m].view(nl.nimg, -1).cpu().numpy()
+    print("table vs quadrature", rel_err(a, b), flush=True)
+    assert rel_err(a, b) < 1e-13
+    scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
+    assert (np.abs(a - b) / scale).max() < 1e-12
