@@ -51,6 +51,29 @@ extern "C" {
 int pxff_tune_set(int knob, int value);
 int pxff_tune_get(int knob); /* -1 for an unknown knob */
 
+/* Bytes of the caller-owned buffers that are neither inputs nor outputs of
+ * their entry point (the ABI allocates nothing else on the device apart
+ * from the immutable per-parameter tables it builds once, under a mutex):
+ *   PXFF_WS_NLIST        the int32 workspace of pxff_nlist_count / _fill
+ *                        (bin_count, bin_start, atom_bin, wrap, bin_atoms),
+ *                        n0 = natoms, n1 = nbins_total; params unused
+ *   PXFF_WS_SO4_Z        d_z of pxff_so4_zb (Z and the B row partials),
+ *                        params = pxff_so4_params, n0 = natoms
+ *   PXFF_WS_SO4_TANGENT  d_dutot of pxff_so4_tangent, params =
+ *                        pxff_so4_params, n0 = natoms
+ *   PXFF_WS_SO3_PAIRS    d_work of pxff_so3_ctot_pairs, n1 = image capacity
+ *   PXFF_WS_SO3_TABLE    d_table of pxff_so3_radial_table (its d_scratch is
+ *                        the same size), params = pxff_so3_params
+ * The launch knobs below are the only other state: process-wide atomics
+ * read once per launch, so calls on several threads and streams are
+ * reentrant. */
+#define PXFF_WS_NLIST 0
+#define PXFF_WS_SO4_Z 1
+#define PXFF_WS_SO4_TANGENT 2
+#define PXFF_WS_SO3_PAIRS 3
+#define PXFF_WS_SO3_TABLE 4
+int pxff_workspace_size(int kind, const void *params, int64_t n0, int64_t n1, int64_t *bytes);
+
 /* Copy the calling thread's last error message into buf (NUL-terminated). */
 int pxff_last_error(char *buf, size_t len);
 

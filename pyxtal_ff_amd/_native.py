@@ -41,6 +41,7 @@ _SIGS = {
     "pxff_scan_i64_capped": ([P, I64, I64, P, P, P], I32),
     "pxff_tune_set": ([ctypes.c_int, ctypes.c_int], I32),
     "pxff_tune_get": ([ctypes.c_int], I32),
+    "pxff_workspace_size": ([ctypes.c_int, P, I64, I64, P], I32),
     "pxff_nlist_count": ([P, I64, P, P, P, P, P, P, I64, F64, P, P, P, P, P, P, P], I32),
     "pxff_nlist_fill": ([P, I64, P, P, P, P, P, F64, P, P, P, P, P, P, P, P, P, P, P], I32),
     "pxff_nlist_rev": ([I64, I64, P, P, P, P, P, P, P], I32),

@@ -1515,7 +1515,8 @@ using namespace pxff;
 namespace {
 int so3_series_init();
 
-int so3_check(const pxff_so3_params *p) {
+// The parameter checks alone (host only: the size queries use them).
+static int so3_check_params(const pxff_so3_params *p) {
     PXFF_REQUIRE(p != nullptr, "null params");
     if (p->nmax < 1 || p->nmax > 11 || p->lmax < 0 || p->lmax > SO3_LMAXC) {
         set_error("SO3: nmax must be in 1..11 and lmax in 0..32");
@@ -1528,7 +1529,12 @@ int so3_check(const pxff_so3_params *p) {
         set_error("SO3: nmax*(lmax+1)*(2lmax+1) > 4096 is not supported on the device path");
         return PXFF_ENOTIMPL;
     }
-    return so3_series_init();
+    return 0;
+}
+
+int so3_check(const pxff_so3_params *p) {
+    const int rc = so3_check_params(p);
+    return rc ? rc : so3_series_init();
 }
 
 So3Args make_args(const pxff_so3_params *p, const double *G, const double *rq, const double *lnorm) {
@@ -1732,7 +1738,7 @@ extern "C" int pxff_so3_ctot_pairs(const pxff_so3_params *p, int64_t natoms, con
 }
 
 extern "C" int pxff_so3_radial_table_size(const pxff_so3_params *p, int *nk, int64_t *len) {
-    int rc = so3_check(p);
+    int rc = so3_check_params(p);
     if (rc) return rc;
     PXFF_REQUIRE(nk != nullptr && len != nullptr, "null output");
     PXFF_REQUIRE(p->nmax <= 6 && p->lmax <= 6, "SO3: the radial table is for nmax, lmax <= 6");

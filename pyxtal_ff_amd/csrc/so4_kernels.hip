@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <atomic>
 #include <mutex>
 #include <tuple>
 #include <utility>
@@ -1574,10 +1575,13 @@ using namespace pxff;
 
 namespace pxff {
 // Process-wide launch parameters set by the autotuner (pxff_tune_set).
-static int g_tune[PXFF_TUNE_COUNT] = {16, 16, 4, 4};
+// Atomic: a call reads each knob once when it launches, so calls from
+// several host threads and streams never race with a concurrent set (a set
+// affects the launches that start after it).
+static std::atomic<int> g_tune[PXFF_TUNE_COUNT] = {16, 16, 4, 4};
 static const int g_tune_lo[PXFF_TUNE_COUNT] = {1, 1, 1, 0};
 static const int g_tune_hi[PXFF_TUNE_COUNT] = {256, 256, 4, 4};
-int tune_value(int knob) { return g_tune[knob]; }
+int tune_value(int knob) { return g_tune[knob].load(std::memory_order_relaxed); }
 
 int so4_check_params(const pxff_so4_params *p) {
     PXFF_REQUIRE(p != nullptr, "null params");
@@ -1625,7 +1629,7 @@ extern "C" int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64
     const So4Dev *T = nullptr;
     if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * g_tune[PXFF_TUNE_ACCUM_WAVES_PER_CU]);
+    const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * tune_value(PXFF_TUNE_ACCUM_WAVES_PER_CU));
     const double fcut0 = cutoff_f0_host(p->cutoff_id);
 #define PXFF_ACCUM_UI(MODE)                                                                              \
     PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, false, true, MODE><<<grid, 64, 0, s>>>(               \
@@ -1703,7 +1707,7 @@ static int so4_deidrj_launch(const pxff_so4_params *p, int64_t nimg, const int64
     // few enough that the grid has about 16 wavefronts per CU
     const int nr = (int)std::max<int64_t>(1, std::min<int64_t>(64 / ng, nimg / ((int64_t)num_cus() * 16 * ng)));
     const int nb = ng * nr;  // images per wavefront (staged batch)
-    const int wpb = g_tune[PXFF_TUNE_IMAGE_WAVES_PER_BLOCK];
+    const int wpb = tune_value(PXFF_TUNE_IMAGE_WAVES_PER_BLOCK);
     const int grid = div_up(div_up(nimg, nb), wpb);
     const size_t lds = sizeof(double) * (size_t)wpb * (DEIDRJ_NST + 5) * 64;  // per-wavefront stage
     PXFF_DISPATCH_TWOJ(p->twol, (so4_deidrj_kernel<TJ><<<grid, 64 * wpb, lds, s>>>(nimg, d_img_atom, d_img_j, d_img_r, d_atom_w,
@@ -1766,7 +1770,7 @@ static int so4_tangent_launch(const pxff_so4_params *p, const So4Dev *T, int64_t
                               const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
                               const double *d_dr, const double *d_z, double *d_dutot, double *d_db, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * g_tune[PXFF_TUNE_ACCUM_WAVES_PER_CU]);
+    const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * tune_value(PXFF_TUNE_ACCUM_WAVES_PER_CU));
 #define PXFF_ACCUM_TAN(MODE)                                                                      \
     PXFF_DISPATCH_TWOJ(p->twol, (so4_accum_kernel<TJ, true, true, MODE><<<grid, 64, 0, s>>>(         \
                                     natoms, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr, d_dr, nullptr, \
@@ -1961,11 +1965,11 @@ extern "C" int pxff_tune_set(int knob, int value) {
     PXFF_REQUIRE(knob >= 0 && knob < PXFF_TUNE_COUNT, "unknown tuning knob");
     PXFF_REQUIRE(value >= g_tune_lo[knob] && value <= g_tune_hi[knob], "tuning value out of range");
     if (knob == PXFF_TUNE_IMAGE_WAVES_PER_BLOCK) PXFF_REQUIRE(value == 1 || value == 2 || value == 4, "waves per block must be 1, 2 or 4");
-    g_tune[knob] = value;
+    g_tune[knob].store(value, std::memory_order_relaxed);
     return 0;
 }
 
 extern "C" int pxff_tune_get(int knob) {
     if (knob < 0 || knob >= PXFF_TUNE_COUNT) return -1;
-    return g_tune[knob];
+    return tune_value(knob);
 }

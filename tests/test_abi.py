@@ -53,3 +53,38 @@ def test_error_reporting_without_gpu():
     assert "2J" in _native.last_error()
     assert L.pxff_so4_sizes(8, *[ctypes.byref(x) for x in v]) == 0
     assert [x.value for x in v] == [285, 155, 2386, 55]
+
+
+def test_workspace_sizes_without_gpu():
+    """pxff_workspace_size (a host-side query) against the buffer sizes the
+    entry points document."""
+    from pyxtal_ff_amd import _native
+    L = _native.load()
+    b = ctypes.c_int64()
+    n = 4097
+    p4 = _native.So4Params(8, 4.9, 0.99363, 1, 0)
+    assert L.pxff_workspace_size(1, ctypes.byref(p4), n, 0, ctypes.byref(b)) == 0
+    zl = ctypes.c_int64()
+    assert L.pxff_so4_zlen(8, n, ctypes.byref(zl)) == 0
+    assert b.value == 8 * zl.value
+    # tangent: complex half rows (155 at 2J=8) of 65 padded tiles of 64 atoms
+    assert L.pxff_workspace_size(2, ctypes.byref(p4), n, 0, ctypes.byref(b)) == 0
+    assert b.value == 65 * 64 * 155 * 16
+    # neighbour-list int32 workspace
+    assert L.pxff_workspace_size(0, None, n, 1000, ctypes.byref(b)) == 0
+    assert b.value == 4 * (1000 + 1001 + n + 3 * n + n)
+    # SO3 pair ordering: 256 x 64 bin counts, the count, the capacity's order
+    assert L.pxff_workspace_size(3, None, 0, 576000, ctypes.byref(b)) == 0
+    assert b.value == 8 * (256 * 64 + 1 + 576000)
+    # SO3 radial table: intervals = the least power of two >= 256 covering
+    # 2.5 x the steepness 2 alpha rc^2 (at most 4096), 12 coefficients for
+    # each of (lmax + 1) 2 nmax functions
+    for nmax, lmax, rc, alpha in [(6, 6, 5.0, 2.0), (3, 4, 5.0, 6.0), (2, 1, 4.0, 0.5)]:
+        p3 = _native.So3Params(nmax, lmax, rc, alpha, 1, 10 * (nmax + lmax + 1), 0)
+        assert L.pxff_workspace_size(4, ctypes.byref(p3), 0, 0, ctypes.byref(b)) == 0
+        nk = 256
+        while nk < 4096 and nk < 2.5 * 2 * alpha * rc * rc:
+            nk *= 2
+        assert b.value == 8 * nk * (lmax + 1) * 2 * nmax * 12, (nmax, lmax, rc, alpha)
+    assert L.pxff_workspace_size(9, None, 0, 0, ctypes.byref(b)) != 0
+    assert "workspace kind" in _native.last_error()
