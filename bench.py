@@ -124,20 +124,43 @@ KERNEL_SYMBOL = {"ui": "so4_accum_kernel", "z": "so4_z_tile_kernel", "bi": "so4_
                  "rows": "so4_rows_kernel", "dxdr_contract": "dxdr_contract_kernel"}
 
 
-def isolated_kernel_ms(dso4, potential, model, species_atom, nl, cfg, w, reps=5):
-    """Mean duration of each pxff kernel of both paths, from reps
-    back-to-back launches between two HIP events on the launch stream."""
-    import torch
+_SPIN_CYCLES_PER_MS = None
 
-    def t(fn):
-        fn()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+def gpu_time_ms(fn, reps=5):
+    """Mean GPU duration of fn's launches.  After a warm-up call a GPU spin
+    (torch.cuda._sleep) is queued ahead of the bracketing events, so every
+    timed launch is enqueued while the GPU is still busy and host time between
+    the calls is not counted (back-to-back Python calls of a short kernel
+    otherwise leave the GPU idle between launches)."""
+    import torch
+    global _SPIN_CYCLES_PER_MS
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    if _SPIN_CYCLES_PER_MS is None:
         a.record()
-        for _ in range(reps):
-            fn()
+        torch.cuda._sleep(1 << 20)
         b.record()
         b.synchronize()
-        return a.elapsed_time(b) / reps
+        _SPIN_CYCLES_PER_MS = (1 << 20) / max(a.elapsed_time(b), 1e-3)
+    # about 2 ms of spin, plus 1 ms per call of host time
+    torch.cuda._sleep(int(_SPIN_CYCLES_PER_MS * (2.0 + reps)))
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def isolated_kernel_ms(dso4, potential, model, species_atom, nl, cfg, w, reps=5):
+    """Mean duration of each pxff kernel of both paths, from reps
+    back-to-back launches between two HIP events on the launch stream
+    (gpu_time_ms)."""
+
+    def t(fn):
+        return gpu_time_ms(fn, reps)
 
     out = {}
     u = dso4.ui(nl, cfg, w, w)
@@ -440,14 +463,7 @@ def run_so3_sio2(steps=5):
     nimg = nl.nimg
 
     def t(fn, reps=5):
-        fn()
-        a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a0.record()
-        for _ in range(reps):
-            fn()
-        a1.record()
-        a1.synchronize()
-        return a0.elapsed_time(a1) / reps
+        return gpu_time_ms(fn, reps)
 
     kt = {}
     x, ct, rad = dso3.forward(nl, cfg, z, radial=True)

@@ -64,6 +64,7 @@ int pxff_tune_get(int knob); /* -1 for an unknown knob */
  *   PXFF_WS_SO3_PAIRS    d_work of pxff_so3_ctot_pairs, n1 = image capacity
  *   PXFF_WS_SO3_TABLE    d_table of pxff_so3_radial_table (its d_scratch is
  *                        the same size), params = pxff_so3_params
+ *   PXFF_WS_SO4_UI       d_stage of pxff_so4_ui_staged, n1 = image capacity
  * The launch knobs below are the only other state: process-wide atomics
  * read once per launch, so calls on several threads and streams are
  * reentrant. */
@@ -72,6 +73,7 @@ int pxff_tune_get(int knob); /* -1 for an unknown knob */
 #define PXFF_WS_SO4_TANGENT 2
 #define PXFF_WS_SO3_PAIRS 3
 #define PXFF_WS_SO3_TABLE 4
+#define PXFF_WS_SO4_UI 5
 int pxff_workspace_size(int kind, const void *params, int64_t n0, int64_t n1, int64_t *bytes);
 
 /* Copy the calling thread's last error message into buf (NUL-terminated). */
@@ -179,6 +181,19 @@ int pxff_so4_sizes(int twol, int *nu, int *nhalf, int *nz, int *nb);
 int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_j,
                 const double *d_img_r, const double *d_atom_w, const double *d_site_w,
                 const int64_t *d_work_order, double *d_utot, void *stream);
+
+/* pxff_so4_ui with the per-image values of the sweeps (conj(a), conj(b),
+ * w fc) computed first by a kernel of their own into d_stage (6 doubles per
+ * image, pxff_workspace_size PXFF_WS_SO4_UI, n1 = cap), for images
+ * e < min(d_img_start[natoms], cap): the sweep kernel then reads one record
+ * per image instead of evaluating cutoff and Cayley-Klein parameters
+ * between its sweeps (same construction as pxff_so4_ui, SNAP.py:1480-1500;
+ * equal to it to rounding).  cap must be at least the image count.  With
+ * a summation mode other than the register sums (pxff_tune_set
+ * PXFF_TUNE_ACCUM_LDSADD != 4) or 2J > 8 it is pxff_so4_ui. */
+int pxff_so4_ui_staged(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start, const int64_t *d_img_j,
+                       const double *d_img_r, const double *d_atom_w, const double *d_site_w,
+                       const int64_t *d_work_order, int64_t cap, double *d_stage, double *d_utot, void *stream);
 
 /* Z (and B) from the tile-major U_tot (compute_zi + compute_bi,
  * SNAP.py:1050-1141).  d_z out, pxff_so4_zlen(2J, natoms) doubles: Z

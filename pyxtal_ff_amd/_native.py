@@ -48,6 +48,7 @@ _SIGS = {
     "pxff_nlist_rows": ([I64, I64, I64, P, P, P, P, P, P, P, P, P, P], I32),
     "pxff_so4_sizes": ([ctypes.c_int, P, P, P, P], I32),
     "pxff_so4_ui": ([P, I64, P, P, P, P, P, P, P, P], I32),
+    "pxff_so4_ui_staged": ([P, I64, P, P, P, P, P, P, I64, P, P, P], I32),
     "pxff_so4_zb": ([P, I64, P, P, P, P], I32),
     "pxff_so4_zlen": ([ctypes.c_int, I64, P], I32),
     "pxff_so4_z_atom_major": ([P, I64, P, P, P], I32),

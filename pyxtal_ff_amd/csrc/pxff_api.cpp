@@ -50,6 +50,9 @@ extern "C" int pxff_workspace_size(int kind, const void *params, int64_t n0, int
     case PXFF_WS_SO3_PAIRS:
         *bytes = 8 * (256 * 64 + 1 + n1);
         return 0;
+    case PXFF_WS_SO4_UI:
+        *bytes = 8 * 6 * n1;
+        return 0;
     case PXFF_WS_SO3_TABLE: {
         PXFF_REQUIRE(params != nullptr, "null params");
         int nk = 0;

@@ -298,6 +298,59 @@ __device__ inline void sweep_rows(int mb, const double *__restrict__ rpq, const 
     }
 }
 
+// The same sweep with the recursion coefficients taken out.  The
+// coefficients are sqrt(p / q) (init_rootpqarray, SNAP.py:861-868):
+// c1 = sqrt(l - m) / sqrt(q), c2 = sqrt(m) / sqrt(q) with q = l - mb (q = l
+// on row 0), so with
+//   u^l_mb[m] = sqrt(m! (l - m)! / (mb! (l - mb)!)) W^l[m]
+// the row recursion becomes W^l[m] = ca W^{l-1}[m] - cb W^{l-1}[m-1]
+// (binomial, no coefficients: W^l on row 0 is C(l, m) ca^(l-m) (-cb)^m),
+// and the inversion flip keeps its form (the factor is symmetric under
+// m <-> l - m).  f(l, w, half) receives W; the factor (so4_wscale) is
+// image-independent, so sums over images take it once.  |W| <= C(l, l/2)
+// (252 at 2J = 10).
+template <int TWOJ, typename F>
+__device__ inline void sweep_rows_w(int mb, const CK &ck, F &&f) {
+    cd w[TWOJ + 1];
+#pragma unroll
+    for (int m = 0; m <= TWOJ; m++) w[m] = cmake(0.0, 0.0);
+    w[0] = cmake(1.0, 0.0);
+    f(0, w, mb == 0);
+    const cd mcb = cmake(-ck.cb.re, -ck.cb.im);
+#pragma unroll
+    for (int l = 1; l <= TWOJ; l++) {
+#pragma unroll
+        for (int m = TWOJ; m >= 0; --m) {
+            if (m > l) continue;
+            if (m == l)
+                w[m] = cmul(mcb, w[m - 1]);
+            else if (m >= 1)
+                w[m] = cmadd(mcb, w[m - 1], cmul(ck.ca, w[m]));
+            else
+                w[m] = cmul(ck.ca, w[m]);
+        }
+        if (l == mb) {
+            cd t[TWOJ + 1];
+#pragma unroll
+            for (int m = 0; m <= TWOJ; m++) {
+                if (m > l) continue;
+                const cd o = w[l - m];
+                t[m] = ((l - m) & 1) ? cmake(-o.re, o.im) : cmake(o.re, -o.im);
+            }
+#pragma unroll
+            for (int m = 0; m <= TWOJ; m++)
+                if (m <= l) w[m] = t[m];
+        }
+        f(l, w, 2 * mb <= l);
+    }
+}
+
+// sqrt(m! (l - m)! / (mb! (l - mb)!)), the factor of sweep_rows_w, from
+// sf[k] = sqrt(k!) and isf[k] = 1 / sqrt(k!).
+__device__ inline double so4_wscale(int l, int m, int mb, const double *sf, const double *isf) {
+    return (sf[m] * sf[l - m]) * (isf[mb] * isf[l - mb]);
+}
+
 // ------------------------------------------------------ su(2) generators
 // U^l is a representation of SU(2): U^l(g h) = U^l(g) U^l(h) for the
 // Cayley-Klein matrix g = [[conj a, -conj b], [b, a]] of an image.  Hence
@@ -590,6 +643,89 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void so
     }
 }
 
+// The per-image values the U_tot sweeps read (compute_uarray_recursive's
+// a, b, SNAP.py:1515-1555, with the cutoff and weight of add_uarraytot):
+// conj(a), conj(b) and w fc (0 for coincident atoms, SNAP.py:1487), and for
+// the tangent w fc p(eta), w fc q(eta), w dfc (rhat . dr) and a valid flag
+// in place of w fc.
+template <bool TANGENT>
+__device__ inline void so4_stage_image(bool ok, int64_t img, const int64_t *__restrict__ img_j,
+                                       const double *__restrict__ img_r, const double *__restrict__ atom_w,
+                                       const double *__restrict__ dr, double rcut, double rfac0, int cutoff_id,
+                                       double *v) {
+    double x = 0, y = 0, z = 1.0, r = 1.0, w = 0.0;
+    if (ok) {
+        x = img_r[3 * img];
+        y = img_r[3 * img + 1];
+        z = img_r[3 * img + 2];
+        r = sqrt(x * x + y * y + z * z);
+        w = atom_w[img_j[img]];
+        if (r < 1e-8) ok = false;  // SNAP.py:1487 (padding rows)
+    }
+    if (!ok) {
+        x = 0;
+        y = 0;
+        z = 1.0;
+        r = 1.0;
+    }
+    double fc, dfc;
+    cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
+    CK ck;
+    cayley_klein(x, y, z, r, rfac0 * M_PI * r / rcut, TANGENT, ck);
+    v[0] = ck.ca.re;
+    v[1] = ck.ca.im;
+    v[2] = ck.cb.re;
+    v[3] = ck.cb.im;
+    if constexpr (!TANGENT) {
+        v[4] = ok ? fc * w : 0.0;  // a zero weight adds nothing either way
+    } else {
+        double p[3];
+        cd q[3];
+        su2_generators(ck, p, q);
+        double d0 = 0, d1 = 0, d2 = 0;
+        if (ok) {
+            d0 = dr[3 * img];
+            d1 = dr[3 * img + 1];
+            d2 = dr[3 * img + 2];
+        }
+        const double wf = ok ? w * fc : 0.0;
+        v[4] = wf * (p[0] * d0 + p[1] * d1 + p[2] * d2);
+        v[5] = wf * (q[0].re * d0 + q[1].re * d1 + q[2].re * d2);
+        v[6] = wf * (q[0].im * d0 + q[1].im * d1 + q[2].im * d2);
+        // D . dr = fc (dU . dr) + dfc (rhat . dr) U
+        v[7] = ok ? w * dfc * (x * d0 + y * d1 + z * d2) / r : 0.0;
+        v[8] = ok ? 1.0 : 0.0;
+    }
+}
+
+// Record stride (doubles) of the staged values: 5 (9 with the tangent) used,
+// padded to 16-byte multiples.
+template <bool TANGENT>
+constexpr int so4_stage_stride() { return TANGENT ? 10 : 6; }
+
+// The staged values of every image e < min(img_start[natoms], cap), one
+// thread per image (grid stride): the U_tot sweep kernel then reads one
+// record per image instead of evaluating the cutoff and the Cayley-Klein
+// parameters between its sweeps.
+template <bool TANGENT>
+__global__ __launch_bounds__(256) void so4_stage_kernel(int64_t natoms, int64_t cap,
+                                                        const int64_t *__restrict__ img_start,
+                                                        const int64_t *__restrict__ img_j,
+                                                        const double *__restrict__ img_r,
+                                                        const double *__restrict__ atom_w,
+                                                        const double *__restrict__ dr, double rcut, double rfac0,
+                                                        int cutoff_id, double *__restrict__ stage) {
+    constexpr int NS = so4_stage_stride<TANGENT>();
+    const int64_t n = img_start[natoms] < cap ? img_start[natoms] : cap;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        double v[TANGENT ? 9 : 5];
+        so4_stage_image<TANGENT>(true, e, img_j, img_r, atom_w, dr, rcut, rfac0, cutoff_id, v);
+        double *o = stage + e * NS;
+#pragma unroll
+        for (int k = 0; k < (TANGENT ? 9 : 5); k++) o[k] = v[k];
+    }
+}
+
 // U_tot (and its tangent) with the partial sums in registers.  One
 // wavefront per centre, G = 2J/2+1 lanes per image as in so4_accum_kernel,
 // but lane (g, mb) adds row mb of every level of all its images into
@@ -598,26 +734,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void so
 // LDS in a fixed order (groups g < NP store, groups g + NP add, then the NP
 // arrays are summed in order: deterministic).  2J <= 8 (at 2J=10 the
 // register partials would exceed the register file; so4_accum_kernel).
-template <int TWOJ, bool TANGENT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) void so4_utot_kernel(
+template <int TWOJ, bool TANGENT, bool STAGED>
+__device__ __forceinline__ void so4_utot_body(
     int64_t natoms, const int64_t *__restrict__ img_start, const int64_t *__restrict__ img_j,
     const double *__restrict__ img_r, const double *__restrict__ atom_w, const double *__restrict__ site_w,
     const double *__restrict__ dr, const int64_t *__restrict__ work_order, const double *rootpq_g,
-    const double *lnorm_g, double rcut, double rfac0, int cutoff_id, double fcut0, double *__restrict__ out) {
+    const double *lnorm_g, double rcut, double rfac0, int cutoff_id, double fcut0, double *__restrict__ out,
+    const double *__restrict__ stage) {
     constexpr int G = TWOJ / 2 + 1;
     constexpr int NG = 64 / G;
     constexpr int NH = nhalf_of(TWOJ);
     constexpr int NSLOT = (TWOJ + 1) * (TWOJ + 2) / 2;  // (l, m <= l) of every level
     constexpr int NP = (NG + 1) / 2;                    // LDS partial arrays
     __shared__ cd part[NP][NH];
-    __shared__ double rpq[(TWOJ + 1) * (TWOJ + 1)];
+    __shared__ double sf[TWOJ + 1], isf[TWOJ + 1];  // sqrt(k!), 1 / sqrt(k!)
     __shared__ double lnorm[TWOJ + 1];
     // staged per image: ca, cb, fc w (0: no image); with TANGENT ca, cb,
     // w fc p(eta), w fc q(eta), w dfc (rhat . dr), valid
     __shared__ double stg[TANGENT ? 9 : 5][64];
     const int lane = threadIdx.x;
     const int g = lane / G, mb = lane % G;
-    for (int k = lane; k < (TWOJ + 1) * (TWOJ + 1); k += 64) rpq[k] = rootpq_g[k];
+    (void)rootpq_g;  // the W recursion has no coefficients
+    if (lane <= TWOJ) {
+        double f = 1.0;
+        for (int q = 2; q <= lane; q++) f *= q;
+        sf[lane] = sqrt(f);
+        isf[lane] = 1.0 / sqrt(f);
+    }
     for (int k = lane; k <= TWOJ; k += 64) lnorm[k] = lnorm_g[k];
     if (blockIdx.x == 0 && (natoms & 63)) {
         // padding lanes of the last tile (read by the Z kernel) are zero
@@ -628,70 +771,62 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) void so
     __syncthreads();
     for (int64_t c = blockIdx.x; c < natoms; c += gridDim.x) {
         const int64_t i = work_order ? work_order[c] : c;
-        cd acc[NSLOT];
-#pragma unroll
-        for (int q = 0; q < NSLOT; q++) acc[q] = cmake(0.0, 0.0);
         const int64_t s = img_start[i], e = img_start[i + 1];
+        const double wself = TANGENT ? 0.0 : site_w[i];  // loaded ahead of the sweeps
+        if (s == e) {  // no image: the partial arrays start (and stay) at zero
+            for (int k = lane; k < NP * NH; k += 64) part[k / NH][k % NH] = cmake(0.0, 0.0);
+            __syncthreads();
+        }
         for (int64_t b64 = s; b64 < e; b64 += 64) {
             const int64_t e64 = b64 + 64 < e ? b64 + 64 : e;
-            {
+            if constexpr (!STAGED) {
                 const int64_t img = b64 + lane;
-                bool ok = img < e64;
-                double x = 0, y = 0, z = 1.0, r = 1.0, w = 0.0;
-                if (ok) {
-                    x = img_r[3 * img];
-                    y = img_r[3 * img + 1];
-                    z = img_r[3 * img + 2];
-                    r = sqrt(x * x + y * y + z * z);
-                    w = atom_w[img_j[img]];
-                    if (r < 1e-8) ok = false;  // SNAP.py:1487 (padding rows)
-                }
-                if (!ok) {
-                    x = 0;
-                    y = 0;
-                    z = 1.0;
-                    r = 1.0;
-                }
-                double fc, dfc;
-                cutoff_fdf(cutoff_id, r, rcut, fc, dfc);
-                CK ck;
-                cayley_klein(x, y, z, r, rfac0 * M_PI * r / rcut, TANGENT, ck);
-                stg[0][lane] = ck.ca.re;
-                stg[1][lane] = ck.ca.im;
-                stg[2][lane] = ck.cb.re;
-                stg[3][lane] = ck.cb.im;
-                if constexpr (!TANGENT) {
-                    stg[4][lane] = ok ? fc * w : 0.0;  // a zero weight adds nothing either way
-                } else {
-                    double p[3];
-                    cd q[3];
-                    su2_generators(ck, p, q);
-                    double d0 = 0, d1 = 0, d2 = 0;
-                    if (ok) {
-                        d0 = dr[3 * img];
-                        d1 = dr[3 * img + 1];
-                        d2 = dr[3 * img + 2];
-                    }
-                    const double wf = ok ? w * fc : 0.0;
-                    stg[4][lane] = wf * (p[0] * d0 + p[1] * d1 + p[2] * d2);
-                    stg[5][lane] = wf * (q[0].re * d0 + q[1].re * d1 + q[2].re * d2);
-                    stg[6][lane] = wf * (q[0].im * d0 + q[1].im * d1 + q[2].im * d2);
-                    // D . dr = fc (dU . dr) + dfc (rhat . dr) U
-                    stg[7][lane] = ok ? w * dfc * (x * d0 + y * d1 + z * d2) / r : 0.0;
-                    stg[8][lane] = ok ? 1.0 : 0.0;
-                }
+                double v[TANGENT ? 9 : 5];
+                so4_stage_image<TANGENT>(img < e64, img, img_j, img_r, atom_w, dr, rcut, rfac0, cutoff_id, v);
+#pragma unroll
+                for (int q = 0; q < (TANGENT ? 9 : 5); q++) stg[q][lane] = v[q];
             }
-            __syncthreads();
+            if constexpr (!STAGED) __syncthreads();
+            // the chunk's sums: live only over its sweeps (not across the
+            // staging above), flushed to the partial arrays below
+            cd acc[NSLOT];
+#pragma unroll
+            for (int q = 0; q < NSLOT; q++) acc[q] = cmake(0.0, 0.0);
+            constexpr int NV = TANGENT ? 9 : 5;
+            // STAGED: the group's record of the next pass, loaded one pass
+            // ahead (the loads are in flight during the current sweeps)
+            double nv[NV];
+            auto fetch = [&](int64_t bs) {
+                const int64_t im = bs + g;
+                const bool ok = g < NG && im < e64;
+                const double *rec = stage + (ok ? im : b64) * so4_stage_stride<TANGENT>();
+#pragma unroll
+                for (int q = 0; q < NV; q++) nv[q] = rec[q];
+                if (!ok) {
+#pragma unroll
+                    for (int q = 0; q < NV; q++) nv[q] = 0.0;
+                }
+            };
+            if constexpr (STAGED) fetch(b64);
             for (int64_t base = b64; base < e64; base += NG) {
                 const int64_t img = base + g;
                 const int si = (g < NG && img < e64) ? (int)(img - b64) : 0;
-                const bool valid = (g < NG) && (img < e64) && stg[TANGENT ? 8 : 4][si] != 0.0;
+                double sv[NV];
+                if constexpr (STAGED) {
+#pragma unroll
+                    for (int q = 0; q < NV; q++) sv[q] = nv[q];
+                    if (base + NG < e64) fetch(base + NG);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NV; q++) sv[q] = stg[q][si];
+                }
+                const bool valid = (g < NG) && (img < e64) && sv[TANGENT ? 8 : 4] != 0.0;
                 CK ck;
-                ck.ca = cmake(stg[0][si], stg[1][si]);
-                ck.cb = cmake(stg[2][si], stg[3][si]);
+                ck.ca = cmake(sv[0], sv[1]);
+                ck.cb = cmake(sv[2], sv[3]);
                 if constexpr (!TANGENT) {
-                    const double sc = stg[4][si];
-                    sweep_rows<TWOJ, false>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
+                    const double sc = sv[4];
+                    sweep_rows_w<TWOJ>(mb, ck, [&](int l, const cd *u, bool half) {
                         if (!(valid && half)) return;
                         const int q0 = l * (l + 1) / 2;
 #pragma unroll
@@ -702,9 +837,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) void so
                         }
                     });
                 } else {
-                    // dU . dr_e = U M(eta_e): (U M)_m = i p (l-2m) U_m + c_m-1 q U_m-1 - c_m conj(q) U_m+1
-                    const double P = stg[4][si], Qr = stg[5][si], Qi = stg[6][si], Gu = stg[7][si];
-                    sweep_rows<TWOJ, false>(mb, rpq, ck, [&](int l, const cd *u, const cd(*du)[TWOJ + 1], bool half) {
+                    // dU . dr_e = U M(eta_e): (U M)_m = i p (l-2m) U_m + c_m-1 q U_m-1 - c_m conj(q) U_m+1,
+                    // c_m = sqrt((m+1)(l-m)); on W (U_m = s_m W_m, s_m-1 / s_m = sqrt((l-m+1)/m))
+                    // the coefficients become c_m-1 s_m-1 / s_m = l-m+1 and c_m s_m+1 / s_m = m+1
+                    const double P = sv[4], Qr = sv[5], Qi = sv[6], Gu = sv[7];
+                    sweep_rows_w<TWOJ>(mb, ck, [&](int l, const cd *u, bool half) {
                         if (!(valid && half)) return;
                         const int q0 = l * (l + 1) / 2;
 #pragma unroll
@@ -714,12 +851,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) void so
                             double re = fma(-pm, u[m].im, Gu * u[m].re);
                             double im = fma(pm, u[m].re, Gu * u[m].im);
                             if (m >= 1) {
-                                const double cc = sqrt((double)(m * (l - m + 1)));  // c_(m-1)
+                                const double cc = (double)(l - m + 1);
                                 re = fma(cc, Qr * u[m - 1].re - Qi * u[m - 1].im, re);
                                 im = fma(cc, Qr * u[m - 1].im + Qi * u[m - 1].re, im);
                             }
                             if (m < l) {
-                                const double cc = sqrt((double)((m + 1) * (l - m)));  // c_m
+                                const double cc = (double)(m + 1);
                                 re = fma(-cc, Qr * u[m + 1].re + Qi * u[m + 1].im, re);
                                 im = fma(-cc, Qr * u[m + 1].im - Qi * u[m + 1].re, im);
                             }
@@ -730,27 +867,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) void so
                 }
             }
             __syncthreads();  // the next 64 images reuse the stage
-        }
-        // reduce the NG groups' register sums: g < NP store, g >= NP add
+            // reduce the NG groups' register sums: g < NP store (add after
+            // the centre's first chunk), g >= NP add
 #pragma unroll
-        for (int pass = 0; pass < 2; pass++) {
-            const bool mine = g < NG && (pass == 0 ? g < NP : g >= NP);
-            const int pidx = g < NG ? (pass == 0 ? g : g - NP) : 0;
-            if (mine) {
+            for (int pass = 0; pass < 2; pass++) {
+                const bool mine = g < NG && (pass == 0 ? g < NP : g >= NP);
+                const bool store = pass == 0 && b64 == s;
+                const int pidx = g < NG ? (pass == 0 ? g : g - NP) : 0;
+                if (mine) {
 #pragma unroll
-                for (int l = 0; l <= TWOJ; l++) {
-                    if (2 * mb > l) continue;
-                    const int h0 = hoff_of(l) + mb * (l + 1), q0 = l * (l + 1) / 2;
+                    for (int l = 0; l <= TWOJ; l++) {
+                        if (2 * mb > l) continue;
+                        const int h0 = hoff_of(l) + mb * (l + 1), q0 = l * (l + 1) / 2;
 #pragma unroll
-                    for (int m = 0; m <= l; m++) {
-                        if (pass == 0)
-                            part[pidx][h0 + m] = acc[q0 + m];
-                        else
-                            part[pidx][h0 + m] = cadd(part[pidx][h0 + m], acc[q0 + m]);
+                        for (int m = 0; m <= l; m++) {
+                            const cd v = cscale(acc[q0 + m], so4_wscale(l, m, mb, sf, isf));
+                            if (store)
+                                part[pidx][h0 + m] = v;
+                            else
+                                part[pidx][h0 + m] = cadd(part[pidx][h0 + m], v);
+                        }
                     }
                 }
+                __syncthreads();  // (odd NG: array NP - 1 has no second group)
             }
-            __syncthreads();  // (odd NG: array NP - 1 has no second group)
         }
         for (int h = lane; h < NH; h += 64) {
             cd a = cmake(0.0, 0.0);
@@ -763,12 +903,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) void so
                     hm = (h - hoff_of(L)) / (L + 1);
                     hma = (h - hoff_of(L)) % (L + 1);
                 }
-            if (!TANGENT && hm == hma) a.re += fcut0 * site_w[i];  // addself_uarraytot
+            if (!TANGENT && hm == hma) a.re += fcut0 * wself;  // addself_uarraytot
             a = cscale(a, lnorm[l]);                                 // ulisttot *= u_norm
             reinterpret_cast<cd *>(out)[((size_t)(i >> 6) * NH + h) * 64 + (i & 63)] = a;
         }
         __syncthreads();  // the next centre rewrites part
     }
+}
+
+template <int TWOJ, bool TANGENT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) void so4_utot_kernel(
+    int64_t natoms, const int64_t *__restrict__ img_start, const int64_t *__restrict__ img_j,
+    const double *__restrict__ img_r, const double *__restrict__ atom_w, const double *__restrict__ site_w,
+    const double *__restrict__ dr, const int64_t *__restrict__ work_order, const double *rootpq_g,
+    const double *lnorm_g, double rcut, double rfac0, int cutoff_id, double fcut0, double *__restrict__ out) {
+    so4_utot_body<TWOJ, TANGENT, false>(natoms, img_start, img_j, img_r, atom_w, site_w, dr, work_order, rootpq_g,
+                                        lnorm_g, rcut, rfac0, cutoff_id, fcut0, out, nullptr);
+}
+
+// U_tot from the staged per-image records (so4_stage_kernel): without the
+// cutoff and Cayley-Klein evaluation between its sweeps the kernel fits two
+// wavefronts per SIMD.
+template <int TWOJ>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void so4_utot_staged_kernel(
+    int64_t natoms, const int64_t *__restrict__ img_start, const double *__restrict__ site_w,
+    const int64_t *__restrict__ work_order, const double *lnorm_g, double fcut0, const double *__restrict__ stage,
+    double *__restrict__ out) {
+    so4_utot_body<TWOJ, false, true>(natoms, img_start, nullptr, nullptr, nullptr, site_w, nullptr, work_order,
+                                     nullptr, lnorm_g, 0.0, 0.0, 1, fcut0, out, stage);
 }
 
 // B (compute_bi, SNAP.py:1101-1141) from the tile-major U_tot and Z: one
@@ -1652,6 +1814,36 @@ extern "C" int pxff_so4_ui(const pxff_so4_params *p, int64_t natoms, const int64
         }
     }
 #undef PXFF_ACCUM_UI
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_so4_ui_staged(const pxff_so4_params *p, int64_t natoms, const int64_t *d_img_start,
+                                  const int64_t *d_img_j, const double *d_img_r, const double *d_atom_w,
+                                  const double *d_site_w, const int64_t *d_work_order, int64_t cap, double *d_stage,
+                                  double *d_utot, void *stream) {
+    int rc = so4_check_params(p);
+    if (rc) return rc;
+    PXFF_REQUIRE(cap >= 0 && (cap == 0 || d_stage != nullptr), "stage workspace missing");
+    if (natoms <= 0) return 0;
+    // the register-sum kernel is the one with a staged form (2J <= 8);
+    // the other summation modes stage in LDS themselves
+    if (tune_value(PXFF_TUNE_ACCUM_LDSADD) != 4 || p->twol > 8)
+        return pxff_so4_ui(p, natoms, d_img_start, d_img_j, d_img_r, d_atom_w, d_site_w, d_work_order, d_utot,
+                           stream);
+    const So4Dev *T = nullptr;
+    if ((rc = get_so4_tables(p->twol, p->normalize_u, &T))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const int grid = (int)std::min<int64_t>(natoms, (int64_t)num_cus() * tune_value(PXFF_TUNE_ACCUM_WAVES_PER_CU));
+    const double fcut0 = cutoff_f0_host(p->cutoff_id);
+    const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>(div_up(cap, 256), (int64_t)num_cus() * 8));
+    so4_stage_kernel<false><<<sgrid, 256, 0, s>>>(natoms, cap, d_img_start, d_img_j, d_img_r, d_atom_w, nullptr,
+                                                  p->rcut, p->rfac0, p->cutoff_id, d_stage);
+    PXFF_LAUNCH_CHECK();
+    PXFF_DISPATCH_TWOJ(p->twol, if constexpr (TJ <= 8) {
+        so4_utot_staged_kernel<TJ><<<grid, 64, 0, s>>>(natoms, d_img_start, d_site_w, d_work_order, T->lnorm, fcut0,
+                                                      d_stage, d_utot);
+    });
     PXFF_LAUNCH_CHECK();
     return 0;
 }

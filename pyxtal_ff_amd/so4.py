@@ -45,14 +45,23 @@ def _f64(n, dev):
     return torch.empty(max(int(n), 1), dtype=torch.float64, device=dev)
 
 
-def ui(nl, cfg, atom_w, site_w, work_order=None, img_r=None):
+def ui(nl, cfg, atom_w, site_w, work_order=None, img_r=None, staged=True):
     """U_tot half rows, tile-major: [ceil(N/64), nhalf, 64] complex (float64
-    pairs), padding lanes zero."""
+    pairs), padding lanes zero.  staged: the per-image values of the sweeps
+    computed by a kernel of their own first (pxff_so4_ui_staged, a workspace
+    of 6 doubles per image of the list's capacity)."""
     p = nat.ptr
     ntiles = (nl.natoms + 63) // 64
     # every element, padding lanes of the last tile included, is written
     out = torch.empty(max(ntiles * 64 * cfg.nhalf * 2, 1), dtype=torch.float64, device=nl.pos.device)
     r = nl.img_r if img_r is None else img_r.contiguous()
+    if staged:
+        cap = nl.img_capacity
+        stage = torch.empty(max(cap * 6, 1), dtype=torch.float64, device=nl.pos.device)
+        nat.check(nat.lib().pxff_so4_ui_staged(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r),
+                                               p(atom_w), p(site_w), p(work_order), cap, p(stage), p(out),
+                                               nat.stream()), "pxff_so4_ui_staged")
+        return out
     nat.check(nat.lib().pxff_so4_ui(cfg.params, nl.natoms, p(nl.img_start), p(nl.img_j), p(r), p(atom_w),
                                     p(site_w), p(work_order), p(out), nat.stream()), "pxff_so4_ui")
     return out

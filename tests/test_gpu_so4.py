@@ -284,6 +284,49 @@ def _engine(s, lmax=4, rc=4.9, weights=None, rfac0=0.99363):
     return nl, cfg, aw, sw, model, species, state
 
 
+@pytest.mark.parametrize("lmax", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("cutoff,norm", [("cosine", False), ("tanh", True), ("poly3", False)])
+def test_utot_staged_and_coefficient_free_recursion(lmax, cutoff, norm):
+    """U_tot three ways on a two-species cluster with a coincident pair and
+    an isolated atom (no images): the staged records with the coefficient-free
+    W recursion (pxff_so4_ui_staged), the in-kernel staging with the same
+    recursion (pxff_so4_ui), and the reference's recursion with its sqrt(p/q)
+    coefficients (summation mode 0).  They agree to rounding, so the rescaling
+    u = sqrt(m! (l-m)! / (mb! (l-mb)!)) W (a wrong factorial or a missed flip
+    sign fails it) and the staged records are pinned by the reference's
+    recursion."""
+    from pyxtal_ff_amd import autotune
+    from pyxtal_ff_amd.structure import Structure
+    rng = np.random.default_rng(40 + lmax)
+    pos = rng.uniform(0.0, 9.0, size=(40, 3))
+    pos[1] = pos[0]  # coincident: zero weight
+    pos = np.vstack([pos, [[40.0, 40.0, 40.0]]])  # isolated: only the self term
+    sym = ["Si" if k % 3 else "O" for k in range(len(pos))]
+    s = Structure(pos, np.zeros((3, 3)), symbols=sym, pbc=False)
+    nl = DeviceNeighborList([s.positions], [s.cell], [s.pbc], 4.5)
+    cfg = dso4.So4Config(2 * lmax, 4.5, 0.99363, cutoff, normalize_u=norm)
+    w = {"Si": 1.0, "O": 0.8}
+    aw = torch.as_tensor([w[e] for e in sym], dtype=torch.float64, device="cuda")
+    sw = torch.as_tensor([1.0 + 0.1 * (k % 4) for k in range(len(sym))], dtype=torch.float64, device="cuda")
+    staged = dso4.ui(nl, cfg, aw, sw).cpu().numpy()
+    inkernel = dso4.ui(nl, cfg, aw, sw, staged=False).cpu().numpy()
+    old = autotune.get_knob(autotune.ACCUM_LDSADD)
+    try:
+        autotune.set_knob(autotune.ACCUM_LDSADD, 0)
+        ref = dso4.ui(nl, cfg, aw, sw, staged=False).cpu().numpy()
+    finally:
+        autotune.set_knob(autotune.ACCUM_LDSADD, old)
+    fl = 1e-12 * float(np.abs(ref).max())
+    _print("2J", 2 * lmax, cutoff, "staged/ref", max_rel_elem(staged, ref, fl), "staged==inkernel",
+           bool(np.array_equal(staged, inkernel)))
+    assert rel_err(staged, ref) < 1e-14 and max_rel_elem(staged, ref, fl) < 1e-3
+    assert rel_err(inkernel, ref) < 1e-14
+    # the isolated atom's U_tot is w_self fc(0) u_norm on the diagonal
+    n = len(sym) - 1
+    h = staged.reshape(-1, cfg.nhalf, 64, 2)[n // 64, :, n % 64, :]
+    assert np.count_nonzero(h) > 0 and np.all(h[:, 1] == 0.0)
+
+
 @pytest.mark.parametrize("lmax", [0, 1, 2, 3, 4, 5])
 def test_dbdr_centres_equals_per_image(lmax):
     """The per-centre dB/dr kernel (generator form, Z staged per level from
