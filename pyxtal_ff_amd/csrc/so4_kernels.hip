@@ -770,7 +770,14 @@ __device__ __forceinline__ void so4_utot_body(
     }
     __syncthreads();
     for (int64_t c = blockIdx.x; c < natoms; c += gridDim.x) {
-        const int64_t i = work_order ? work_order[c] : c;
+        // without a work order, the eight centres of a 64-atom tile whose
+        // elements share a 128-byte line of the tile-major output run on one
+        // XCD (workgroups go to the XCDs round-robin, and the grid is a
+        // multiple of 8 or a single round): swap the 3-bit fields c % 8 and
+        // (c / 8) % 8, a bijection on every whole tile (a partial last tile
+        // keeps the identity)
+        const int64_t cx = (c & ~(int64_t)63) | ((c & 7) << 3) | ((c >> 3) & 7);
+        const int64_t i = work_order ? work_order[c] : ((c | 63) < natoms ? cx : c);
         const int64_t s = img_start[i], e = img_start[i + 1];
         const double wself = TANGENT ? 0.0 : site_w[i];  // loaded ahead of the sweeps
         if (s == e) {  // no image: the partial arrays start (and stay) at zero
