@@ -120,10 +120,13 @@ int pxff_nlist_fill(const double *d_pos, int64_t natoms, const int64_t *d_atom_s
  * (j, i, -T) for image (i, j, T) -- the list is exactly symmetric, which
  * the deterministic force gather relies on.  nimg may be an upper bound
  * (the image capacity): images at or beyond d_img_start[natoms] are
- * skipped.  A missing reverse (cannot happen for this construction) sets
- * d_flag[0] (int32, zeroed by the call); with d_flag NULL nothing is
- * checked (one launch fewer; the call is deterministic, so a later call
- * with a flag checks the same result). */
+ * skipped.  With d_flag, each reverse is searched by (j, T) and a missing
+ * one (cannot happen for this construction) sets d_flag[0] (int32, zeroed
+ * by the call).  With d_flag NULL nothing is checked and the symmetry is
+ * used instead: the run of images (i -> j) and the run (j -> i) have the
+ * same length and opposite order in T, so only the start of the run
+ * (j -> i) is searched, by neighbour index (one launch fewer; for a
+ * symmetric list the same indices as the checked call). */
 int pxff_nlist_rev(int64_t natoms, int64_t nimg, const int64_t *d_img_start, const int64_t *d_img_atom,
                    const int64_t *d_img_j, const int32_t *d_img_T, int64_t *d_img_rev, int32_t *d_flag,
                    void *stream);

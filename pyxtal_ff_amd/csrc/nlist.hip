@@ -373,6 +373,74 @@ __global__ void nl_rev_kernel(int64_t natoms, int64_t nimg, const int64_t *__res
     }
 }
 
+// Reverse images without searching on T.  Within centre i's list the images
+// with neighbour j form a run sorted by T; in j's list the images with
+// neighbour i form a run of the same length sorted by T' = -T, and negation
+// reverses the lexicographic order, so the reverse of the k-th image of the
+// run (i -> j) of length n is the (n-1-k)-th of the run (j -> i).  One
+// wavefront per centre, lanes over its images 64 at a time: k and n come
+// from ballots over the lanes' neighbours, the start of the run (j -> i)
+// from a lower-bound search of i in j's neighbour indices alone (lanes of one
+// run search the same addresses).  A run that crosses a 64-image chunk
+// (centres with more than 64 images) is searched with T as in
+// nl_rev_kernel.  Assumes the symmetric list every build produces
+// (nl_rev_kernel with its flag checks it); an index outside the images in
+// use (a list cut at its capacity, which is rebuilt before use) is replaced
+// by the image itself.
+__global__ __launch_bounds__(64) void nl_rev_runs_kernel(int64_t natoms, int64_t nimg,
+                                                         const int64_t *__restrict__ img_start,
+                                                         const int64_t *__restrict__ img_j,
+                                                         const int32_t *__restrict__ img_T,
+                                                         int64_t *__restrict__ img_rev) {
+    const int64_t i = blockIdx.x;
+    if (i >= natoms) return;
+    const int lane = threadIdx.x;
+    const int64_t ntot = img_start[natoms] < nimg ? img_start[natoms] : nimg;
+    const int64_t s = img_start[i];
+    const int64_t e = img_start[i + 1] < ntot ? img_start[i + 1] : ntot;
+    for (int64_t base = s; base < e; base += 64) {
+        const int64_t idx = base + lane;
+        const bool in = idx < e;
+        const int64_t j = in ? img_j[idx] : -1;
+        const int64_t jp = __shfl_up(j, 1, 64), jn = __shfl_down(j, 1, 64);
+        const bool first = in && (lane == 0 || jp != j);
+        const bool last = in && (lane == 63 || jn != j);
+        const uint64_t fm = __ballot(first), lm = __ballot(last);
+        if (!in) continue;
+        // run start: the highest first lane <= lane; run end: the lowest
+        // last lane >= lane (both exist for a lane in use)
+        const int st = 63 - __clzll(fm & (~0ull >> (63 - lane)));
+        const int en = __ffsll(lm & (~0ull << lane)) - 1;
+        const bool cross = (st == 0 && base > s && img_j[base - 1] == j) ||
+                           (en == 63 && base + 64 < e && img_j[base + 64] == j);
+        const int64_t jlo = img_start[j], jhi = img_start[j + 1] < ntot ? img_start[j + 1] : ntot;
+        int64_t r;
+        if (!cross) {
+            int64_t lo = jlo, hi = jhi;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (img_j[mid] < i)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            r = lo + (en - lane);
+        } else {
+            const int32_t T[3] = {-img_T[3 * idx], -img_T[3 * idx + 1], -img_T[3 * idx + 2]};
+            int64_t lo = jlo, hi = jhi;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (key_less(img_j[mid], img_T + 3 * mid, i, T))
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            r = lo;
+        }
+        img_rev[idx] = (r >= 0 && r < ntot) ? r : idx;
+    }
+}
+
 __global__ void nl_row_rev_kernel(int64_t nrows, const int64_t *__restrict__ seq, const int64_t *__restrict__ row_start,
                                   int64_t *__restrict__ row_rev, int32_t *__restrict__ flag) {
     const int64_t rw = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -778,8 +846,13 @@ extern "C" int pxff_nlist_rev(int64_t natoms, int64_t nimg, const int64_t *d_img
         nl_zero_i32_kernel<<<1, 64, 0, s>>>(d_flag, 1);
         PXFF_LAUNCH_CHECK();
     }
-    nl_rev_kernel<<<div_up(nimg, 256), 256, 0, s>>>(natoms, nimg, d_img_start, d_img_atom, d_img_j, d_img_T, d_img_rev,
-                                                    d_flag);
+    if (d_flag) {
+        // the checking search (verify)
+        nl_rev_kernel<<<div_up(nimg, 256), 256, 0, s>>>(natoms, nimg, d_img_start, d_img_atom, d_img_j, d_img_T,
+                                                        d_img_rev, d_flag);
+    } else {
+        nl_rev_runs_kernel<<<(unsigned)natoms, 64, 0, s>>>(natoms, nimg, d_img_start, d_img_j, d_img_T, d_img_rev);
+    }
     PXFF_LAUNCH_CHECK();
     return 0;
 }

@@ -112,6 +112,32 @@ def test_nlist_dense_centres_serial_path(oracle_mod):
     assert np.bincount(gi).max() > 128
     rev = nl.img_rev[: nl.nimg].cpu().numpy()
     assert np.array_equal(gr[rev], -gr) and np.array_equal(gj[rev], gi)
+    # the build's reverse images (runs (i -> j) reversed into (j -> i), no
+    # search on T; runs here cross the 64-image chunks) equal the checked
+    # search by (j, T) that verify() runs
+    fast = nl.img_rev[: nl.nimg].clone()
+    nl.verify()
+    assert torch.equal(nl.img_rev[: nl.nimg], fast)
+
+
+def test_nlist_reverse_runs_equal_checked_search():
+    """Reverse images from the run reversal equal the checked search on a
+    two-species random cell, a small cell with several images per pair and
+    its self-images (runs of equal neighbour longer than one), and a batch of
+    both."""
+    rng = np.random.default_rng(8)
+    a = workloads.random_sio_structure(rng, 40, 60)
+    small = (np.array([[0.0, 0.0, 0.0], [1.1, 0.9, 1.4], [2.0, 2.3, 0.4]]), np.eye(3) * 3.1, np.array([True] * 3))
+    for cases, rc in (([(a.positions, a.cell, a.pbc)], 5.0), ([small], 6.0),
+                      ([(a.positions, a.cell, a.pbc), small], 5.5)):
+        nl = DeviceNeighborList([c[0] for c in cases], [c[1] for c in cases], [c[2] for c in cases], rc)
+        n = nl.nimg
+        fast = nl.img_rev[:n].clone()
+        nl.verify()
+        assert torch.equal(nl.img_rev[:n], fast)
+        gi, gj, gT, gr = [t.cpu().numpy() for t in nl.images()]
+        rv = fast.cpu().numpy()
+        assert np.array_equal(gr[rv], -gr) and np.array_equal(gj[rv], gi) and np.array_equal(rv[rv], np.arange(n))
 
 
 def test_nlist_unwrapped_positions_large_translations(oracle_mod):
