@@ -1431,10 +1431,6 @@ __global__ __launch_bounds__(64) void so4_dbdr_centre_kernel(
             const int nrow = l / 2 + 1;
             for (int it = lane; it < ne * nbl; it += 64) {
                 const int bk = it / ne, gg = it - bk * ne;
-                double au = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-                for (int row = 0; row < nrow; row++)
-                    generator_dots<TWOJ>(l, &Ul[gg][row * (l + 1)], &Zl[bk * nloc + row * (l + 1)], au, a1, a2, a3);
-                const double *S = sc[gg];
                 const int blk = b0 + bk;
                 double *o = dbdr + (size_t)(e0 + gg) * nb * 3;
                 int tb[3];
@@ -1444,12 +1440,25 @@ __global__ __launch_bounds__(64) void so4_dbdr_centre_kernel(
                     tb[q] = blk_tb[3 * blk + q];
                     tf[q] = blk_tf[3 * blk + q] * nrm2;
                 }
+                // the targets' current values, loaded ahead of the sums so
+                // that their latency overlaps them (a block's targets are
+                // distinct b, duplicates being merged in the table, so none
+                // of these loads can see one of the stores below)
+                double old[3][3];
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+#pragma unroll
+                    for (int k = 0; k < 3; k++) old[q][k] = tb[q] >= 0 ? o[3 * tb[q] + k] : 0.0;
+                double au = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+                for (int row = 0; row < nrow; row++)
+                    generator_dots<TWOJ>(l, &Ul[gg][row * (l + 1)], &Zl[bk * nloc + row * (l + 1)], au, a1, a2, a3);
+                const double *S = sc[gg];
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
                     const double pk = S[k] * a1 + S[3 + k] * a2 + S[6 + k] * a3 + S[9 + k] * au;
 #pragma unroll
                     for (int q = 0; q < 3; q++)
-                        if (tb[q] >= 0) o[3 * tb[q] + k] += tf[q] * pk;
+                        if (tb[q] >= 0) o[3 * tb[q] + k] = fma(tf[q], pk, old[q][k]);
                 }
             }
             __syncthreads();
