@@ -800,29 +800,18 @@ __device__ __forceinline__ void so4_utot_body(
 #pragma unroll
             for (int q = 0; q < NSLOT; q++) acc[q] = cmake(0.0, 0.0);
             constexpr int NV = TANGENT ? 9 : 5;
-            // STAGED: the group's record of the next pass, loaded one pass
-            // ahead (the loads are in flight during the current sweeps)
-            double nv[NV];
-            auto fetch = [&](int64_t bs) {
-                const int64_t im = bs + g;
-                const bool ok = g < NG && im < e64;
-                const double *rec = stage + (ok ? im : b64) * so4_stage_stride<TANGENT>();
-#pragma unroll
-                for (int q = 0; q < NV; q++) nv[q] = rec[q];
-                if (!ok) {
-#pragma unroll
-                    for (int q = 0; q < NV; q++) nv[q] = 0.0;
-                }
-            };
-            if constexpr (STAGED) fetch(b64);
             for (int64_t base = b64; base < e64; base += NG) {
                 const int64_t img = base + g;
                 const int si = (g < NG && img < e64) ? (int)(img - b64) : 0;
                 double sv[NV];
                 if constexpr (STAGED) {
+                    // the group's record (loaded where it is used: a copy
+                    // loaded one pass ahead costs registers, and its spills
+                    // cost more than the wait: 39 against 36 us)
+                    const bool ok = g < NG && img < e64;
+                    const double *rec = stage + (ok ? img : b64) * so4_stage_stride<TANGENT>();
 #pragma unroll
-                    for (int q = 0; q < NV; q++) sv[q] = nv[q];
-                    if (base + NG < e64) fetch(base + NG);
+                    for (int q = 0; q < NV; q++) sv[q] = ok ? rec[q] : 0.0;
                 } else {
 #pragma unroll
                     for (int q = 0; q < NV; q++) sv[q] = stg[q][si];
