@@ -65,6 +65,8 @@ int pxff_tune_get(int knob); /* -1 for an unknown knob */
  *   PXFF_WS_SO3_TABLE    d_table of pxff_so3_radial_table (its d_scratch is
  *                        the same size), params = pxff_so3_params
  *   PXFF_WS_SO4_UI       d_stage of pxff_so4_ui_staged, n1 = image capacity
+ *   PXFF_WS_NLIST_SLOTS  the slot arrays of pxff_nlist_count_slots / _fill_slots
+ *                        (44 bytes per slot entry), n0 = natoms, n1 = slot
  * The launch knobs below are the only other state: process-wide atomics
  * read once per launch, so calls on several threads and streams are
  * reentrant. */
@@ -74,6 +76,7 @@ int pxff_tune_get(int knob); /* -1 for an unknown knob */
 #define PXFF_WS_SO3_PAIRS 3
 #define PXFF_WS_SO3_TABLE 4
 #define PXFF_WS_SO4_UI 5
+#define PXFF_WS_NLIST_SLOTS 6
 int pxff_workspace_size(int kind, const void *params, int64_t n0, int64_t n1, int64_t *bytes);
 
 /* Copy the calling thread's last error message into buf (NUL-terminated). */
@@ -104,6 +107,22 @@ int pxff_nlist_count(const double *d_pos, int64_t natoms, const int64_t *d_atom_
                      int32_t *d_bin_count, int32_t *d_bin_start, int32_t *d_atom_bin, int32_t *d_wrap,
                      int32_t *d_bin_atoms, int64_t *d_img_count, void *stream);
 
+/* Phase 1 with the images kept: pxff_nlist_count, and every centre with at
+ * most `slot` images (1..128) also ranks them and writes them in (j, T)
+ * order into its slot -- d_slot_j [natoms*slot] int64, d_slot_T
+ * [natoms*slot*3] int32, d_slot_r [natoms*slot*3] double, at [i*slot,
+ * i*slot + count) -- with its seq-row count in d_row_count [natoms].
+ * pxff_nlist_fill_slots then copies the slots into place and fills the other
+ * centres as pxff_nlist_fill does, so most centres are enumerated once per
+ * build instead of twice.  Workspace: pxff_workspace_size
+ * PXFF_WS_NLIST_SLOTS (n0 = natoms, n1 = slot). */
+int pxff_nlist_count_slots(const double *d_pos, int64_t natoms, const int64_t *d_atom_struct,
+                           const int64_t *d_atom_start, const double *d_cells, const double *d_inv_cells,
+                           const int32_t *d_pbc, const int32_t *d_geom, int64_t nbins_total, double rc,
+                           int32_t *d_bin_count, int32_t *d_bin_start, int32_t *d_atom_bin, int32_t *d_wrap,
+                           int32_t *d_bin_atoms, int64_t *d_img_count, int slot, int64_t *d_slot_j,
+                           int32_t *d_slot_T, double *d_slot_r, int64_t *d_row_count, void *stream);
+
 /* Phase 2: fill images (sorted per centre by (j, T)) and count seq rows.
  *   d_img_start [natoms+1] int64 = exclusive scan of d_img_count.
  *   Outputs: d_img_atom [nimg] int64 (centre), d_img_j [nimg] int64,
@@ -115,6 +134,18 @@ int pxff_nlist_fill(const double *d_pos, int64_t natoms, const int64_t *d_atom_s
                     const int32_t *d_wrap, const int32_t *d_bin_atoms, const int64_t *d_img_start,
                     int64_t *d_img_atom, int64_t *d_img_j, int32_t *d_img_T, double *d_img_r,
                     int64_t *d_row_count, void *stream);
+
+/* Phase 2 after pxff_nlist_count_slots (same slot and slot arrays, and its
+ * d_img_count): centres whose images are in their slot are copied into
+ * place (their seq-row counts are written already), the others filled as
+ * by pxff_nlist_fill.  Same output as pxff_nlist_count + pxff_nlist_fill. */
+int pxff_nlist_fill_slots(const double *d_pos, int64_t natoms, const int64_t *d_atom_struct,
+                          const int64_t *d_atom_start, const double *d_cells, const int32_t *d_pbc,
+                          const int32_t *d_geom, double rc, const int32_t *d_bin_start, const int32_t *d_atom_bin,
+                          const int32_t *d_wrap, const int32_t *d_bin_atoms, const int64_t *d_img_start,
+                          const int64_t *d_img_count, int slot, const int64_t *d_slot_j, const int32_t *d_slot_T,
+                          const double *d_slot_r, int64_t *d_img_atom, int64_t *d_img_j, int32_t *d_img_T,
+                          double *d_img_r, int64_t *d_row_count, void *stream);
 
 /* Phase 3: reverse images.  d_img_rev [nimg] int64 = index of image
  * (j, i, -T) for image (i, j, T) -- the list is exactly symmetric, which

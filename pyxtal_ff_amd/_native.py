@@ -44,6 +44,10 @@ _SIGS = {
     "pxff_workspace_size": ([ctypes.c_int, P, I64, I64, P], I32),
     "pxff_nlist_count": ([P, I64, P, P, P, P, P, P, I64, F64, P, P, P, P, P, P, P], I32),
     "pxff_nlist_fill": ([P, I64, P, P, P, P, P, F64, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "pxff_nlist_count_slots": ([P, I64, P, P, P, P, P, P, I64, F64, P, P, P, P, P, P, ctypes.c_int, P, P, P, P, P],
+                               I32),
+    "pxff_nlist_fill_slots": ([P, I64, P, P, P, P, P, F64, P, P, P, P, P, P, ctypes.c_int, P, P, P, P, P, P, P, P, P],
+                              I32),
     "pxff_nlist_rev": ([I64, I64, P, P, P, P, P, P, P], I32),
     "pxff_nlist_rows": ([I64, I64, I64, P, P, P, P, P, P, P, P, P, P], I32),
     "pxff_so4_sizes": ([ctypes.c_int, P, P, P, P], I32),

@@ -257,7 +257,7 @@ __device__ inline bool key_less(int64_t ja, const int32_t *Ta, int64_t jb, const
 // Images of centre i by one thread, sorted in place in the output arrays
 // (insertion sort by (j, T)): the path of centres with more images than
 // the wavefront fill's LDS list holds.
-__device__ inline void nl_fill_serial(int64_t i, const double *__restrict__ pos,
+__device__ __forceinline__ void nl_fill_serial(int64_t i, const double *__restrict__ pos,
                                       const int64_t *__restrict__ atom_struct, const double *__restrict__ cells,
                                       const int32_t *__restrict__ pbc, const int32_t *__restrict__ geom, double rc2,
                                       const int32_t *__restrict__ bin_start, const int32_t *__restrict__ atom_bin,
@@ -482,7 +482,7 @@ struct NlHit {
 constexpr int NL_U = 4;  // candidate rounds per step, their loads in flight together
 
 template <bool FILL>
-__device__ inline int nl_wave_enumerate(int64_t i, int lane, const double *__restrict__ pos,
+__device__ __forceinline__ int nl_wave_enumerate(int64_t i, int lane, const double *__restrict__ pos,
                                         const int64_t *__restrict__ atom_struct, const double *__restrict__ cells,
                                         const int32_t *__restrict__ pbc, const int32_t *__restrict__ geom, double rc2,
                                         const int32_t *__restrict__ bin_start, const int32_t *__restrict__ atom_bin,
@@ -617,33 +617,14 @@ __device__ inline bool hit_less(const NlHit &a, const NlHit &b) {
     return a.T[2] < b.T[2];
 }
 
-__global__ __launch_bounds__(64) void nl_fill_wave_kernel(
-    int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
-    const double *__restrict__ cells, const int32_t *__restrict__ pbc, const int32_t *__restrict__ geom, double rc2,
-    const int32_t *__restrict__ bin_start, const int32_t *__restrict__ atom_bin, const int32_t *__restrict__ wrap,
-    const int32_t *__restrict__ bin_atoms, const int64_t *__restrict__ img_start, int64_t *__restrict__ img_atom,
-    int64_t *__restrict__ img_j, int32_t *__restrict__ img_T, double *__restrict__ img_r,
-    int64_t *__restrict__ row_count) {
-    __shared__ int s_pref[64];
-    __shared__ int s_info[256];
-    __shared__ NlHit hits[NL_KMAX];
-    __shared__ uint64_t keys[NL_KMAX];
-    const int lane = threadIdx.x;
-    const int64_t i = blockIdx.x;
-    if (i >= natoms) return;
-    const int64_t s0 = img_start[i];
-    const int k = (int)(img_start[i + 1] - s0);
-    if (k > NL_KMAX) {
-        // more images than the LDS list holds: one lane fills the centre
-        if (lane == 0)
-            nl_fill_serial(i, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap, bin_atoms, img_start,
-                           img_atom, img_j, img_T, img_r, row_count);
-        return;
-    }
-    const int c = nl_wave_enumerate<true>(i, lane, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap,
-                                          bin_atoms, s_pref, s_info, hits);
-    (void)c;
-    __syncthreads();
+// Ranks the centre's first k LDS hits by (j, T) and hands each hit of rank
+// < kcap to put(rank, hit); returns (in every lane) the centre's seq-row
+// count: the distinct neighbours among them, plus one for the (i, i) row
+// when i is not its own neighbour.  Starts with the wavefront's LDS keys
+// (every lane calls it).
+template <typename Put>
+__device__ __forceinline__ int64_t nl_rank_hits(int64_t i, int k, int kcap, int lane, const NlHit *hits, uint64_t *keys,
+                                       Put &&put) {
     // sort keys (j, T0, T1, T2) packed in 64 bits, T_d + 128 in 8 bits each,
     // when every |T_d| < 128 (the lexicographic order of the packed key is
     // that of hit_less); otherwise the comparison of the fields
@@ -681,7 +662,41 @@ __global__ __launch_bounds__(64) void nl_fill_wave_kernel(
                 }
             }
         }
-        if (rank >= k) continue;  // the range was clamped to the image capacity
+        if (rank >= kcap) continue;  // (the fill's range was clamped to the image capacity)
+        put(rank, h);
+        if (first) firsts++;
+        if (h.j == i) self = true;
+    }
+    for (int o = 32; o > 0; o >>= 1) firsts += __shfl_xor(firsts, o);
+    const bool any_self = __ballot(self) != 0;
+    return firsts + (any_self ? 0 : 1);
+}
+
+// The fill of one centre by its wavefront: enumerate into LDS, rank, write
+// the images at img_start[i] in (j, T) order and the seq-row count.
+__device__ __forceinline__ void nl_fill_centre(int64_t i, int lane, const double *__restrict__ pos,
+                                      const int64_t *__restrict__ atom_struct, const double *__restrict__ cells,
+                                      const int32_t *__restrict__ pbc, const int32_t *__restrict__ geom, double rc2,
+                                      const int32_t *__restrict__ bin_start, const int32_t *__restrict__ atom_bin,
+                                      const int32_t *__restrict__ wrap, const int32_t *__restrict__ bin_atoms,
+                                      const int64_t *__restrict__ img_start, int64_t *__restrict__ img_atom,
+                                      int64_t *__restrict__ img_j, int32_t *__restrict__ img_T,
+                                      double *__restrict__ img_r, int64_t *__restrict__ row_count, int *s_pref,
+                                      int *s_info, NlHit *hits, uint64_t *keys) {
+    const int64_t s0 = img_start[i];
+    const int k = (int)(img_start[i + 1] - s0);
+    if (k > NL_KMAX) {
+        // more images than the LDS list holds: one lane fills the centre
+        if (lane == 0)
+            nl_fill_serial(i, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap, bin_atoms, img_start,
+                           img_atom, img_j, img_T, img_r, row_count);
+        return;
+    }
+    const int c = nl_wave_enumerate<true>(i, lane, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap,
+                                          bin_atoms, s_pref, s_info, hits);
+    (void)c;
+    __syncthreads();
+    const int64_t nrow = nl_rank_hits(i, k, k, lane, hits, keys, [&](int rank, const NlHit &h) {
         const int64_t d = s0 + rank;
         img_atom[d] = i;
         img_j[d] = h.j;
@@ -691,12 +706,100 @@ __global__ __launch_bounds__(64) void nl_fill_wave_kernel(
         img_r[3 * d] = h.r[0];
         img_r[3 * d + 1] = h.r[1];
         img_r[3 * d + 2] = h.r[2];
-        if (first) firsts++;
-        if (h.j == i) self = true;
+    });
+    if (lane == 0) row_count[i] = nrow;
+}
+
+__global__ __launch_bounds__(64) void nl_fill_wave_kernel(
+    int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
+    const double *__restrict__ cells, const int32_t *__restrict__ pbc, const int32_t *__restrict__ geom, double rc2,
+    const int32_t *__restrict__ bin_start, const int32_t *__restrict__ atom_bin, const int32_t *__restrict__ wrap,
+    const int32_t *__restrict__ bin_atoms, const int64_t *__restrict__ img_start, int64_t *__restrict__ img_atom,
+    int64_t *__restrict__ img_j, int32_t *__restrict__ img_T, double *__restrict__ img_r,
+    int64_t *__restrict__ row_count) {
+    __shared__ int s_pref[64];
+    __shared__ int s_info[256];
+    __shared__ NlHit hits[NL_KMAX];
+    __shared__ uint64_t keys[NL_KMAX];
+    const int64_t i = blockIdx.x;
+    if (i >= natoms) return;
+    nl_fill_centre(i, threadIdx.x, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap, bin_atoms,
+                   img_start, img_atom, img_j, img_T, img_r, row_count, s_pref, s_info, hits, keys);
+}
+
+// The count with the images kept: every centre's wavefront enumerates and
+// counts as nl_count_wave_kernel does, and a centre with at most `slot`
+// images also ranks them and writes them, in (j, T) order, into its slot
+// (slot_j / slot_T / slot_r at [i * slot, i * slot + count)) together with
+// its seq-row count; nl_fill_slots_kernel then copies the slots into place
+// after the scan, so those centres are enumerated once per build.
+__global__ __launch_bounds__(64) void nl_count_slots_kernel(
+    int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
+    const double *__restrict__ cells, const int32_t *__restrict__ pbc, const int32_t *__restrict__ geom, double rc2,
+    const int32_t *__restrict__ bin_start, const int32_t *__restrict__ atom_bin, const int32_t *__restrict__ wrap,
+    const int32_t *__restrict__ bin_atoms, int64_t *__restrict__ img_count, int slot, int64_t *__restrict__ slot_j,
+    int32_t *__restrict__ slot_T, double *__restrict__ slot_r, int64_t *__restrict__ row_count) {
+    __shared__ int s_pref[64];
+    __shared__ int s_info[256];
+    __shared__ NlHit hits[NL_KMAX];
+    __shared__ uint64_t keys[NL_KMAX];
+    const int lane = threadIdx.x;
+    const int64_t i = blockIdx.x;
+    if (i >= natoms) return;  // uniform for the whole workgroup
+    const int c = nl_wave_enumerate<true>(i, lane, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap,
+                                          bin_atoms, s_pref, s_info, hits);
+    if (lane == 0) img_count[i] = c;
+    if (c > slot) return;  // uniform (the count is the wavefront's); the fill enumerates this centre
+    __syncthreads();
+    const int64_t nrow = nl_rank_hits(i, c, c, lane, hits, keys, [&](int rank, const NlHit &h) {
+        const int64_t d = i * (int64_t)slot + rank;
+        slot_j[d] = h.j;
+        slot_T[3 * d] = h.T[0];
+        slot_T[3 * d + 1] = h.T[1];
+        slot_T[3 * d + 2] = h.T[2];
+        slot_r[3 * d] = h.r[0];
+        slot_r[3 * d + 1] = h.r[1];
+        slot_r[3 * d + 2] = h.r[2];
+    });
+    if (lane == 0) row_count[i] = nrow;
+}
+
+// The fill after nl_count_slots_kernel: a centre whose images are in its
+// slot copies them into place (its seq-row count is written already); the
+// others are filled as by nl_fill_wave_kernel.
+__global__ __launch_bounds__(64) void nl_fill_slots_kernel(
+    int64_t natoms, const double *__restrict__ pos, const int64_t *__restrict__ atom_struct,
+    const double *__restrict__ cells, const int32_t *__restrict__ pbc, const int32_t *__restrict__ geom, double rc2,
+    const int32_t *__restrict__ bin_start, const int32_t *__restrict__ atom_bin, const int32_t *__restrict__ wrap,
+    const int32_t *__restrict__ bin_atoms, const int64_t *__restrict__ img_start, const int64_t *__restrict__ img_count,
+    int slot, const int64_t *__restrict__ slot_j, const int32_t *__restrict__ slot_T,
+    const double *__restrict__ slot_r, int64_t *__restrict__ img_atom, int64_t *__restrict__ img_j,
+    int32_t *__restrict__ img_T, double *__restrict__ img_r, int64_t *__restrict__ row_count) {
+    __shared__ int s_pref[64];
+    __shared__ int s_info[256];
+    __shared__ NlHit hits[NL_KMAX];
+    __shared__ uint64_t keys[NL_KMAX];
+    const int lane = threadIdx.x;
+    const int64_t i = blockIdx.x;
+    if (i >= natoms) return;
+    if (img_count[i] <= slot) {
+        const int64_t s0 = img_start[i];
+        const int k = (int)(img_start[i + 1] - s0);  // <= the count (clamped to the image capacity)
+        for (int e = lane; e < k; e += 64) {
+            const int64_t d = s0 + e, q = i * (int64_t)slot + e;
+            img_atom[d] = i;
+            img_j[d] = slot_j[q];
+            img_T[3 * d] = slot_T[3 * q];
+            img_T[3 * d + 1] = slot_T[3 * q + 1];
+            img_T[3 * d + 2] = slot_T[3 * q + 2];
+            img_r[3 * d] = slot_r[3 * q];
+            img_r[3 * d + 1] = slot_r[3 * q + 1];
+            img_r[3 * d + 2] = slot_r[3 * q + 2];
+        }
+        return;
     }
-    for (int o = 32; o > 0; o >>= 1) firsts += __shfl_xor(firsts, o);
-    const bool any_self = __ballot(self) != 0;
-    if (lane == 0) row_count[i] = firsts + (any_self ? 0 : 1);
+    nl_fill_centre(i, lane, pos, atom_struct, cells, pbc, geom, rc2, bin_start, atom_bin, wrap, bin_atoms, img_start,
+                   img_atom, img_j, img_T, img_r, row_count, s_pref, s_info, hits, keys);
 }
 
 // Zeroes n ints.  Used instead of a memset so that a build can be captured
@@ -782,25 +885,17 @@ extern "C" int pxff_scan_i64_capped(const int64_t *d_in, int64_t n, int64_t cap,
     return 0;
 }
 
-extern "C" int pxff_nlist_count(const double *d_pos, int64_t natoms, const int64_t *d_atom_struct,
-                                const int64_t *d_atom_start, const double *d_cells, const double *d_inv_cells,
-                                const int32_t *d_pbc, const int32_t *d_geom, int64_t nbins_total, double rc,
-                                int32_t *d_bin_count, int32_t *d_bin_start, int32_t *d_atom_bin, int32_t *d_wrap,
-                                int32_t *d_bin_atoms, int64_t *d_img_count, void *stream) {
-    (void)d_atom_start;
-    PXFF_REQUIRE(rc > 0, "rc must be > 0");
-    PXFF_REQUIRE(nbins_total > 0, "nbins_total must be > 0");
-    if (natoms <= 0) return 0;
-    hipStream_t s = (hipStream_t)stream;
+// Binning of pxff_nlist_count: the atoms' bins and wraps, the bin starts
+// and the bin members.
+static int nl_bin_launch(const double *d_pos, int64_t natoms, const int64_t *d_atom_struct, const double *d_inv_cells,
+                         const int32_t *d_pbc, const int32_t *d_geom, int64_t nbins_total, int32_t *d_bin_count,
+                         int32_t *d_bin_start, int32_t *d_atom_bin, int32_t *d_wrap, int32_t *d_bin_atoms,
+                         hipStream_t s) {
     if (natoms <= NL_SMALL_ATOMS && nbins_total <= NL_SMALL_BINS) {
         nl_bin_place_kernel<<<div_up(natoms, 256), 256, 0, s>>>(natoms, d_pos, d_atom_struct, d_inv_cells, d_pbc,
                                                                 d_geom, d_atom_bin, d_wrap);
         PXFF_LAUNCH_CHECK();
         nl_bin_small_kernel<<<1, 1024, 0, s>>>(natoms, (int)nbins_total, d_atom_bin, d_bin_start, d_bin_atoms);
-        PXFF_LAUNCH_CHECK();
-        nl_count_wave_kernel<<<(unsigned)natoms, 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom,
-                                                               rc * rc, d_bin_start, d_atom_bin, d_wrap, d_bin_atoms,
-                                                               d_img_count);
         PXFF_LAUNCH_CHECK();
         return 0;
     }
@@ -815,8 +910,49 @@ extern "C" int pxff_nlist_count(const double *d_pos, int64_t natoms, const int64
     PXFF_LAUNCH_CHECK();
     nl_bin_fill_kernel<<<div_up(natoms, 256), 256, 0, s>>>(natoms, d_atom_bin, d_bin_start, d_bin_count, d_bin_atoms);
     PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_nlist_count(const double *d_pos, int64_t natoms, const int64_t *d_atom_struct,
+                                const int64_t *d_atom_start, const double *d_cells, const double *d_inv_cells,
+                                const int32_t *d_pbc, const int32_t *d_geom, int64_t nbins_total, double rc,
+                                int32_t *d_bin_count, int32_t *d_bin_start, int32_t *d_atom_bin, int32_t *d_wrap,
+                                int32_t *d_bin_atoms, int64_t *d_img_count, void *stream) {
+    (void)d_atom_start;
+    PXFF_REQUIRE(rc > 0, "rc must be > 0");
+    PXFF_REQUIRE(nbins_total > 0, "nbins_total must be > 0");
+    if (natoms <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    int rc_ = nl_bin_launch(d_pos, natoms, d_atom_struct, d_inv_cells, d_pbc, d_geom, nbins_total, d_bin_count,
+                            d_bin_start, d_atom_bin, d_wrap, d_bin_atoms, s);
+    if (rc_) return rc_;
     nl_count_wave_kernel<<<(unsigned)natoms, 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom, rc * rc,
                                                            d_bin_start, d_atom_bin, d_wrap, d_bin_atoms, d_img_count);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_nlist_count_slots(const double *d_pos, int64_t natoms, const int64_t *d_atom_struct,
+                                      const int64_t *d_atom_start, const double *d_cells, const double *d_inv_cells,
+                                      const int32_t *d_pbc, const int32_t *d_geom, int64_t nbins_total, double rc,
+                                      int32_t *d_bin_count, int32_t *d_bin_start, int32_t *d_atom_bin,
+                                      int32_t *d_wrap, int32_t *d_bin_atoms, int64_t *d_img_count, int slot,
+                                      int64_t *d_slot_j, int32_t *d_slot_T, double *d_slot_r, int64_t *d_row_count,
+                                      void *stream) {
+    (void)d_atom_start;
+    PXFF_REQUIRE(rc > 0, "rc must be > 0");
+    PXFF_REQUIRE(nbins_total > 0, "nbins_total must be > 0");
+    PXFF_REQUIRE(slot >= 1 && slot <= NL_KMAX, "slot must be in 1..128");
+    PXFF_REQUIRE(d_slot_j && d_slot_T && d_slot_r && d_row_count, "slot arrays missing");
+    if (natoms <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    int rc_ = nl_bin_launch(d_pos, natoms, d_atom_struct, d_inv_cells, d_pbc, d_geom, nbins_total, d_bin_count,
+                            d_bin_start, d_atom_bin, d_wrap, d_bin_atoms, s);
+    if (rc_) return rc_;
+    nl_count_slots_kernel<<<(unsigned)natoms, 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom,
+                                                            rc * rc, d_bin_start, d_atom_bin, d_wrap, d_bin_atoms,
+                                                            d_img_count, slot, d_slot_j, d_slot_T, d_slot_r,
+                                                            d_row_count);
     PXFF_LAUNCH_CHECK();
     return 0;
 }
@@ -833,6 +969,28 @@ extern "C" int pxff_nlist_fill(const double *d_pos, int64_t natoms, const int64_
     nl_fill_wave_kernel<<<(unsigned)natoms, 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom, rc * rc,
                                                           d_bin_start, d_atom_bin, d_wrap, d_bin_atoms, d_img_start,
                                                           d_img_atom, d_img_j, d_img_T, d_img_r, d_row_count);
+    PXFF_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int pxff_nlist_fill_slots(const double *d_pos, int64_t natoms, const int64_t *d_atom_struct,
+                                     const int64_t *d_atom_start, const double *d_cells, const int32_t *d_pbc,
+                                     const int32_t *d_geom, double rc, const int32_t *d_bin_start,
+                                     const int32_t *d_atom_bin, const int32_t *d_wrap, const int32_t *d_bin_atoms,
+                                     const int64_t *d_img_start, const int64_t *d_img_count, int slot,
+                                     const int64_t *d_slot_j, const int32_t *d_slot_T, const double *d_slot_r,
+                                     int64_t *d_img_atom, int64_t *d_img_j, int32_t *d_img_T, double *d_img_r,
+                                     int64_t *d_row_count, void *stream) {
+    (void)d_atom_start;
+    PXFF_REQUIRE(slot >= 1 && slot <= NL_KMAX, "slot must be in 1..128");
+    PXFF_REQUIRE(d_slot_j && d_slot_T && d_slot_r && d_img_count, "slot arrays missing");
+    if (natoms <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    nl_fill_slots_kernel<<<(unsigned)natoms, 64, 0, s>>>(natoms, d_pos, d_atom_struct, d_cells, d_pbc, d_geom,
+                                                           rc * rc, d_bin_start, d_atom_bin, d_wrap, d_bin_atoms,
+                                                           d_img_start, d_img_count, slot, d_slot_j, d_slot_T,
+                                                           d_slot_r, d_img_atom, d_img_j, d_img_T, d_img_r,
+                                                           d_row_count);
     PXFF_LAUNCH_CHECK();
     return 0;
 }

@@ -50,6 +50,10 @@ extern "C" int pxff_workspace_size(int kind, const void *params, int64_t n0, int
     case PXFF_WS_SO3_PAIRS:
         *bytes = 8 * (256 * 64 + 1 + n1);
         return 0;
+    case PXFF_WS_NLIST_SLOTS:
+        // slot_j int64, slot_T 3 x int32, slot_r 3 x double per entry
+        *bytes = (8 + 12 + 24) * n0 * n1;
+        return 0;
     case PXFF_WS_SO4_UI:
         *bytes = 8 * 6 * n1;
         return 0;

@@ -74,6 +74,8 @@ class NeighborListBuilder:
     # first image capacity of a builder: max(MIN_CAPACITY, CAPACITY_PER_ATOM x atoms)
     MIN_CAPACITY = 1024
     CAPACITY_PER_ATOM = 48
+    # images a centre keeps from the count for the fill (more: enumerated again)
+    SLOT = 64
 
     def __init__(self, natoms_per_struct, cells, pbcs, rc, device="cuda"):
         counts = np.asarray(natoms_per_struct, dtype=np.int64).reshape(-1)
@@ -116,9 +118,11 @@ class NeighborListBuilder:
             self.inv_vol = torch.as_tensor(np.repeat(1.0 / self.volumes, counts), **f64)
         n = self.natoms
         nb = max(self.nbins, 1)
+        sl = max(n, 1) * self.SLOT
         self._ws = dict(bin_count=torch.empty(nb, **i32), bin_start=torch.empty(nb + 1, **i32),
                         atom_bin=torch.empty(max(n, 1), **i32), wrap=torch.empty(max(n, 1) * 3, **i32),
-                        bin_atoms=torch.empty(max(n, 1), **i32))
+                        bin_atoms=torch.empty(max(n, 1), **i32), slot_j=torch.empty(sl, **i64),
+                        slot_T=torch.empty(sl * 3, **i32), slot_r=torch.empty(sl * 3, dtype=torch.float64, device=dev))
 
         # image capacity of the next build: the arrays are sized before the
         # image count is known on the host (no synchronisation in build);
@@ -179,14 +183,18 @@ class DeviceNeighborList:
 
     def _count(self):
         """Bin the atoms (into the builder's shared workspace) and count the
-        images of every centre."""
+        images of every centre; centres with at most SLOT images keep them,
+        sorted, in the builder's slot arrays for the fill that follows (every
+        count is followed by its fill on the same stream)."""
         ws = self.builder._ws
         p = nat.ptr
-        nat.check(nat.lib().pxff_nlist_count(
+        self._row_count = torch.empty(max(self.natoms, 1), dtype=torch.int64, device=self.pos.device)
+        nat.check(nat.lib().pxff_nlist_count_slots(
             p(self.pos), self.natoms, p(self.atom_struct), p(self.atom_start), p(self.cells), p(self.inv_cells),
             p(self.pbc), p(self.geom), max(self.builder.nbins, 1), self.rc, p(ws["bin_count"]), p(ws["bin_start"]),
-            p(ws["atom_bin"]), p(ws["wrap"]), p(ws["bin_atoms"]), p(self._img_count), nat.stream()),
-            "pxff_nlist_count")
+            p(ws["atom_bin"]), p(ws["wrap"]), p(ws["bin_atoms"]), p(self._img_count), self.builder.SLOT,
+            p(ws["slot_j"]), p(ws["slot_T"]), p(ws["slot_r"]), p(self._row_count), nat.stream()),
+            "pxff_nlist_count_slots")
 
     def _fill(self, cap):
         """Scan (clamped to cap), fill and reverse phases into arrays of cap
@@ -220,11 +228,12 @@ class DeviceNeighborList:
         self.img_j = torch.empty(m, **i64)
         self.img_T = torch.empty(m * 3, **i32)
         self.img_r = torch.empty(m * 3, **f64)
-        self._row_count = torch.empty(max(n, 1), **i64)
-        nat.check(L.pxff_nlist_fill(p(self.pos), n, p(self.atom_struct), p(self.atom_start), p(self.cells),
-                                    p(self.pbc), p(self.geom), self.rc, p(bin_start), p(atom_bin), p(wrap),
-                                    p(bin_atoms), p(img_start), p(self.img_atom), p(self.img_j), p(self.img_T),
-                                    p(self.img_r), p(self._row_count), s), "pxff_nlist_fill")
+        nat.check(L.pxff_nlist_fill_slots(p(self.pos), n, p(self.atom_struct), p(self.atom_start), p(self.cells),
+                                          p(self.pbc), p(self.geom), self.rc, p(bin_start), p(atom_bin), p(wrap),
+                                          p(bin_atoms), p(img_start), p(self._img_count), self.builder.SLOT,
+                                          p(ws["slot_j"]), p(ws["slot_T"]), p(ws["slot_r"]), p(self.img_atom),
+                                          p(self.img_j), p(self.img_T), p(self.img_r), p(self._row_count), s),
+                  "pxff_nlist_fill_slots")
         self.img_rev = torch.empty(m, **i64)
         # no symmetry flag here (one launch fewer per build); verify() redoes
         # this deterministic phase with the flag

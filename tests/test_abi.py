@@ -86,5 +86,11 @@ def test_workspace_sizes_without_gpu():
         while nk < 4096 and nk < 2.5 * 2 * alpha * rc * rc:
             nk *= 2
         assert b.value == 8 * nk * (lmax + 1) * 2 * nmax * 12, (nmax, lmax, rc, alpha)
+    # neighbour-list slots: j int64, T 3 x int32, r 3 x double per entry
+    assert L.pxff_workspace_size(6, None, n, 64, ctypes.byref(b)) == 0
+    assert b.value == n * 64 * (8 + 3 * 4 + 3 * 8)
+    # staged U_tot records: 6 doubles per image of the capacity
+    assert L.pxff_workspace_size(5, None, 0, 1000, ctypes.byref(b)) == 0
+    assert b.value == 1000 * 6 * 8
     assert L.pxff_workspace_size(9, None, 0, 0, ctypes.byref(b)) != 0
     assert "workspace kind" in _native.last_error()
