@@ -1018,21 +1018,34 @@ __device__ inline void so3_ylm_tables(int lmx, double st, double ct, double cp, 
 // 205-226) with the radial integrals I_nl read from the radial store.  One
 // wavefront per centre (persistent), one lane per (l, m), (lmax + 1)^2 <= 64:
 // lane (l, m) keeps its NMAX sums in registers over all of the centre's
-// images, so no reduction is needed.  The per-image scalars are computed
-// once per 64 images (one lane each) and staged in LDS, where every lane
-// reads the same address (broadcast).  Y_lm of the lane's (l, |m|): the
-// normalised associated Legendre recurrence (the same coefficient tables as
-// ylm_row, held in registers for the lane's fixed (l, m)) times
-// e^{i|m|phi} by repeated multiplication; Y_l^-m = (-1)^m conj(Y_l^m).
-// Entries with |m| > l are written as zero.
-constexpr int SO3_LM_NST = 8;  // staged per image: s, t, cos phi, sin phi, wf (w ex fc), 0: skip
+// images, so no reduction is needed.  The images run in chunks of
+// SO3_LM_CH: the per-image scalars are computed once per chunk (one lane
+// each) and staged in LDS, where every lane reads the same address
+// (broadcast), and the chunk's I_nl rows are copied from the radial store to
+// LDS beside them, all loads of a batch in flight together (one memory
+// latency per chunk instead of one per image).  Y_lm of the lane's (l, |m|):
+// Pbar_l^|m|(cos theta) e^{i|m|phi} = Q(t) z^|m| with t = cos theta,
+// z = (x + i y) / R = sin theta e^{i phi} and Q = Pbar_l^|m| / sin^|m| theta
+// a polynomial of degree l - |m| in t; its coefficients come from the
+// normalised associated Legendre recurrence of ylm_row (the same
+// coefficient tables) run once per lane on coefficient vectors, so per
+// image a lane evaluates Q by Horner's rule and z^|m| by two squarings and
+// two products; Y_l^-m = (-1)^m conj(Y_l^m).  Entries with |m| > l are
+// written as zero.
+constexpr int SO3_LM_NST = 4;  // staged per image: t, Re z, Im z, wf (w ex fc)
+// images per chunk: a centre of up to 40 neighbours in one chunk, and with
+// NMAX <= 6 at most 18 KB of LDS (eight workgroups per CU)
+constexpr int SO3_LM_CH = 40;
+constexpr int SO3_LM_RB = 12;  // I_nl loads in flight per lane
 template <int NMAX>
 __global__ __launch_bounds__(64) void so3_ctot_lm_kernel(So3Args a, int64_t natoms, const int64_t *__restrict__ img_start,
                                                           const int64_t *__restrict__ img_j,
                                                           const double *__restrict__ img_r,
                                                           const int64_t *__restrict__ atom_z,
                                                           const double *__restrict__ rad, double *__restrict__ out) {
-    __shared__ double stg[SO3_LM_NST][64];
+    constexpr int CH = SO3_LM_CH;
+    __shared__ double stg[SO3_LM_NST][CH];
+    __shared__ double rI[CH * 8 * NMAX];  // [image][l][n], L <= 8
     const int L = a.lmax + 1;
     const int M2 = 2 * a.lmax + 1;
     const int lane = threadIdx.x;
@@ -1042,20 +1055,40 @@ __global__ __launch_bounds__(64) void so3_ctot_lm_kernel(So3Args a, int64_t nato
     const int m = lane - l * l - l;
     const bool act = lane < L * L;
     const int am = m < 0 ? -m : m;
-    // Legendre coefficients of the lane's (l, |m|), once (lmax <= 7)
+    // coefficients q[j] of t^j in Q = Pbar_l^|m| / s^|m| (lmax <= 7): the
+    // recurrence P_|m| = sqrt(1/4pi) prod_k pmm_k s^|m|, P_|m|+1 = p1 t P_|m|,
+    // P_k = lega_k (t P_k-1 - legb_k P_k-2), on vectors of coefficients
     constexpr int LK = 8;
-    double pmmf[LK], lega_[LK], legb_[LK];
-    double p1 = 0.0;
-    const double lnorm = act ? a.lnorm[l] : 0.0;
+    double q[LK], q2[LK];
+    {
+        double c0 = sqrt(0.25 / M_PI);
 #pragma unroll
-    for (int k = 0; k < LK; k++) {
-        pmmf[k] = (k >= 1 && k <= am) ? g_so3_pmm[k] : 1.0;
-        lega_[k] = (k >= am + 2 && k <= l) ? g_so3_lega[k * SO3_LEGD + am] : 0.0;
-        legb_[k] = (k >= am + 2 && k <= l) ? g_so3_legb[k * SO3_LEGD + am] : 0.0;
+        for (int k = 1; k < LK; k++)
+            if (k <= am) c0 *= g_so3_pmm[k];
+#pragma unroll
+        for (int j = 0; j < LK; j++) q[j] = j == 0 ? c0 : 0.0;
+        if (act && l > am) {
+            const double p1 = g_so3_p1[am];
+#pragma unroll
+            for (int j = LK - 1; j >= 0; j--) {
+                q2[j] = q[j];
+                q[j] = j > 0 ? p1 * q[j - 1] : 0.0;
+            }
+#pragma unroll
+            for (int k = 2; k < LK; k++)
+                if (k >= am + 2 && k <= l) {
+                    const double la = g_so3_lega[k * SO3_LEGD + am], lb = g_so3_legb[k * SO3_LEGD + am];
+#pragma unroll
+                    for (int j = LK - 1; j >= 0; j--) {
+                        const double nq = la * ((j > 0 ? q[j - 1] : 0.0) - lb * q2[j]);
+                        q2[j] = q[j];
+                        q[j] = nq;
+                    }
+                }
+        }
     }
-    if (act && l > am) p1 = g_so3_p1[am];
+    const double lnorm = act ? a.lnorm[l] : 0.0;
     const double sgn = (m < 0 && (am & 1)) ? -1.0 : 1.0;
-    const double inv4pi = 0.25 / M_PI;
     for (int64_t c = blockIdx.x; c < natoms; c += gridDim.x) {
         double accr[NMAX], acci[NMAX];
 #pragma unroll
@@ -1064,50 +1097,59 @@ __global__ __launch_bounds__(64) void so3_ctot_lm_kernel(So3Args a, int64_t nato
             acci[n] = 0.0;
         }
         const int64_t s0 = img_start[c], s1 = img_start[c + 1];
-        for (int64_t b64 = s0; b64 < s1; b64 += 64) {
-            const int nb = (int)((s1 - b64) < 64 ? (s1 - b64) : 64);
+        for (int64_t b64 = s0; b64 < s1; b64 += CH) {
+            const int nb = (int)((s1 - b64) < CH ? (s1 - b64) : CH);
             __syncthreads();  // the previous chunk's stage has been read
             {
                 const int64_t e = b64 + lane;
                 const bool ok = lane < nb;
                 So3Img mi;
                 load_image(a, ok, e, c, img_j, img_r, atom_z, mi);
-                const double rho = sqrt(mi.x * mi.x + mi.y * mi.y);
-                stg[0][lane] = rho / mi.R;                    // sin theta
-                stg[1][lane] = mi.z / mi.R;                   // cos theta
-                stg[2][lane] = rho > 0.0 ? mi.x / rho : 1.0;  // cos phi
-                stg[3][lane] = rho > 0.0 ? mi.y / rho : 0.0;  // sin phi
-                stg[4][lane] = ok ? mi.w * mi.ex * mi.fc : 0.0;
+                if (lane < CH) {
+                    stg[0][lane] = mi.z / mi.R;  // t = cos theta
+                    stg[1][lane] = mi.x / mi.R;  // z = sin theta e^{i phi}
+                    stg[2][lane] = mi.y / mi.R;
+                    stg[3][lane] = ok ? mi.w * mi.ex * mi.fc : 0.0;
+                }
+            }
+            {
+                // rI[(b L + l) NMAX + n] = I_nl of image b64 + b: the first
+                // NMAX of each 2 NMAX-row of the radial store
+                const double *rb = rad + (size_t)b64 * L * 2 * NMAX;
+                const int nri = nb * L * NMAX;
+                for (int k0 = 0; k0 < nri; k0 += 64 * SO3_LM_RB) {
+                    double t[SO3_LM_RB];
+#pragma unroll
+                    for (int u = 0; u < SO3_LM_RB; u++) {
+                        const int k = k0 + u * 64 + lane;
+                        const int row = k / NMAX;
+                        t[u] = k < nri ? rb[(size_t)row * 2 * NMAX + (k - row * NMAX)] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < SO3_LM_RB; u++) {
+                        const int k = k0 + u * 64 + lane;
+                        if (k < nri) rI[k] = t[u];
+                    }
+                }
             }
             __syncthreads();
             if (act) {
                 for (int b = 0; b < nb; b++) {
-                    const double st = stg[0][b], ct = stg[1][b], cp = stg[2][b], sp = stg[3][b];
-                    const double wf = stg[4][b] * lnorm;
-                    // Pbar_l^|m|(cos theta)
-                    double pmm = sqrt(inv4pi);
-                    cd em = cmake(1.0, 0.0);
+                    const double ct = stg[0][b];
+                    const cd z = cmake(stg[1][b], stg[2][b]);
+                    const double wf = stg[3][b] * lnorm;
+                    // Q(t) by Horner's rule
+                    double pl = q[LK - 1];
 #pragma unroll
-                    for (int k = 1; k < LK; k++)
-                        if (k <= am) {
-                            pmm = pmmf[k] * st * pmm;
-                            em = cmul(em, cmake(cp, sp));
-                        }
-                    double pl = pmm;
-                    if (l > am) {
-                        double pm2 = pmm, pm1 = p1 * ct * pmm;
-#pragma unroll
-                        for (int k = 2; k < LK; k++)
-                            if (k >= am + 2 && k <= l) {
-                                const double pk = lega_[k] * (ct * pm1 - legb_[k] * pm2);
-                                pm2 = pm1;
-                                pm1 = pk;
-                            }
-                        pl = pm1;
-                    }
+                    for (int j = LK - 2; j >= 0; j--) pl = fma(pl, ct, q[j]);
+                    // z^|m|, |m| <= 7
+                    const cd z2 = cmul(z, z), z4 = cmul(z2, z2);
+                    cd em = (am & 1) ? z : cmake(1.0, 0.0);
+                    if (am & 2) em = cmul(em, z2);
+                    if (am & 4) em = cmul(em, z4);
                     // Y_l^m w ex fc norm_l, then times I_nl
                     const double yr = pl * em.re * wf * sgn, yi = (m < 0 ? -1.0 : 1.0) * pl * em.im * wf * sgn;
-                    const double *ri = rad + ((size_t)(b64 + b) * L + l) * 2 * NMAX;
+                    const double *ri = rI + (b * L + l) * NMAX;
 #pragma unroll
                     for (int n = 0; n < NMAX; n++) {
                         const double in = ri[n];
