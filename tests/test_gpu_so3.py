@@ -33,6 +33,21 @@ def test_so3_nacl_parity_cutoffs(so3o, cutoff):
     check_layout(g, r)
 
 
+@pytest.mark.parametrize("lmax", [6, 7])
+def test_so3_images_on_the_polar_axis(so3o, lmax):
+    """The unrotated rocksalt cluster has images straight along +-z (sin
+    theta = 0, where Y_lm vanishes for m != 0 and phi is undefined): x, dxdr
+    and rdxdr against the oracle through the per-(l, m) expansion sums."""
+    s = workloads.nacl_cluster(0.0)
+    d = s.positions[:, None, :] - s.positions[None, :, :]
+    assert np.any((np.abs(d[..., 0]) < 1e-12) & (np.abs(d[..., 1]) < 1e-12) & (np.abs(d[..., 2]) > 0))
+    print("so3 nacl on axis lmax", lmax, flush=True)
+    g = SO3(nmax=3, lmax=lmax, rcut=6.0, derivative=True, stress=True).calculate(s)
+    r = so3o.calculate(s.positions, s.cell, s.pbc, s.numbers, nmax=3, lmax=lmax, rcut=6.0, derivative=True,
+                       stress=True)
+    check_layout(g, r)
+
+
 @pytest.mark.parametrize("nmax,lmax", [(3, 4), (6, 6), (1, 0), (4, 9)])
 def test_so3_diamond_parity(so3o, nmax, lmax):
     s = workloads.diamond_si(reps=2, sigma=0.05, seed=0)
