@@ -1524,29 +1524,41 @@ __global__ void so3_power_kernel(int64_t natoms, int nmax, int lmax, const doubl
 }
 
 // A[i][n][l][m] = sum_n' S[n][n'][l] C[i][n'][l][m], S from dE/dP with the
-// diagonal doubled (P is quadratic in C).
-__global__ void so3_adj_a_kernel(int64_t natoms, int nmax, int lmax, const double *__restrict__ ct,
-                                 const double *__restrict__ beta, double *__restrict__ A) {
-    const int L = lmax + 1, M2 = 2 * lmax + 1;
-    const int ntri = nmax * (nmax + 1) / 2;
-    const int csz = nmax * L * M2;
+// diagonal doubled (P is quadratic in C).  One thread per (centre, l, m)
+// forms all NMAX outputs from the column C[i][.][l][m] read once into
+// registers: every C entry is read by one thread (with a thread per output,
+// a centre's entries straddle workgroups that run on different XCDs, and
+// each XCD fetched them from HBM through its own L2).  Loads and stores run
+// along (l, m), consecutive across the threads.
+template <int NMAX>
+__global__ __launch_bounds__(256) void so3_adj_a_kernel(int64_t natoms, int lmax, const double *__restrict__ ct,
+                                                         const double *__restrict__ beta, double *__restrict__ A) {
+    constexpr int NTRI = NMAX * (NMAX + 1) / 2;
+    const int L = lmax + 1, M2 = 2 * lmax + 1, LM = L * M2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= natoms * csz) return;
-    const int64_t i = t / csz;
-    const int k = (int)(t - i * csz);
-    const int n = k / (L * M2), rem = k - n * L * M2, l = rem / M2, mi = rem - l * M2;
-    const cd *c = reinterpret_cast<const cd *>(ct) + (size_t)i * csz;
-    const double *b = beta + (size_t)i * ntri * L;
-    cd s = cmake(0, 0);
-    if (mi - lmax >= -l && mi - lmax <= l) {
-        for (int np = 0; np < nmax; np++) {
+    if (t >= natoms * LM) return;
+    const int64_t i = t / LM;
+    const int k = (int)(t - i * LM);
+    const int l = k / M2, mi = k - l * M2;
+    const bool in = mi - lmax >= -l && mi - lmax <= l;
+    const cd *c = reinterpret_cast<const cd *>(ct) + (size_t)i * NMAX * LM + k;
+    cd *o = reinterpret_cast<cd *>(A) + (size_t)i * NMAX * LM + k;
+    cd cv[NMAX];
+#pragma unroll
+    for (int np = 0; np < NMAX; np++) cv[np] = in ? c[(size_t)np * LM] : cmake(0.0, 0.0);
+    const double *b = beta + (size_t)i * NTRI * L + l;
+#pragma unroll
+    for (int n = 0; n < NMAX; n++) {
+        cd s = cmake(0, 0);
+#pragma unroll
+        for (int np = 0; np < NMAX; np++) {
             const int hi = n > np ? n : np, lo = n > np ? np : n;
-            double f = b[(hi * (hi + 1) / 2 + lo) * L + l];
+            double f = b[(hi * (hi + 1) / 2 + lo) * L];
             if (np == n) f *= 2.0;
-            s = cadd(s, cscale(c[(np * L + l) * M2 + mi], f));
+            s = cadd(s, cscale(cv[np], f));
         }
+        o[(size_t)n * LM] = s;
     }
-    reinterpret_cast<cd *>(A)[t] = s;
 }
 
 }  // namespace pxff
@@ -1827,9 +1839,11 @@ extern "C" int pxff_so3_adjoint_a(const pxff_so3_params *p, int64_t natoms, cons
     int rc = so3_check(p);
     if (rc) return rc;
     if (natoms <= 0) return 0;
-    const int64_t csz = (int64_t)p->nmax * (p->lmax + 1) * (2 * p->lmax + 1);
-    so3_adj_a_kernel<<<div_up(natoms * csz, 256), 256, 0, (hipStream_t)stream>>>(natoms, p->nmax, p->lmax, d_ctot,
-                                                                                 d_beta, d_a);
+    // one thread per (centre, l, m)
+    const int64_t nlm = (int64_t)(p->lmax + 1) * (2 * p->lmax + 1);
+    hipStream_t s = (hipStream_t)stream;
+    SO3_DISPATCH_NMAX(p->nmax, (so3_adj_a_kernel<NM><<<div_up(natoms * nlm, 256), 256, 0, s>>>(natoms, p->lmax, d_ctot,
+                                                                                            d_beta, d_a)));
     PXFF_LAUNCH_CHECK();
     return 0;
 }
