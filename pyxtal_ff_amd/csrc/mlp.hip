@@ -295,134 +295,144 @@ __global__ __launch_bounds__(256) void mlp_mfma_kernel(int64_t natoms, int nin, 
     double *wl = hder + (size_t)nlm * MLP_TILE * HP;    // W_k [r16(nout)][row(nin)], b_k [r16(nout)]
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int ri = lane & 15, kq = lane >> 4;
-    const int64_t a0 = (int64_t)blockIdx.x * MLP_TILE;
-    if (tid < MLP_TILE) {
-        const int64_t a = a0 + tid;
-        int e = -2;
-        if (a < natoms) {
-            const int64_t sp = species[a];
-            e = (sp >= 0 && sp < nelem) ? (int)sp : -1;
-        }
-        s_sp[tid] = e;
-    }
-    __syncthreads();
-    for (int el = 0; el < nelem; el++) {
-        bool any = false;
-        for (int t = 0; t < MLP_TILE; t++) any = any || s_sp[t] == el;
-        if (!any) continue;  // uniform over the workgroup
-        const int *lay = layout + el * MLP_STRIDE;
-        const int nl = lay[0];
-        const double *dm = dmin + (size_t)el * nin, *sc = scale + (size_t)el * nin;
-        // staging: eight loads in flight per thread before their LDS stores
-        for (int q0 = tid; q0 < MLP_TILE * P0; q0 += 256 * 8) {
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int q = q0 + 256 * u;
-                const int t = q / P0, j = q - t * P0;
-                const int64_t a = a0 + t;
-                v[u] = (q < MLP_TILE * P0 && a < natoms && j < nin) ? (x[(size_t)a * nin + j] - dm[j]) * sc[j]
-                                                                     : 0.0;
+    // tiles [t0, t1) of this workgroup, in order: an element's weights stay
+    // staged from one tile to the next (atoms come sorted by element)
+    const int64_t ntiles = (natoms + MLP_TILE - 1) / MLP_TILE;
+    const int64_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
+    int staged = -1;  // element whose weights are in wl
+    for (int64_t tl = t0; tl < t1; tl++) {
+        const int64_t a0 = tl * MLP_TILE;
+        __syncthreads();  // the previous tile is done with s_sp and the scratch
+        if (tid < MLP_TILE) {
+            const int64_t a = a0 + tid;
+            int e = -2;
+            if (a < natoms) {
+                const int64_t sp = species[a];
+                e = (sp >= 0 && sp < nelem) ? (int)sp : -1;
             }
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (q0 + 256 * u < MLP_TILE * P0) h0[q0 + 256 * u] = v[u];
-        }
-        {
-            int wo = 0;
-            for (int k = 0; k < nl; k++) {
-                const int *L = lay + 1 + MLP_LSTRIDE * k;
-                const int ni = L[0], no = L[1], R = mlp_r16(no), C = mlp_row(ni);
-                const double *W = params + L[3], *bias = params + L[4];
-                for (int q0 = tid; q0 < R * C + R; q0 += 256 * 8) {
-                    double v[8];
-#pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        const int q = q0 + 256 * u;
-                        v[u] = 0.0;
-                        if (q < R * C) {
-                            const int r = q / C, c = q - r * C;
-                            if (r < no && c < ni) v[u] = W[r * ni + c];
-                        } else if (q - R * C < no) {
-                            v[u] = bias[q - R * C];
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; u++)
-                        if (q0 + 256 * u < R * C + R) wl[wo + q0 + 256 * u] = v[u];
-                }
-                if (tid == 0) s_wo[k] = wo;
-                wo += R * C + R;
-            }
+            s_sp[tid] = e;
         }
         __syncthreads();
-        // forward
-        const double *hin = h0;
-        int ldi = P0;
-        for (int k = 0; k < nl; k++) {
-            const int *L = lay + 1 + MLP_LSTRIDE * k;
-            const int ni = L[0], no = L[1], act = L[2], C = mlp_row(ni);
-            const double *W = wl + s_wo[k], *bias = W + mlp_r16(no) * C;
-            double *ho = hact + (size_t)k * MLP_TILE * HP, *hd = hder + (size_t)k * MLP_TILE * HP;
-            for (int ct = wv; ct * 16 < no; ct += 4) {
-                // D[j][i] = sum_k W[j][k] H[i][k]: A rows = neurons of the
-                // tile, B columns = atoms
-                const mlp_d4 acc = mlp_gemm_tile(mlp_r4(ni), W + (ct * 16 + ri) * C + kq, hin + ri * ldi + kq, 4);
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int j = ct * 16 + kq + 4 * r;  // neuron; the lane's atom is ri
-                    double av = 0.0, da = 0.0;          // zero past the layer width
-                    if (j < no) act_fdf(act, acc[r] + bias[j], av, da);
-                    ho[ri * HP + j] = av;
-                    hd[ri * HP + j] = da;
+        for (int el = 0; el < nelem; el++) {
+            bool any = false;
+            for (int t = 0; t < MLP_TILE; t++) any = any || s_sp[t] == el;
+            if (!any) continue;  // uniform over the workgroup
+            const int *lay = layout + el * MLP_STRIDE;
+            const int nl = lay[0];
+            const double *dm = dmin + (size_t)el * nin, *sc = scale + (size_t)el * nin;
+            // staging: eight loads in flight per thread before their LDS stores
+            for (int q0 = tid; q0 < MLP_TILE * P0; q0 += 256 * 8) {
+                double v[8];
+    #pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int q = q0 + 256 * u;
+                    const int t = q / P0, j = q - t * P0;
+                    const int64_t a = a0 + t;
+                    v[u] = (q < MLP_TILE * P0 && a < natoms && j < nin) ? (x[(size_t)a * nin + j] - dm[j]) * sc[j]
+                                                                         : 0.0;
+                }
+    #pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (q0 + 256 * u < MLP_TILE * P0) h0[q0 + 256 * u] = v[u];
+            }
+            if (el != staged) {  // uniform over the workgroup
+                staged = el;
+                int wo = 0;
+                for (int k = 0; k < nl; k++) {
+                    const int *L = lay + 1 + MLP_LSTRIDE * k;
+                    const int ni = L[0], no = L[1], R = mlp_r16(no), C = mlp_row(ni);
+                    const double *W = params + L[3], *bias = params + L[4];
+                    for (int q0 = tid; q0 < R * C + R; q0 += 256 * 8) {
+                        double v[8];
+    #pragma unroll
+                        for (int u = 0; u < 8; u++) {
+                            const int q = q0 + 256 * u;
+                            v[u] = 0.0;
+                            if (q < R * C) {
+                                const int r = q / C, c = q - r * C;
+                                if (r < no && c < ni) v[u] = W[r * ni + c];
+                            } else if (q - R * C < no) {
+                                v[u] = bias[q - R * C];
+                            }
+                        }
+    #pragma unroll
+                        for (int u = 0; u < 8; u++)
+                            if (q0 + 256 * u < R * C + R) wl[wo + q0 + 256 * u] = v[u];
+                    }
+                    if (tid == 0) s_wo[k] = wo;
+                    wo += R * C + R;
                 }
             }
             __syncthreads();
-            hin = ho;
-            ldi = HP;
-        }
-        if (tid < MLP_TILE && s_sp[tid] == el) e_out[a0 + tid] = hin[tid * HP];
-        // backward; hder of the output layer (one neuron) already is dE/dz
-        for (int k = nl - 1; k >= 1; k--) {
-            const int *L = lay + 1 + MLP_LSTRIDE * k;
-            const int ni = L[0], no = L[1], C = mlp_row(ni);
-            const double *W = wl + s_wo[k];
-            const double *g = hder + (size_t)k * MLP_TILE * HP;
-            double *gp = hder + (size_t)(k - 1) * MLP_TILE * HP;
-            for (int ct = wv; ct * 16 < ni; ct += 4) {
-                const int j = ct * 16 + ri;
-                // B[n][j] = W[n][j]
-                const mlp_d4 acc = mlp_gemm_tile(mlp_r4(no), g + ri * HP + kq, W + kq * C + j, 4 * C);
-#pragma unroll
-                for (int r = 0; r < 4; r++) gp[(kq + 4 * r) * HP + j] *= acc[r];
-            }
-            __syncthreads();
-        }
-        {
-            const int *L = lay + 1;
-            const int no = L[1], C = mlp_row(nin);
-            const double *W = wl + s_wo[0];
-            for (int ct = wv; ct * 16 < nin; ct += 4) {
-                const int j = ct * 16 + ri;
-                const mlp_d4 acc = mlp_gemm_tile(mlp_r4(no), hder + ri * HP + kq, W + kq * C + j, 4 * C);
-                if (j < nin) {
-                    const double sj = sc[j];
-#pragma unroll
+            // forward
+            const double *hin = h0;
+            int ldi = P0;
+            for (int k = 0; k < nl; k++) {
+                const int *L = lay + 1 + MLP_LSTRIDE * k;
+                const int ni = L[0], no = L[1], act = L[2], C = mlp_row(ni);
+                const double *W = wl + s_wo[k], *bias = W + mlp_r16(no) * C;
+                double *ho = hact + (size_t)k * MLP_TILE * HP, *hd = hder + (size_t)k * MLP_TILE * HP;
+                for (int ct = wv; ct * 16 < no; ct += 4) {
+                    // D[j][i] = sum_k W[j][k] H[i][k]: A rows = neurons of the
+                    // tile, B columns = atoms
+                    const mlp_d4 acc = mlp_gemm_tile(mlp_r4(ni), W + (ct * 16 + ri) * C + kq, hin + ri * ldi + kq, 4);
+    #pragma unroll
                     for (int r = 0; r < 4; r++) {
-                        const int row = kq + 4 * r;
-                        if (s_sp[row] == el) beta[(size_t)(a0 + row) * nin + j] = acc[r] * sj;
+                        const int j = ct * 16 + kq + 4 * r;  // neuron; the lane's atom is ri
+                        double av = 0.0, da = 0.0;          // zero past the layer width
+                        if (j < no) act_fdf(act, acc[r] + bias[j], av, da);
+                        ho[ri * HP + j] = av;
+                        hd[ri * HP + j] = da;
+                    }
+                }
+                __syncthreads();
+                hin = ho;
+                ldi = HP;
+            }
+            if (tid < MLP_TILE && s_sp[tid] == el) e_out[a0 + tid] = hin[tid * HP];
+            // backward; hder of the output layer (one neuron) already is dE/dz
+            for (int k = nl - 1; k >= 1; k--) {
+                const int *L = lay + 1 + MLP_LSTRIDE * k;
+                const int ni = L[0], no = L[1], C = mlp_row(ni);
+                const double *W = wl + s_wo[k];
+                const double *g = hder + (size_t)k * MLP_TILE * HP;
+                double *gp = hder + (size_t)(k - 1) * MLP_TILE * HP;
+                for (int ct = wv; ct * 16 < ni; ct += 4) {
+                    const int j = ct * 16 + ri;
+                    // B[n][j] = W[n][j]
+                    const mlp_d4 acc = mlp_gemm_tile(mlp_r4(no), g + ri * HP + kq, W + kq * C + j, 4 * C);
+    #pragma unroll
+                    for (int r = 0; r < 4; r++) gp[(kq + 4 * r) * HP + j] *= acc[r];
+                }
+                __syncthreads();
+            }
+            {
+                const int *L = lay + 1;
+                const int no = L[1], C = mlp_row(nin);
+                const double *W = wl + s_wo[0];
+                for (int ct = wv; ct * 16 < nin; ct += 4) {
+                    const int j = ct * 16 + ri;
+                    const mlp_d4 acc = mlp_gemm_tile(mlp_r4(no), hder + ri * HP + kq, W + kq * C + j, 4 * C);
+                    if (j < nin) {
+                        const double sj = sc[j];
+    #pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int row = kq + 4 * r;
+                            if (s_sp[row] == el) beta[(size_t)(a0 + row) * nin + j] = acc[r] * sj;
+                        }
                     }
                 }
             }
+            __syncthreads();  // the next element reuses the scratch
         }
-        __syncthreads();  // the next element reuses the scratch
+        // atoms without a network: NaN results
+        for (int q = tid; q < MLP_TILE * nin; q += 256) {
+            const int t = q / nin;
+            if (s_sp[t] == -1) beta[(size_t)(a0 + t) * nin + (q - t * nin)] = __builtin_nan("");
+        }
+        if (tid < MLP_TILE && s_sp[tid] == -1) e_out[a0 + tid] = __builtin_nan("");
     }
-    // atoms without a network: NaN results
-    for (int q = tid; q < MLP_TILE * nin; q += 256) {
-        const int t = q / nin;
-        if (s_sp[t] == -1) beta[(size_t)(a0 + t) * nin + (q - t * nin)] = __builtin_nan("");
-    }
-    if (tid < MLP_TILE && s_sp[tid] == -1) e_out[a0 + tid] = __builtin_nan("");
 }
 
 }  // namespace pxff
@@ -493,7 +503,11 @@ extern "C" int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const in
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsm));
             mfma_dyn_max = (int)ldsm;
         }
-        mlp_mfma_kernel<<<(unsigned)div_up(natoms, MLP_TILE), 256, ldsm, s>>>(
+        // persistent: as many workgroups as the LDS keeps resident (at most
+        // 8 per CU), each over a contiguous range of 16-atom tiles
+        const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(8, (int64_t)(160 * 1024) / (int64_t)(ldsm + 1024)));
+        const int64_t nwg = std::min<int64_t>(div_up(natoms, MLP_TILE), (int64_t)num_cus() * per_cu);
+        mlp_mfma_kernel<<<(unsigned)nwg, 256, ldsm, s>>>(
             natoms, nin, nelem, d_species, d_layout, d_params, d_dmin, d_scale, d_x, d_e, d_beta, nlm, hw);
         PXFF_LAUNCH_CHECK();
         return 0;
