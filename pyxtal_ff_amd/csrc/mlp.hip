@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 
 #include "pxff_common.h"
 #include "so4_dev.h"
@@ -497,11 +499,17 @@ extern "C" int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const in
     // above 64 KB (long descriptors, e.g. SO3 6/6: 147 inputs) one workgroup
     // per CU; the kernel's static LDS (a few hundred bytes) comes on top
     if (ldsm <= 160 * 1024 - 1024) {
-        static int mfma_dyn_max = 65536;  // dynamic LDS limit set so far
-        if ((int)ldsm > mfma_dyn_max) {
-            PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)mlp_mfma_kernel,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsm));
-            mfma_dyn_max = (int)ldsm;
+        // dynamic LDS limit set so far; raised under a lock so that
+        // concurrent callers never lower it below what another relies on
+        static std::mutex mfma_attr_mu;
+        static int mfma_dyn_max = 65536;
+        {
+            std::lock_guard<std::mutex> lock(mfma_attr_mu);
+            if ((int)ldsm > mfma_dyn_max) {
+                PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)mlp_mfma_kernel,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsm));
+                mfma_dyn_max = (int)ldsm;
+            }
         }
         // persistent: as many workgroups as the LDS keeps resident (at most
         // 8 per CU), each over a contiguous range of 16-atom tiles
@@ -514,11 +522,11 @@ extern "C" int pxff_mlp_energy_grad(int64_t natoms, int nin, int nelem, const in
     }
     const size_t ldsp = sizeof(double) * (size_t)nparams + MLP_LW * scratch;
     if (ldsp <= 160 * 1024) {  // the weights fit in LDS next to the scratch
-        static int attr_set = 0;
-        if (ldsp > 65536 && !attr_set) {
+        static std::atomic<int> attr_set{0};  // the limit is one value: setting it twice is harmless
+        if (ldsp > 65536 && !attr_set.load(std::memory_order_relaxed)) {
             PXFF_HIP_CHECK(hipFuncSetAttribute((const void *)mlp_lds_kernel,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            attr_set = 1;
+            attr_set.store(1, std::memory_order_relaxed);
         }
         // as many resident workgroups as the LDS holds (at most 4 per CU)
         const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)(160 * 1024) / (int64_t)ldsp));

@@ -1761,16 +1761,19 @@ int so4_check_params(const pxff_so4_params *p) {
     return 0;
 }
 
-static int g_num_cus = 0;
+// queried once; concurrent first calls store the same value
+static std::atomic<int> g_num_cus{0};
 int num_cus() {
-    if (g_num_cus == 0) {
+    int n = g_num_cus.load(std::memory_order_relaxed);
+    if (n == 0) {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess) return 256;
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
-        g_num_cus = prop.multiProcessorCount;
+        n = prop.multiProcessorCount;
+        g_num_cus.store(n, std::memory_order_relaxed);
     }
-    return g_num_cus;
+    return n;
 }
 }  // namespace pxff
 
