@@ -981,37 +981,57 @@ __global__ __launch_bounds__(64) void so3_accum_kernel(So3Args a, int64_t natoms
     }
 }
 
-// Per-image tables of the spherical harmonics, built by one lane per image
-// (so3_ylm_tables): Pbar_l^m(cos theta) for m <= l < LT (the normalised
-// associated Legendre recurrence of ylm_row, in the same order) and
-// e^{i m phi} for m < LT (repeated multiplication).  Y_l^m = Pbar_l^m
-// e^{i m phi} and Y_l^-m = (-1)^m conj(Y_l^m): a lane of a fixed (l, m)
-// then needs two LDS reads per image instead of its own recurrences (the
-// adjoint kernel below; in the expansion kernel, whose per-image work is
-// small, the lanes' own recurrences are faster than building tables).
-template <int LT>
-__device__ inline void so3_ylm_tables(int lmx, double st, double ct, double cp, double sp, double *P, cd *E) {
-    const double inv4pi = 0.25 / M_PI;
-    double pmm = sqrt(inv4pi);
-    cd em = cmake(1.0, 0.0);
-    for (int m = 0; m <= lmx; m++) {
-        if (m > 0) {
-            pmm = g_so3_pmm[m] * st * pmm;
-            em = cmul(em, cmake(cp, sp));
+// Y_l^|m| without tables, for l <= 7: Pbar_l^|m|(cos theta) e^{i|m|phi} =
+// Q(t) z^|m| with t = cos theta, z = (x + i y) / R = sin theta e^{i phi} and
+// Q = Pbar_l^|m| / sin^|m| theta a polynomial of degree l - |m| in t.  Its
+// coefficients q[j] (of t^j) come from the normalised associated Legendre
+// recurrence of ylm_row (the same coefficient tables) run on coefficient
+// vectors: P_|m| = sqrt(1/4pi) prod_k pmm_k s^|m|, P_|m|+1 = p1 t P_|m|,
+// P_k = lega_k (t P_k-1 - legb_k P_k-2).  With on false q is P_|m| / s^|m|
+// alone (the lanes without a row).
+constexpr int SO3_QK = 8;
+__device__ __forceinline__ void so3_legendre_poly(int l, int am, bool on, double (&q)[SO3_QK]) {
+    double q2[SO3_QK];
+    double c0 = sqrt(0.25 / M_PI);
+#pragma unroll
+    for (int k = 1; k < SO3_QK; k++)
+        if (k <= am) c0 *= g_so3_pmm[k];
+#pragma unroll
+    for (int j = 0; j < SO3_QK; j++) q[j] = j == 0 ? c0 : 0.0;
+    if (on && l > am) {
+        const double p1 = g_so3_p1[am];
+#pragma unroll
+        for (int j = SO3_QK - 1; j >= 0; j--) {
+            q2[j] = q[j];
+            q[j] = j > 0 ? p1 * q[j - 1] : 0.0;
         }
-        E[m] = em;
-        P[m * (m + 1) / 2 + m] = pmm;
-        if (m < lmx) {
-            double pm2 = pmm, pm1 = g_so3_p1[m] * ct * pmm;
-            P[(m + 1) * (m + 2) / 2 + m] = pm1;
-            for (int k = m + 2; k <= lmx; k++) {
-                const double pk = g_so3_lega[k * SO3_LEGD + m] * (ct * pm1 - g_so3_legb[k * SO3_LEGD + m] * pm2);
-                pm2 = pm1;
-                pm1 = pk;
-                P[k * (k + 1) / 2 + m] = pk;
+#pragma unroll
+        for (int k = 2; k < SO3_QK; k++)
+            if (k >= am + 2 && k <= l) {
+                const double la = g_so3_lega[k * SO3_LEGD + am], lb = g_so3_legb[k * SO3_LEGD + am];
+#pragma unroll
+                for (int j = SO3_QK - 1; j >= 0; j--) {
+                    const double nq = la * ((j > 0 ? q[j - 1] : 0.0) - lb * q2[j]);
+                    q2[j] = q[j];
+                    q[j] = nq;
+                }
             }
-        }
     }
+}
+// Q(t) by Horner's rule
+__device__ __forceinline__ double so3_legendre_eval(const double (&q)[SO3_QK], double t) {
+    double p = q[SO3_QK - 1];
+#pragma unroll
+    for (int j = SO3_QK - 2; j >= 0; j--) p = fma(p, t, q[j]);
+    return p;
+}
+// z^am, 0 <= am <= 7: two squarings and two products
+__device__ __forceinline__ cd so3_zpow(cd z, int am) {
+    const cd z2 = cmul(z, z), z4 = cmul(z2, z2);
+    cd w = (am & 1) ? z : cmake(1.0, 0.0);
+    if (am & 2) w = cmul(w, z2);
+    if (am & 4) w = cmul(w, z4);
+    return w;
 }
 
 // ctot[i][n][l][m] = sum_e c_e,nlm = sum_e w norm_l ex fc I_nl Y_lm (SO3.py:
@@ -1023,15 +1043,11 @@ __device__ inline void so3_ylm_tables(int lmx, double st, double ct, double cp, 
 // each) and staged in LDS, where every lane reads the same address
 // (broadcast), and the chunk's I_nl rows are copied from the radial store to
 // LDS beside them, all loads of a batch in flight together (one memory
-// latency per chunk instead of one per image).  Y_lm of the lane's (l, |m|):
-// Pbar_l^|m|(cos theta) e^{i|m|phi} = Q(t) z^|m| with t = cos theta,
-// z = (x + i y) / R = sin theta e^{i phi} and Q = Pbar_l^|m| / sin^|m| theta
-// a polynomial of degree l - |m| in t; its coefficients come from the
-// normalised associated Legendre recurrence of ylm_row (the same
-// coefficient tables) run once per lane on coefficient vectors, so per
-// image a lane evaluates Q by Horner's rule and z^|m| by two squarings and
-// two products; Y_l^-m = (-1)^m conj(Y_l^m).  Entries with |m| > l are
-// written as zero.
+// latency per chunk instead of one per image).  Y_lm of the lane's (l, |m|)
+// as Q(t) z^|m| (so3_legendre_poly, built once per lane), so per image a
+// lane evaluates Q by Horner's rule and z^|m| by two squarings and two
+// products; Y_l^-m = (-1)^m conj(Y_l^m).  Entries with |m| > l are written
+// as zero.
 constexpr int SO3_LM_NST = 4;  // staged per image: t, Re z, Im z, wf (w ex fc)
 // images per chunk: a centre of up to 40 neighbours in one chunk, and with
 // NMAX <= 6 at most 18 KB of LDS (eight workgroups per CU)
@@ -1055,38 +1071,9 @@ __global__ __launch_bounds__(64) void so3_ctot_lm_kernel(So3Args a, int64_t nato
     const int m = lane - l * l - l;
     const bool act = lane < L * L;
     const int am = m < 0 ? -m : m;
-    // coefficients q[j] of t^j in Q = Pbar_l^|m| / s^|m| (lmax <= 7): the
-    // recurrence P_|m| = sqrt(1/4pi) prod_k pmm_k s^|m|, P_|m|+1 = p1 t P_|m|,
-    // P_k = lega_k (t P_k-1 - legb_k P_k-2), on vectors of coefficients
-    constexpr int LK = 8;
-    double q[LK], q2[LK];
-    {
-        double c0 = sqrt(0.25 / M_PI);
-#pragma unroll
-        for (int k = 1; k < LK; k++)
-            if (k <= am) c0 *= g_so3_pmm[k];
-#pragma unroll
-        for (int j = 0; j < LK; j++) q[j] = j == 0 ? c0 : 0.0;
-        if (act && l > am) {
-            const double p1 = g_so3_p1[am];
-#pragma unroll
-            for (int j = LK - 1; j >= 0; j--) {
-                q2[j] = q[j];
-                q[j] = j > 0 ? p1 * q[j - 1] : 0.0;
-            }
-#pragma unroll
-            for (int k = 2; k < LK; k++)
-                if (k >= am + 2 && k <= l) {
-                    const double la = g_so3_lega[k * SO3_LEGD + am], lb = g_so3_legb[k * SO3_LEGD + am];
-#pragma unroll
-                    for (int j = LK - 1; j >= 0; j--) {
-                        const double nq = la * ((j > 0 ? q[j - 1] : 0.0) - lb * q2[j]);
-                        q2[j] = q[j];
-                        q[j] = nq;
-                    }
-                }
-        }
-    }
+    // Q = Pbar_l^|m| / s^|m| of the lane (lmax <= 7)
+    double q[SO3_QK];
+    so3_legendre_poly(l, am, act, q);
     const double lnorm = act ? a.lnorm[l] : 0.0;
     const double sgn = (m < 0 && (am & 1)) ? -1.0 : 1.0;
     for (int64_t c = blockIdx.x; c < natoms; c += gridDim.x) {
@@ -1138,15 +1125,8 @@ __global__ __launch_bounds__(64) void so3_ctot_lm_kernel(So3Args a, int64_t nato
                     const double ct = stg[0][b];
                     const cd z = cmake(stg[1][b], stg[2][b]);
                     const double wf = stg[3][b] * lnorm;
-                    // Q(t) by Horner's rule
-                    double pl = q[LK - 1];
-#pragma unroll
-                    for (int j = LK - 2; j >= 0; j--) pl = fma(pl, ct, q[j]);
-                    // z^|m|, |m| <= 7
-                    const cd z2 = cmul(z, z), z4 = cmul(z2, z2);
-                    cd em = (am & 1) ? z : cmake(1.0, 0.0);
-                    if (am & 2) em = cmul(em, z2);
-                    if (am & 4) em = cmul(em, z4);
+                    const double pl = so3_legendre_eval(q, ct);
+                    const cd em = so3_zpow(z, am);
                     // Y_l^m w ex fc norm_l, then times I_nl
                     const double yr = pl * em.re * wf * sgn, yi = (m < 0 ? -1.0 : 1.0) * pl * em.im * wf * sgn;
                     const double *ri = rI + (b * L + l) * NMAX;
@@ -1197,9 +1177,9 @@ __device__ inline void wave_sum3_lane63(double &a, double &b, double &c) {
 // Adjoint dE/dr per image, one lane per (l, m) with l <= lmax + 1
 // ((lmax + 2)^2 <= 64): flat over images, a wavefront takes 64 consecutive
 // images (their scalars staged in LDS, one lane each) and runs them one at
-// a time.  Per 16 images, one lane per image builds its Y tables
-// (so3_ylm_tables, rows <= lmax + 1); per image every lane forms its Y_lm
-// from them into a 64-entry LDS table, from which the lanes with l <= lmax read the
+// a time.  Per image every lane forms its Y_lm (rows <= lmax + 1) as
+// Q(cos theta) z^|m| (so3_legendre_poly, built once per lane) into a
+// 64-entry LDS table, from which the lanes with l <= lmax read the
 // Y_{l+-1, m'} of the covariant gradient (SO3.py:649-674; its coefficients
 // depend on (l, m) only and are held in registers).  With
 // alpha = sum_n I_nl A_nlm and beta = sum_n dI_nl A_nlm (A of the image's
@@ -1209,10 +1189,8 @@ __device__ inline void wave_sum3_lane63(double &a, double &b, double &c) {
 // cu = w norm_l ex (dfc - 2 alpha R fc), cv = w norm_l ex fc: the same sums
 // as so3_image_kernel<SO3_ADJOINT> in another order, reduced over the
 // lanes with DPP moves (wave_sum3_lane63).
-constexpr int SO3_ADJ_NST = 10;  // staged: s, t, cos phi, sin phi, R, w ex fc, w ex (dfc - 2 a R fc), rhat
-constexpr int SO3_ADJ_LT = 8;                                   // Y rows l <= 7
-constexpr int SO3_ADJ_NP = SO3_ADJ_LT * (SO3_ADJ_LT + 1) / 2;  // (l, m >= 0)
-constexpr int SO3_ADJ_TB = 16;                                  // images per table batch
+// staged: t, Re z, Im z, (unused), R, w ex fc, w ex (dfc - 2 a R fc), rhat
+constexpr int SO3_ADJ_NST = 10;
 template <int NMAX>
 __global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t nimg, const int64_t *__restrict__ img_atom,
                                                             const int64_t *__restrict__ img_j,
@@ -1224,8 +1202,6 @@ __global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t ni
     __shared__ double stg[SO3_ADJ_NST][64];
     __shared__ int64_t s_at[64];
     __shared__ cd ysh[64];
-    __shared__ double Pt[SO3_ADJ_TB][SO3_ADJ_NP + 1];
-    __shared__ cd Et[SO3_ADJ_TB][SO3_ADJ_LT];
     if (d_count) {
         const int64_t c = *d_count;
         nimg = c < nimg ? c : nimg;
@@ -1242,7 +1218,8 @@ __global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t ni
     const bool ylane = lane < (L + 1) * (L + 1);  // rows 0..lmax+1
     const bool act = lane < L * L;                // rows 0..lmax
     const int am = m < 0 ? -m : m;
-    const int pidx = ylane ? l * (l + 1) / 2 + am : 0;
+    double q[SO3_QK];  // Y rows l <= lmax + 1 <= 7
+    so3_legendre_poly(l, am, ylane, q);
     const double msg = (m < 0 && (am & 1)) ? -1.0 : 1.0, imsg = m < 0 ? -msg : msg;
     // covariant gradient coefficients of the lane's (l, m) and the lanes of
     // the Y values they multiply (-1: term absent)
@@ -1277,11 +1254,10 @@ __global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t ni
         const bool ok = lane < nb;
         So3Img mi;
         load_image(a, ok, e, ok ? img_atom[e] : 0, img_j, img_r, atom_z, mi);
-        const double rho = sqrt(mi.x * mi.x + mi.y * mi.y);
-        stg[0][lane] = rho / mi.R;
-        stg[1][lane] = mi.z / mi.R;
-        stg[2][lane] = rho > 0.0 ? mi.x / rho : 1.0;
-        stg[3][lane] = rho > 0.0 ? mi.y / rho : 0.0;
+        stg[0][lane] = mi.z / mi.R;  // t = cos theta
+        stg[1][lane] = mi.x / mi.R;  // z = sin theta e^{i phi}
+        stg[2][lane] = mi.y / mi.R;
+        stg[3][lane] = 0.0;
         stg[4][lane] = mi.R;
         stg[5][lane] = ok ? mi.w * mi.ex * mi.fc : 0.0;
         stg[6][lane] = ok ? mi.w * mi.ex * (mi.dfc - 2.0 * a.alpha * mi.R * mi.fc) : 0.0;
@@ -1312,18 +1288,9 @@ __global__ __launch_bounds__(64) void so3_deidrj_lm_kernel(So3Args a, int64_t ni
                 }
             }
         }
-        if ((b % SO3_ADJ_TB) == 0) {  // uniform: the next 16 images' tables
-            __syncthreads();          // the previous tables have been read
-            const int bb = b + lane;
-            if (lane < SO3_ADJ_TB && bb < nb)
-                so3_ylm_tables<SO3_ADJ_LT>(a.lmax + 1, stg[0][bb], stg[1][bb], stg[2][bb], stg[3][bb], Pt[lane],
-                                           Et[lane]);
-            __syncthreads();
-        }
         // Y_lm of every lane (rows 0..lmax+1)
-        const int bt = b % SO3_ADJ_TB;
-        const double pl = ylane ? Pt[bt][pidx] : 0.0;
-        const cd em = Et[bt][ylane ? am : 0];
+        const double pl = ylane ? so3_legendre_eval(q, stg[0][b]) : 0.0;
+        const cd em = so3_zpow(cmake(stg[1][b], stg[2][b]), am);
         const cd y = cmake(msg * pl * em.re, imsg * pl * em.im);
         ysh[lane] = y;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
