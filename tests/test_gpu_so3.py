@@ -116,10 +116,14 @@ def _adjoint_forces(nl, state, model, species):
     return e.detach(), -g
 
 
-def test_so3_adjoint_matches_plain():
+@pytest.mark.parametrize("nmax,lmax", [(4, 4), (2, 0), (3, 1), (5, 6)])
+def test_so3_adjoint_matches_plain(nmax, lmax):
+    """Adjoint energies and forces (per-lane Y rows to lmax + 1, lanes past
+    them idle) against the plain dP/dr path."""
     rng = np.random.default_rng(4)
     for s in [workloads.diamond_si(reps=2, sigma=0.08, seed=1), workloads.random_sio_structure(rng, 8, 30)]:
-        nl, cfg, z, model, species, st = _so3_engine(s)
+        print("so3 adjoint vs plain", nmax, lmax, len(s), flush=True)
+        nl, cfg, z, model, species, st = _so3_engine(s, nmax=nmax, lmax=lmax)
         ea, fa = _adjoint_forces(nl, st, model, species)
         ep, fp = _plain_forces(nl, cfg, z, model, species)
         assert abs(float(ea - ep)) <= 1e-12 * abs(float(ep))
